@@ -1,0 +1,127 @@
+"""Benchmark of the hologram-generation hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+
+Prints ONE JSON line.  A step is one hologram of the workload, inputs resident
+in HBM before the timed region (device pointers, OPH_ASYNC launches), bracketed
+by stream synchronisation; `value` is holograms per second.  The dominant
+kernel's average launch duration is measured live with HIP events on the
+library's stream (oph_profile_*) and reported as a roofline fraction.  The CPU
+baseline times the oracle (FP64 restatement of the reference CPU path) on a
+bounded sample of the same workload on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "holograms/sec @4096² at rel-L2≤1e-3 vs FP64; point·pixel/s; HBM GB/s"
+HBM_PEAK_GBS = 8000.0
+VALU_LANE_OPS = 256 * 64 * 2.4e9          # unpacked FP32 lane-ops/s (3.93e13)
+RS_SLOTS_PER_PAIR = 21                     # issue-slot model of one exact RS pair
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.complex128).ravel()
+    b = np.asarray(b, np.complex128).ravel()
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def host_cores():
+    try:
+        return len(os.sched_getaffinity(0))
+    except Exception:
+        return os.cpu_count() or 1
+
+
+def bench_rs(ctx, args):
+    """Plain RS point-cloud field: 1000 points, 1024^2, complex128 out."""
+    import openhologram_amd as oph
+    from openhologram_amd import scenes
+    x, y, z, a, ph = scenes.rs_points(1000, seed=1)
+    spec = dict(scenes.RS_SPEC)
+    d = [ctx.to_device(v) for v in (x, y, z, a, ph)]
+    out = ctx.empty((spec["ny"], spec["nx"]), np.complex128)
+    for _ in range(args.warmup):
+        ctx.pc_rs(spec, *d, out=out, device=True, asynchronous=True)
+    ctx.sync()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        ctx.pc_rs(spec, *d, out=out, device=True, asynchronous=True)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    n, kms = ctx.profile_get("rs_accumulate")
+    ctx.profile(False)
+    pairs = 1000 * spec["nx"] * spec["ny"]
+    k_avg = kms / max(n, 1) / 1e3
+    # accuracy of this very output against the FP64 oracle
+    from oracle import oracle as O
+    err = rel_l2(out.numpy(), O.rs(spec, x, y, z, a, ph))
+    peak_pairs = VALU_LANE_OPS / RS_SLOTS_PER_PAIR
+    line = {
+        "metric": METRIC, "value": args.steps / wall, "unit": "holograms/s",
+        "ms_per_step": wall / args.steps * 1e3, "dtype": "f64-phase/f32-acc",
+        "config": {"workload": "rs_pointcloud_1000pts_1024x1024_8um_532nm"},
+        "rel_l2_vs_fp64": err,
+        "point_pixel_per_s": pairs / (wall / args.steps),
+        "roofline": {"bound": "valu", "achieved": pairs / k_avg / 1e12, "peak": peak_pairs / 1e12,
+                     "unit": "Tpair/s", "frac": pairs / k_avg / peak_pairs, "traffic": None},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_rs(spec)
+    return line
+
+
+def cpu_rs(spec):
+    from oracle import oracle as O
+    from openhologram_amd import scenes
+    x, y, z, a, ph = scenes.rs_points(1000, seed=1)
+    rows = 64
+    s = dict(spec, ny=rows)
+    cores = host_cores()
+    t0 = time.perf_counter()
+    O.rs(s, x, y, z, a, ph, nthreads=cores)
+    dt = time.perf_counter() - t0
+    pairs = 1000 * spec["nx"] * rows
+    full = 1000 * spec["nx"] * spec["ny"]
+    return {"value": (pairs / dt) / full, "unit": "holograms/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle, 1000 points x {rows} of {spec['ny']} rows "
+                      f"({pairs:.3g} point-pixel pairs, {dt:.2f} s), scaled by pair count"}
+
+
+WORKLOADS = {"rs": bench_rs}
+DEFAULT = "rs"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the hologram "
+              "paths here are single-GPU; multi-GPU sharding is a design note, not a run.",
+              file=sys.stderr)
+        sys.exit(2)
+    import openhologram_amd as oph
+    with oph.Context(0) as ctx:
+        line = WORKLOADS[args.workload](ctx, args)
+    base = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "data": "synthetic (seeded SplitMix64)"}
+    base.update(line)
+    print(json.dumps(base), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,138 @@
+/*
+ * oph_hip.h -- C ABI of the MI355X (gfx950) hologram-generation library
+ * liboph_hip.so.
+ *
+ * Plain C types only: pointers, sizes, POD structs.  Every entry point returns
+ * OPH_OK (0) or a negative OPH_E* code and never exits the process; the text
+ * of the last error of a context is returned by oph_last_error().
+ *
+ * Memory: array arguments are host pointers unless the call's flags contain
+ * OPH_DEVICE_PTR, in which case every array argument of that call is a device
+ * pointer (from oph_device_alloc or any HIP allocation on the context's
+ * device).  The context owns its device scratch and one HIP stream; calls are
+ * synchronous unless OPH_ASYNC is given (device-pointer calls only), after
+ * which oph_sync() waits.  A context is used by one host thread at a time;
+ * several contexts may run concurrently.  Results are deterministic for a
+ * fixed input (no floating-point atomics on any path).
+ *
+ * Each generator cites the reference interface it replaces (paths relative to
+ * the reference tree: K/ = OpenHolo_KETI/VS/Openholo_Generation_PointCloud_
+ * Library_beta_v1.0, DM/ = OpenHolo_DepthMap, W/ = OpenHolo_WRP/openwrp_
+ * console_ver 1.0, CWO/ = W/cwo/src).
+ */
+#ifndef OPH_HIP_H
+#define OPH_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define OPH_OK        0
+#define OPH_EINVAL   -1  /* bad argument, size or spec */
+#define OPH_ENOMEM   -2  /* device or host allocation failed */
+#define OPH_EDEVICE  -3  /* HIP runtime error */
+#define OPH_EIO      -4  /* file missing or malformed */
+#define OPH_ENUMERIC -5  /* non-finite input or output */
+#define OPH_ENOTSUP  -6  /* combination not supported */
+
+/* ---- flags ---- */
+#define OPH_DEVICE_PTR  0x0001 /* all array arguments are device pointers */
+#define OPH_ACCUMULATE  0x0002 /* add into the output instead of overwriting */
+#define OPH_ASYNC       0x0004 /* do not wait for the stream (device pointers) */
+/* output element format of the point-cloud generators (one of) */
+#define OPH_OUT_C128    0x0000 /* complex128, {re, im} doubles */
+#define OPH_OUT_C64     0x0010 /* complex64, {re, im} floats */
+#define OPH_OUT_FRINGE  0x0020 /* float Re(U): the KETI real fringe */
+#define OPH_OUT_MASK    0x0030
+
+typedef struct oph_ctx oph_ctx;
+typedef struct { double re, im; } oph_c128;
+typedef struct { float re, im; } oph_c64;
+
+/* ---- context ---- */
+int oph_create(int device, oph_ctx** out);
+void oph_destroy(oph_ctx* ctx);
+const char* oph_last_error(const oph_ctx* ctx);
+int oph_sync(oph_ctx* ctx);
+/* Version string of the library ("oph_hip <major>.<minor> gfx950"). */
+const char* oph_version(void);
+
+/* ---- device memory and timing helpers (for callers that keep data
+ * resident in HBM) ---- */
+int oph_device_alloc(oph_ctx* ctx, size_t bytes, void** dptr);
+int oph_device_free(oph_ctx* ctx, void* dptr);
+int oph_copy_to_device(oph_ctx* ctx, void* dst, const void* src, size_t bytes);
+int oph_copy_to_host(oph_ctx* ctx, void* dst, const void* src, size_t bytes);
+int oph_device_zero(oph_ctx* ctx, void* dptr, size_t bytes);
+/* HIP events recorded on the context's stream around whatever the caller
+ * launches between begin and end; end waits and returns milliseconds. */
+int oph_timer_begin(oph_ctx* ctx);
+int oph_timer_end(oph_ctx* ctx, float* ms);
+
+/* Per-kernel timing: when enabled, every kernel the library launches on the
+ * context's stream is bracketed by a HIP event pair (enable clears previous
+ * records).  oph_profile_get sums the recorded durations of one kernel name
+ * (e.g. "rs_accumulate", "fft_rows") in milliseconds. */
+int oph_profile_enable(oph_ctx* ctx, int on);
+int oph_profile_get(oph_ctx* ctx, const char* kernel, int* count, double* total_ms);
+
+/* ---- hologram spec (replaces KETI OphSpec, K/ophgen_pointcloud/
+ * ophgen_pointcloud.h:45-70, in double: a float wavelength or depth alone
+ * moves k r by ~0.06-0.09 rad at 0.1 m) ---- */
+typedef struct oph_spec {
+    int nx, ny;            /* SLM pixels */
+    double pitch_x, pitch_y;
+    double lambda;         /* metres */
+    double scale[3];       /* point-cloud scale factors x, y, z */
+    double offset_z;       /* added to the scaled z */
+    double tilt_deg[2];    /* reference-wave tilt, degrees */
+} oph_spec;
+
+/* Point cloud, structure of arrays, n points.  amp and phase may be NULL
+ * (amplitude 1, phase 0).  phase is in radians. */
+typedef struct oph_points {
+    const double* x;
+    const double* y;
+    const double* z;
+    const double* amp;
+    const double* phase;
+    int64_t n;
+} oph_points;
+
+/* Parses a 17-line KETI spec file (`key = value`, keys ignored, exactly 17
+ * lines) -- replaces ophLoadSpecConfig, K/ophgen_pointcloud/
+ * ophgen_pointcloud.cpp:112-158.  Wavelength, pitches, scales and offset are
+ * parsed directly to double.  OPH_EIO if the file is missing or has != 17
+ * lines. */
+int oph_spec_from_keti_config(const char* path, oph_spec* spec);
+
+/* Loads a KETI .dat point cloud (first line P, then `idx x y z phase amp`
+ * per line, idx == line index) -- replaces ophLoadPointCloud,
+ * K/ophgen_pointcloud/ophgen_pointcloud.cpp:13-109.  With xyzpa == NULL only
+ * *n is set; otherwise up to cap points are written as 5 doubles each
+ * (x, y, z, phase, amplitude: the file's column order).  OPH_EIO on a missing
+ * file or an index mismatch. */
+int oph_load_keti_points(const char* path, double* xyzpa, int64_t cap, int64_t* n);
+
+/* ---- point cloud: exact Rayleigh-Sommerfeld field ----
+ * Replaces ophGenCghPointCloud (K/ophgen_pointcloud/ophgen_pointcloud.cpp:
+ * 195-365) and gpu::ophGenCghPointCloud_cuda (K/ophgen_pointcloud_gpu/
+ * ophgen_pointcloud_gpu.cpp:391-503).  For every pixel (row, col):
+ *   X = (col+1/2) px - nx px/2,   Y = ny py/2 - (row+1/2) py
+ *   U = sum_n a_n exp(i (k r_n + phi_n)) exp(-i k (X sin tx + Y sin ty))
+ *   r_n = sqrt((X - sx x_n)^2 + (Y - sy y_n)^2 + (sz z_n + offset_z)^2)
+ * out is ny*nx elements, row-major, in the format given by OPH_OUT_*;
+ * OPH_OUT_FRINGE writes Re(U) as float, the reference's fringe (which ignores
+ * the point phase: pass phase = NULL for parity).  OPH_ACCUMULATE adds into
+ * out as the reference's `dst +=`. */
+int oph_pc_rs(oph_ctx* ctx, const oph_spec* spec, const oph_points* pts,
+              void* out, int flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OPH_HIP_H */

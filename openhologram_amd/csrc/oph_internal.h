@@ -1,0 +1,134 @@
+// oph_internal.h -- context, error plumbing and device helpers shared by the
+// HIP translation units of liboph_hip.so.  gfx950 (CDNA4) only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "oph_hip.h"
+
+// ---------------------------------------------------------------------------
+// context
+
+enum ScratchSlot {
+    SLOT_PTS = 0,   // staged point arrays
+    SLOT_OUT,       // staged output
+    SLOT_IN,        // staged input
+    SLOT_WORK0,     // FFT / pipeline work buffers
+    SLOT_WORK1,
+    SLOT_WORK2,
+    SLOT_TABLE,     // small tables (twiddles, transfer phases)
+    SLOT_TABLE2,
+    SLOT_AUX0,
+    SLOT_AUX1,
+    SLOT_COUNT
+};
+
+struct LaunchRecord {
+    const char* name;
+    hipEvent_t a, b;
+};
+
+struct oph_ctx {
+    int device = 0;
+    // per-kernel timing with HIP events on this context's stream
+    // (oph_profile_enable); events are pooled and reused
+    int profiling = 0;
+    std::vector<LaunchRecord> launches;
+    std::vector<hipEvent_t> event_pool;
+    size_t pool_used = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+    void* slot_ptr[SLOT_COUNT] = {};
+    size_t slot_size[SLOT_COUNT] = {};
+    std::string err;
+    int cu_count = 256;
+};
+
+// Records an error message on the context and returns code.
+int oph_fail(oph_ctx* ctx, int code, const char* fmt, ...);
+
+// Grow-only device buffer owned by the context.  Returns nullptr (with the
+// error recorded) when allocation fails.
+void* oph_scratch(oph_ctx* ctx, int slot, size_t bytes);
+
+#define OPH_HIP_CHECK(ctx, expr)                                                  \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return oph_fail((ctx), OPH_EDEVICE, "%s: %s (%s:%d)", #expr,          \
+                            hipGetErrorString(e_), __FILE__, __LINE__);           \
+    } while (0)
+
+#define OPH_CHECK_LAUNCH(ctx) OPH_HIP_CHECK(ctx, hipGetLastError())
+
+// Event pair around one kernel launch when profiling is on.
+hipEvent_t oph_prof_event(oph_ctx* ctx);
+#define OPH_LAUNCH(ctx, name, kernel, grid, block, shmem, ...)                      \
+    do {                                                                          \
+        hipEvent_t pa_ = nullptr, pb_ = nullptr;                                  \
+        if ((ctx)->profiling) {                                                   \
+            pa_ = oph_prof_event(ctx);                                            \
+            pb_ = oph_prof_event(ctx);                                            \
+            if (pa_) hipEventRecord(pa_, (ctx)->stream);                          \
+        }                                                                         \
+        hipLaunchKernelGGL(kernel, grid, block, shmem, (ctx)->stream, __VA_ARGS__); \
+        if (pb_) {                                                                \
+            hipEventRecord(pb_, (ctx)->stream);                                   \
+            (ctx)->launches.push_back(LaunchRecord{name, pa_, pb_});              \
+        }                                                                         \
+        OPH_CHECK_LAUNCH(ctx);                                                    \
+    } while (0)
+
+// Makes the context's device current.
+int oph_enter(oph_ctx* ctx);
+// Waits on the stream unless OPH_ASYNC; reports async errors.
+int oph_finish(oph_ctx* ctx, int flags);
+
+// Staging of one array argument: returns a device pointer holding `bytes`
+// of `host` (copied when `host` is a host pointer) or `host` itself when the
+// call passed device pointers.
+const void* oph_stage_in(oph_ctx* ctx, int slot, const void* p, size_t bytes, int flags,
+                         size_t offset = 0);
+
+// ---------------------------------------------------------------------------
+// device helpers
+
+// sin/cos of 2*pi*t for t in turns, |t| <= 256: v_sin_f32 / v_cos_f32 take
+// their argument in turns.
+__device__ __forceinline__ float sin_turns(float t) { return __builtin_amdgcn_sinf(t); }
+__device__ __forceinline__ float cos_turns(float t) { return __builtin_amdgcn_cosf(t); }
+
+// Fractional part of a double, in [-0.5, 0.5].
+__device__ __forceinline__ double frac_turns(double t) { return t - rint(t); }
+
+// 1/sqrt(s) in double: hardware estimate refined by two Newton steps.
+__device__ __forceinline__ double rsqrt_f64(double s) {
+    double y = __builtin_amdgcn_rsq(s);
+    double h = 0.5 * s;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
+struct f2 { float x, y; };
+
+// Workgroup index remap for the 8 XCDs of MI355X: consecutive hardware ids go
+// round-robin over XCDs; this returns a logical id such that each XCD receives
+// a contiguous range of logical ids (neighbouring tiles then share that XCD's
+// L2).  n = number of workgroups in the grid.
+__device__ __forceinline__ int xcd_remap(int hw, int n) {
+    const int nx = 8;
+    const int per = n / nx;              // full ranges
+    const int rem = n - per * nx;
+    const int xcd = hw % nx, slot = hw / nx;
+    if (slot >= per) return per * nx + xcd;  // tail (n not a multiple of 8)
+    (void)rem;
+    return xcd * per + slot;
+}

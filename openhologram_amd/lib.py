@@ -1,0 +1,250 @@
+"""ctypes binding of liboph_hip.so (the C ABI declared in include/oph_hip.h).
+
+The shared library is loaded from this package directory only; if it is
+missing or does not load, every call raises -- there is no CPU fallback.
+"""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboph_hip.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "oph_hip.h")
+
+OPH_OK, OPH_EINVAL, OPH_ENOMEM, OPH_EDEVICE, OPH_EIO, OPH_ENUMERIC, OPH_ENOTSUP = 0, -1, -2, -3, -4, -5, -6
+DEVICE_PTR, ACCUMULATE, ASYNC = 0x1, 0x2, 0x4
+OUT_C128, OUT_C64, OUT_FRINGE = 0x00, 0x10, 0x20
+
+_ERRNAMES = {-1: "OPH_EINVAL", -2: "OPH_ENOMEM", -3: "OPH_EDEVICE", -4: "OPH_EIO",
+             -5: "OPH_ENUMERIC", -6: "OPH_ENOTSUP"}
+
+
+class OphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{_ERRNAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Spec(C.Structure):
+    """oph_spec"""
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("pitch_x", C.c_double), ("pitch_y", C.c_double),
+                ("lambda_", C.c_double), ("scale", C.c_double * 3), ("offset_z", C.c_double),
+                ("tilt_deg", C.c_double * 2)]
+
+
+class Points(C.Structure):
+    """oph_points"""
+    _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("amp", C.c_void_p),
+                ("phase", C.c_void_p), ("n", C.c_int64)]
+
+
+_lib = None
+
+
+def load():
+    """Loads liboph_hip.so; raises if it is absent (build with
+    `python -m openhologram_amd.build`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m openhologram_amd.build`")
+    lib = C.CDLL(LIB_PATH)
+    P, I, D, I64, SZ = C.c_void_p, C.c_int, C.c_double, C.c_int64, C.c_size_t
+    sig = {
+        "oph_create": (I, [I, C.POINTER(P)]),
+        "oph_destroy": (None, [P]),
+        "oph_last_error": (C.c_char_p, [P]),
+        "oph_sync": (I, [P]),
+        "oph_version": (C.c_char_p, []),
+        "oph_device_alloc": (I, [P, SZ, C.POINTER(P)]),
+        "oph_device_free": (I, [P, P]),
+        "oph_copy_to_device": (I, [P, P, P, SZ]),
+        "oph_copy_to_host": (I, [P, P, P, SZ]),
+        "oph_device_zero": (I, [P, P, SZ]),
+        "oph_timer_begin": (I, [P]),
+        "oph_timer_end": (I, [P, C.POINTER(C.c_float)]),
+        "oph_profile_enable": (I, [P, I]),
+        "oph_profile_get": (I, [P, C.c_char_p, C.POINTER(I), C.POINTER(D)]),
+        "oph_spec_from_keti_config": (I, [C.c_char_p, C.POINTER(Spec)]),
+        "oph_load_keti_points": (I, [C.c_char_p, P, I64, C.POINTER(I64)]),
+        "oph_pc_rs": (I, [P, C.POINTER(Spec), C.POINTER(Points), P, I]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def header_functions(path=HEADER):
+    """Names of the functions declared in a C header of include/."""
+    src = open(path).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(oph_\w+)\s*\(", src, flags=re.M)))
+
+
+def ptr(a):
+    if a is None:
+        return None
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    if isinstance(a, int):
+        return a
+    return a.ctypes.data_as(C.c_void_p).value
+
+
+class Context:
+    """One oph_ctx: a device, a HIP stream and its scratch."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.oph_create(device, C.byref(h))
+        if rc != OPH_OK:
+            raise OphError(rc, f"oph_create(device={device}) failed")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.oph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def check(self, rc):
+        if rc != OPH_OK:
+            raise OphError(rc, self.lib.oph_last_error(self.h).decode())
+        return rc
+
+    # --- device memory ---
+    def empty(self, shape, dtype):
+        return DeviceArray(self, shape, dtype)
+
+    def to_device(self, a):
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(self, a.shape, a.dtype)
+        self.check(self.lib.oph_copy_to_device(self.h, d.ptr, ptr(a), a.nbytes))
+        return d
+
+    def sync(self):
+        self.check(self.lib.oph_sync(self.h))
+
+    def timer_begin(self):
+        self.check(self.lib.oph_timer_begin(self.h))
+
+    def timer_end(self):
+        ms = C.c_float()
+        self.check(self.lib.oph_timer_end(self.h, C.byref(ms)))
+        return ms.value
+
+    def profile(self, on=True):
+        self.check(self.lib.oph_profile_enable(self.h, int(on)))
+
+    def profile_get(self, kernel):
+        """(launch count, total ms) of one kernel since profile(True)."""
+        n, t = C.c_int(), C.c_double()
+        self.check(self.lib.oph_profile_get(self.h, kernel.encode(), C.byref(n), C.byref(t)))
+        return n.value, t.value
+
+    # --- generators ---
+    def pc_rs(self, spec, x, y, z, amp=None, phase=None, out=None, fmt=OUT_C128, accumulate=False,
+              device=False, asynchronous=False):
+        """Exact RS field (oph_pc_rs).  Host numpy in / out unless device=True
+        (then x..phase and out are DeviceArray)."""
+        s = spec if isinstance(spec, Spec) else make_spec(**spec)
+        pts, keep = _points(x, y, z, amp, phase, device)
+        if out is None:
+            dt = {OUT_C128: np.complex128, OUT_C64: np.complex64, OUT_FRINGE: np.float32}[fmt]
+            out = self.empty((s.ny, s.nx), dt) if device else np.zeros((s.ny, s.nx), dt)
+        flags = fmt | (ACCUMULATE if accumulate else 0) | (DEVICE_PTR if device else 0) | \
+            (ASYNC if asynchronous else 0)
+        self.check(self.lib.oph_pc_rs(self.h, C.byref(s), C.byref(pts), ptr(out), flags))
+        del keep
+        return out
+
+
+class DeviceArray:
+    """A device allocation owned by a Context (freed on deletion)."""
+
+    def __init__(self, ctx, shape, dtype):
+        self.ctx = ctx
+        self.shape = tuple(shape) if hasattr(shape, "__len__") else (shape,)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        ctx.check(ctx.lib.oph_device_alloc(ctx.h, max(self.nbytes, 1), C.byref(p)))
+        self.ptr = p.value
+
+    def numpy(self):
+        a = np.empty(self.shape, self.dtype)
+        self.ctx.check(self.ctx.lib.oph_copy_to_host(self.ctx.h, ptr(a), self.ptr, self.nbytes))
+        return a
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, self.dtype)
+        assert a.nbytes == self.nbytes
+        self.ctx.check(self.ctx.lib.oph_copy_to_device(self.ctx.h, self.ptr, ptr(a), a.nbytes))
+
+    def zero(self):
+        self.ctx.check(self.ctx.lib.oph_device_zero(self.ctx.h, self.ptr, self.nbytes))
+
+    def __del__(self):
+        try:
+            if self.ptr and self.ctx.h:
+                self.ctx.lib.oph_device_free(self.ctx.h, self.ptr)
+        except Exception:
+            pass
+        self.ptr = None
+
+
+def make_spec(nx, ny, pitch, lambda_, scale=(1.0, 1.0, 1.0), offset_z=0.0, tilt_deg=(0.0, 0.0)):
+    if not hasattr(pitch, "__len__"):
+        pitch = (pitch, pitch)
+    return Spec(nx, ny, pitch[0], pitch[1], lambda_, (C.c_double * 3)(*scale), offset_z,
+                (C.c_double * 2)(*tilt_deg))
+
+
+def _points(x, y, z, amp, phase, device):
+    if device:
+        arrs = [x, y, z, amp, phase]
+        n = x.shape[0]
+        return Points(*[ptr(a) for a in arrs], n), arrs
+    arrs = [None if a is None else np.ascontiguousarray(a, np.float64) for a in (x, y, z, amp, phase)]
+    n = len(arrs[0])
+    return Points(*[ptr(a) for a in arrs], n), arrs
+
+
+def spec_from_keti_config(path):
+    s = Spec()
+    rc = load().oph_spec_from_keti_config(path.encode(), C.byref(s))
+    if rc != OPH_OK:
+        raise OphError(rc, f"cannot parse {path}")
+    return s
+
+
+def load_keti_points(path):
+    """Returns (n, 5) float64: x, y, z, phase, amplitude (file column order)."""
+    lib = load()
+    n = C.c_int64()
+    rc = lib.oph_load_keti_points(path.encode(), None, 0, C.byref(n))
+    if rc != OPH_OK:
+        raise OphError(rc, f"cannot load {path}")
+    out = np.zeros((n.value, 5), np.float64)
+    rc = lib.oph_load_keti_points(path.encode(), ptr(out), n.value, C.byref(n))
+    if rc != OPH_OK:
+        raise OphError(rc, f"cannot load {path}")
+    return out
