@@ -96,8 +96,102 @@ def cpu_rs(spec):
                       f"({pairs:.3g} point-pixel pairs, {dt:.2f} s), scaled by pair count"}
 
 
-WORKLOADS = {"rs": bench_rs}
-DEFAULT = "rs"
+def _time_steps(ctx, step, args, kernels):
+    """Warm-up, then K timed steps with per-kernel HIP-event timing."""
+    for _ in range(args.warmup):
+        step()
+    ctx.sync()
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    per = {}
+    for k in kernels:
+        n, ms = ctx.profile_get(k)
+        if n:
+            per[k] = {"launches_per_step": n / args.steps, "avg_us": ms / n * 1e3}
+    ctx.profile(False)
+    # wall time of the same K steps without event pairs around every kernel
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    ctx.sync()
+    wall_noprof = time.perf_counter() - t0
+    return min(wall, wall_noprof), per
+
+
+WRP_KERNELS = ["wrp_zero", "wrp_prep", "wrp_scan", "wrp_fill", "wrp_gather", "asm_axis_table",
+               "asm_rows_fwd", "asm_cols", "asm_rows_inv"]
+
+
+def wrp_bytes(n_side, pad):
+    """Algorithmic HBM bytes per launch of the three ASM passes (complex64)."""
+    npx = n_side * pad
+    field = n_side * n_side * 8
+    half = npx * n_side * 8          # the ny rows of the padded grid that carry data
+    return {"asm_rows_fwd": field + half, "asm_cols": 2 * half, "asm_rows_inv": half + field}
+
+
+def bench_wrp(ctx, args):
+    """WRP hologram: 100,000 points -> 2048^2 WRP -> 100 mm ASM on 4096^2."""
+    from openhologram_amd import scenes
+    n_side, pad, pitch, lam = 2048, 2, 8e-6, 532e-9
+    x, y, z, a, zw = scenes.wrp_points(100_000, seed=2)
+    dist = -zw
+    d = [ctx.to_device(v) for v in (x, y, z, a)]
+    out = ctx.empty((n_side, n_side), np.complex64)
+
+    def step():
+        ctx.wrp_hologram(n_side, pitch, lam, zw, dist, *d, pad=pad, out=out, device=True,
+                         asynchronous=True)
+    wall, per = _time_steps(ctx, step, args, WRP_KERNELS)
+    U = out.numpy()
+    line = {"value": args.steps / wall, "unit": "holograms/s", "ms_per_step": wall / args.steps * 1e3,
+            "dtype": "f32 (FP64 phase reduction)",
+            "config": {"workload": "wrp_100k_points_2048sq_wrp_100mm_asm_4096sq_pad2_8um_532nm"},
+            "kernels": per}
+    byt = wrp_bytes(n_side, pad)
+    # dominant kernel among the HBM-bound ASM passes
+    dom = max((k for k in byt if k in per), key=lambda k: per[k]["avg_us"])
+    t = per[dom]["avg_us"] * 1e-6
+    gbs = byt[dom] / t / 1e9
+    line["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
+                        "algorithmic_bytes": byt[dom]}
+    pairs = 0
+    line["hbm_gbs_asm_total"] = sum(byt.values()) / (sum(per[k]["avg_us"] for k in byt if k in per) * 1e-6) / 1e9
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_wrp(x, y, z, a, zw, n_side, pitch, lam, dist, pad)
+    if args.check:
+        from oracle import oracle as O
+        w = O.wrp(n_side, n_side, pitch, lam, zw, x, y, z, a, nthreads=host_cores())
+        ref = O.asm(w, pitch, pitch, lam, dist, pad=pad, nthreads=host_cores())
+        line["rel_l2_vs_fp64"] = rel_l2(U, ref)
+    del pairs
+    return line
+
+
+def cpu_wrp(x, y, z, a, zw, n_side, pitch, lam, dist, pad):
+    """FP64 oracle of the same WRP hologram on a sample: 1/8 of the points for
+    the WRP stage (scaled by point count) plus one full padded ASM."""
+    from oracle import oracle as O
+    cores = host_cores()
+    m = len(x) // 8
+    t0 = time.perf_counter()
+    w = O.wrp(n_side, n_side, pitch, lam, zw, x[:m], y[:m], z[:m], a[:m], nthreads=cores)
+    t_wrp = (time.perf_counter() - t0) * (len(x) / m)
+    t0 = time.perf_counter()
+    O.asm(w, pitch, pitch, lam, dist, pad=pad, nthreads=cores)
+    t_asm = time.perf_counter() - t0
+    return {"value": 1.0 / (t_wrp + t_asm), "unit": "holograms/s", "cores": cores, "kind": "port",
+            "sample": f"FP64 oracle: WRP of {m} of {len(x)} points ({t_wrp * m / len(x):.2f} s, scaled "
+                      f"by point count) + one full {n_side * pad}^2 padded ASM ({t_asm:.2f} s)"}
+
+
+WORKLOADS = {"rs": bench_rs, "wrp": bench_wrp}
+DEFAULT = "wrp"
 
 
 def main():
@@ -107,6 +201,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--check", action="store_true", help="also report rel-L2 of the output vs the FP64 oracle")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the hologram "

@@ -132,6 +132,54 @@ int oph_load_keti_points(const char* path, double* xyzpa, int64_t cap, int64_t* 
 int oph_pc_rs(oph_ctx* ctx, const oph_spec* spec, const oph_points* pts,
               void* out, int flags);
 
+/* ---- 2-D FFT, complex64, in place ----
+ * Replaces the FFTW / cuFFT Z2Z transform of the depth-map generator's
+ * fftwShift / cudaFFT (DM/Hologram/src/HologramGenerator_CPU.cpp:285-322,
+ * DM/Hologram/src/HologramKernel.cu:189-225) without the shifts.
+ * data is ny rows of nx; nx, ny powers of two in 16..8192.
+ * direction -1: X[k] = sum_n x[n] exp(-2 pi i k n / N) (forward),
+ * direction +1: the unnormalised inverse; normalize != 0 divides by nx*ny. */
+int oph_fft2d(oph_ctx* ctx, oph_c64* data, int nx, int ny, int direction,
+              int normalize, int flags);
+
+/* ---- angular-spectrum propagation, complex64, in place ----
+ * Replaces CWO::Diffract with CWO_ANGULAR (CWO/cwo.cpp:2198-2278,
+ * 2907-2936; transfer CWO/cwo_lib.h:66-101): the ny x nx field is zero-padded
+ * to (pad*ny) x (pad*nx) at offset (pad*nx/2 - nx/2, pad*ny/2 - ny/2),
+ * transformed, multiplied by H = exp(i 2 pi z sqrt(1/lambda^2 - fx^2 - fy^2))
+ * (0 for evanescent frequencies; fx = kx/(pad nx px) in FFT order),
+ * inverse-transformed with 1/(Npx Npy), and cropped back.  band_limit != 0
+ * applies the Matsushima band limit of OpenHolo_LFCGH/
+ * fn_FresnelPropagation_as.m:33-38.  pad is 1 or 2; pad*nx and pad*ny must
+ * be powers of two in 16..8192. */
+int oph_asm_propagate(oph_ctx* ctx, oph_c64* field, int nx, int ny, double px,
+                      double py, double lambda, double z, int pad, int band_limit,
+                      int flags);
+
+/* ---- wavefront-recording plane ----
+ * Replaces OPHWRP::SingleWRP (W/openwrp/WRP.cpp:75-142 with CWO::AddPixel,
+ * CWO/cwo.cpp:4298-4309).  Coordinates are along the propagation axis; for
+ * each point, dz = z_wrp - z_n, centre tx = (int)(x/p) + nx/2,
+ * ty = (int)(y/p) + ny/2, and over the window wx, wy in [-w, w):
+ *   wrp[ty+wy][tx+wx] += a exp(i k r) / (r + 0.05),
+ *   r = sign(dz) sqrt((wx p)^2 + (wy p)^2 + dz^2)
+ * (pixels outside the plane skipped).  mode 0: grating-angle half-width
+ * w = (int)(|dz| tan(asin(lambda/(2p)))/p) (CWO/cwoPLS.cpp:613); mode 1:
+ * the reference's w = (int)|lambda dz/(2 p^2) + 0.5| * 2 - 1 (WRP.cpp:111).
+ * pts->phase is ignored (the reference points carry none).  wrp is ny x nx
+ * complex64, row index = y.  OPH_ACCUMULATE adds into wrp. */
+int oph_wrp(oph_ctx* ctx, int nx, int ny, double pitch, double lambda,
+            double z_wrp, int mode, const oph_points* pts, oph_c64* wrp, int flags);
+
+/* ---- WRP hologram: oph_wrp on an n_side^2 plane, then oph_asm_propagate
+ * over `distance` with zero padding `pad` (replaces SingleWRP followed by
+ * OPHWRP::Fresnelpropagation, W/openwrp/WRP.cpp:65-69, whose CWO Fresnel
+ * convolution kernels are closed; the angular-spectrum transfer is used).
+ * out: n_side^2 complex64 hologram. */
+int oph_wrp_hologram(oph_ctx* ctx, int n_side, double pitch, double lambda,
+                     double z_wrp, double distance, int pad, int band_limit,
+                     int mode, const oph_points* pts, oph_c64* out, int flags);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,293 @@
+// asm.hip -- angular-spectrum propagation with zero padding, fused into three
+// FFT passes on gfx950.
+//
+// Follows the plumbing of CWO::Diffract (CWO/cwo.cpp:2198-2278: pad N -> pad*N
+// centred at (Np/2 - N/2), transform, crop the same window) with the
+// CWO_ANGULAR transfer (CWO/cwo_lib.h:66-101) and the open statement of the
+// band-limited form (OpenHolo_LFCGH/fn_FresnelPropagation_as.m:17-44):
+//   H(fx, fy) = exp(i 2 pi z sqrt(1/lambda^2 - fx^2 - fy^2)), 0 if evanescent
+// on FFT-order grids fx = kx/(Npx px), kx in [-Npx/2, Npx/2).
+//
+// Passes (work buffer Npy x Npx complex64 in HBM):
+//   1. rows:    each input row is embedded at column offset ox in an LDS line,
+//               transformed, and written to work row oy + r (the other rows
+//               of the work buffer are known to be zero and never touched);
+//   2. columns: B columns of work rows [oy, oy+ny) are staged in LDS, the
+//               column FFT, the transfer multiply and the inverse column FFT
+//               run back to back in LDS, and only rows [oy, oy+ny) -- the
+//               crop window -- are written back;
+//   3. rows:    inverse row transform of work rows [oy, oy+ny), scaled by
+//               1/(Npx Npy), columns [ox, ox+nx) written to the output.
+// The transfer phase in turns is split as
+//   z/lambda * sqrt(1 - u) = z/lambda - z lambda (fx^2+fy^2)/2
+//                           - z/lambda (u^2/8 + u^3/16 + 5u^4/128 + 7u^5/256 + ...)
+// with u = lambda^2 (fx^2 + fy^2): the constant and the per-axis Fresnel terms
+// frac(-z lambda fx^2/2) are formed in FP64 (tables per axis, ~1e5 turns of
+// range), the small remainder in FP32.  When the truncated series would be
+// off by more than 1e-7 turns at the grid corner the kernel evaluates the
+// square root per element in FP64 instead.
+#include <cmath>
+
+#include "fft_engine.h"
+#include "fft_kernels.h"
+#include "pipelines.h"
+
+namespace {
+
+// per-axis table entry: x = frac(-z lambda f^2 / 2) (turns), y = lambda^2 f^2,
+// z = (|f| z lambda / (Np p / 2))^2 for the band limit, w = f (cycles/m)
+__global__ void asm_axis_table(float4* t, int n, double pitch, double lambda, double z) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int kk = k < n / 2 ? k : k - n;
+    const double f = kk / (n * pitch);
+    double tt = -z * lambda * f * f * 0.5;
+    tt -= rint(tt);
+    const double bl = fabs(f) * z * lambda / (n * pitch * 0.5);
+    t[k] = make_float4((float)tt, (float)(lambda * lambda * f * f), (float)(bl * bl), (float)f);
+}
+
+struct AsmConst {
+    float c0, K;          // frac(z/lambda), z/lambda
+    double zl;            // z/lambda
+    double dfx, dfy;      // frequency steps 1/(Np p)
+    double lam2;          // lambda^2
+    int npx, npy;
+    int band;
+};
+
+// exp(i 2 pi (c0 + tx + ty - K poly(u))) or 0; EXACT: z/lambda sqrt(1-u) in FP64
+template <bool EXACT>
+__device__ __forceinline__ float2 transfer(const float4 ax, const float4 ay, int c, int i, const AsmConst& k) {
+    const float u = ax.y + ay.y;
+    if (u >= 1.f) return make_float2(0.f, 0.f);
+    if (k.band) {
+        const float s = 1.f - u;
+        if (ax.z > s || ay.z > s) return make_float2(0.f, 0.f);
+    }
+    float ph;
+    if (!EXACT) {
+        const float p = u * u * fmaf(u, fmaf(u, fmaf(u, 7.f / 256.f, 5.f / 128.f), 1.f / 16.f), 0.125f);
+        ph = k.c0 + ax.x + ay.x - k.K * p;
+    } else {
+        const int kx = c < k.npx / 2 ? c : c - k.npx;
+        const int ky = i < k.npy / 2 ? i : i - k.npy;
+        const double fx = kx * k.dfx, fy = ky * k.dfy;
+        const double uu = k.lam2 * (fx * fx + fy * fy);
+        if (uu >= 1.0) return make_float2(0.f, 0.f);
+        const double t = k.zl * sqrt(1.0 - uu);
+        ph = (float)(t - rint(t));
+    }
+    return make_float2(cos_turns(ph), sin_turns(ph));
+}
+
+template <int L>
+__global__ __launch_bounds__(512) void asm_rows_fwd(const float2* __restrict__ in, int nx, int ny, int ox,
+                                                    int oy, float2* __restrict__ work,
+                                                    const float2* __restrict__ tw) {
+    constexpr int TPL = L / 16;
+    constexpr int LPW = fftk::row_lines(L);
+    constexpr int PL = fft::padded_len(L);
+    __shared__ float2 lds[LPW * PL];
+    const int sub = threadIdx.x / TPL, t = threadIdx.x % TPL;
+    const int r = blockIdx.x * LPW + sub;
+    float2* buf = lds + sub * PL;
+    const bool ok = r < ny;
+    const float2* src = in + (size_t)(ok ? r : 0) * nx;
+    for (int i = t; i < L; i += TPL) {
+        const int c = i - ox;
+        buf[fft::pad16(i)] = (ok && c >= 0 && c < nx) ? src[c] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft::line<L, -1>(buf, tw, t);
+    if (ok) {
+        float2* dst = work + (size_t)(oy + r) * L;
+        for (int i = t; i < L; i += TPL) dst[i] = buf[fft::pad16(i)];
+    }
+}
+
+template <int L, int B, bool EXACT>
+__global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, int oy,
+                                                 const float4* __restrict__ tabx,
+                                                 const float4* __restrict__ taby, AsmConst k,
+                                                 const float2* __restrict__ tw) {
+    constexpr int TPL = L / 16;
+    constexpr int PL = fft::padded_len(L);
+    extern __shared__ float2 lds[];
+    const int nthr = TPL * B;
+    const int g = xcd_remap(blockIdx.x, gridDim.x);
+    const int cbase = g * B;
+    for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
+        const int b = idx % B, i = idx / B;
+        const int c = cbase + b;
+        const bool in_rows = i >= oy && i < oy + ny;
+        lds[b * PL + fft::pad16(i)] =
+            (in_rows && c < npx) ? work[(size_t)i * npx + c] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
+    float2* line = lds + b * PL;
+    fft::line<L, -1>(line, tw, t);
+    const int c = cbase + b;
+    const float4 ax = tabx[c < npx ? c : 0];
+    for (int i = t; i < L; i += TPL) {
+        const float2 h = transfer<EXACT>(ax, taby[i], c, i, k);
+        line[fft::pad16(i)] = fft::cmul(line[fft::pad16(i)], h);
+    }
+    __syncthreads();
+    fft::line<L, +1>(line, tw, t);
+    for (int idx = threadIdx.x; idx < ny * B; idx += nthr) {
+        const int bb = idx % B, i = oy + idx / B;
+        const int cc = cbase + bb;
+        if (cc < npx) work[(size_t)i * npx + cc] = lds[bb * PL + fft::pad16(i)];
+    }
+}
+
+template <int L>
+__global__ __launch_bounds__(512) void asm_rows_inv(const float2* __restrict__ work, int nx, int ny, int ox,
+                                                    int oy, float scale, float2* __restrict__ out,
+                                                    const float2* __restrict__ tw) {
+    constexpr int TPL = L / 16;
+    constexpr int LPW = fftk::row_lines(L);
+    constexpr int PL = fft::padded_len(L);
+    __shared__ float2 lds[LPW * PL];
+    const int sub = threadIdx.x / TPL, t = threadIdx.x % TPL;
+    const int r = blockIdx.x * LPW + sub;
+    float2* buf = lds + sub * PL;
+    const bool ok = r < ny;
+    const float2* src = work + (size_t)(oy + (ok ? r : 0)) * L;
+    for (int i = t; i < L; i += TPL) buf[fft::pad16(i)] = ok ? src[i] : make_float2(0.f, 0.f);
+    __syncthreads();
+    fft::line<L, +1>(buf, tw, t);
+    if (ok) {
+        float2* dst = out + (size_t)r * nx;
+        for (int c = t; c < nx; c += TPL) {
+            const float2 v = buf[fft::pad16(c + ox)];
+            dst[c] = make_float2(v.x * scale, v.y * scale);
+        }
+    }
+}
+
+template <int LX>
+int launch_rows_fwd(oph_ctx* ctx, const float2* in, int nx, int ny, int ox, int oy, float2* work) {
+    const float2* tw = oph_twiddles(ctx, LX);
+    if (!tw) return OPH_ENOMEM;
+    constexpr int LPW = fftk::row_lines(LX);
+    OPH_LAUNCH(ctx, "asm_rows_fwd", asm_rows_fwd<LX>, dim3((ny + LPW - 1) / LPW), dim3(LPW * (LX / 16)), 0, in,
+               nx, ny, ox, oy, work, tw);
+    return OPH_OK;
+}
+
+template <int LX>
+int launch_rows_inv(oph_ctx* ctx, const float2* work, int nx, int ny, int ox, int oy, float scale, float2* out) {
+    const float2* tw = oph_twiddles(ctx, LX);
+    if (!tw) return OPH_ENOMEM;
+    constexpr int LPW = fftk::row_lines(LX);
+    OPH_LAUNCH(ctx, "asm_rows_inv", asm_rows_inv<LX>, dim3((ny + LPW - 1) / LPW), dim3(LPW * (LX / 16)), 0, work,
+               nx, ny, ox, oy, scale, out, tw);
+    return OPH_OK;
+}
+
+template <int LY, bool EXACT>
+int launch_cols_ex(oph_ctx* ctx, float2* work, int npx, int ny, int oy, const float4* tx, const float4* ty,
+                   const AsmConst& k) {
+    const float2* tw = oph_twiddles(ctx, LY);
+    if (!tw) return OPH_ENOMEM;
+    constexpr int B = fftk::col_lines(LY);
+    const size_t shm = (size_t)B * fft::padded_len(LY) * sizeof(float2);
+    static bool attr = false;
+    if (!attr) {
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&asm_cols<LY, B, EXACT>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipGetLastError();
+        attr = true;
+    }
+    OPH_LAUNCH(ctx, "asm_cols", (asm_cols<LY, B, EXACT>), dim3((npx + B - 1) / B), dim3(B * (LY / 16)), shm,
+               work, npx, ny, oy, tx, ty, k, tw);
+    return OPH_OK;
+}
+
+template <int LY>
+int launch_cols(oph_ctx* ctx, bool exact, float2* work, int npx, int ny, int oy, const float4* tx,
+                const float4* ty, const AsmConst& k) {
+    return exact ? launch_cols_ex<LY, true>(ctx, work, npx, ny, oy, tx, ty, k)
+                 : launch_cols_ex<LY, false>(ctx, work, npx, ny, oy, tx, ty, k);
+}
+
+}  // namespace
+
+// Device-pointer core of oph_asm_propagate: field (ny x nx) in place, FFT grid
+// npx x npy (powers of two >= nx, ny), work buffer of npx*npy.
+int oph_asm_device(oph_ctx* ctx, float2* field, int nx, int ny, int npx, int npy, double px, double py,
+                   double lambda, double z, int band_limit, float2* work) {
+    const int ox = npx / 2 - nx / 2, oy = npy / 2 - ny / 2;
+    float4* tab = static_cast<float4*>(oph_scratch(ctx, SLOT_TABLE, sizeof(float4) * (npx + npy)));
+    if (!tab) return OPH_ENOMEM;
+    OPH_LAUNCH(ctx, "asm_axis_table", asm_axis_table, dim3((npx + 255) / 256), dim3(256), 0, tab, npx, px,
+               lambda, z);
+    OPH_LAUNCH(ctx, "asm_axis_table", asm_axis_table, dim3((npy + 255) / 256), dim3(256), 0, tab + npx, npy, py,
+               lambda, z);
+    const double zl = z / lambda;
+    AsmConst k;
+    k.zl = zl;
+    k.c0 = (float)(zl - rint(zl));
+    k.K = (float)zl;
+    k.dfx = 1.0 / (npx * px);
+    k.dfy = 1.0 / (npy * py);
+    k.lam2 = lambda * lambda;
+    k.npx = npx;
+    k.npy = npy;
+    k.band = band_limit;
+    // truncation of the series at u^5: next term 21/1024 u^6 * z/lambda
+    const double umax = lambda * lambda * (0.25 / (px * px) + 0.25 / (py * py));
+    const bool exact = umax >= 0.5 || fabs(zl) * pow(umax, 6) * (21.0 / 1024.0) > 1e-7;
+    int rc;
+    switch (npx) {
+#define OPH_RF(n) case n: rc = launch_rows_fwd<n>(ctx, field, nx, ny, ox, oy, work); break;
+        OPH_FFT_LENGTHS(OPH_RF)
+#undef OPH_RF
+        default: return oph_fail(ctx, OPH_ENOTSUP, "ASM grid width %d unsupported", npx);
+    }
+    if (rc) return rc;
+    switch (npy) {
+#define OPH_CF(n) case n: rc = launch_cols<n>(ctx, exact, work, npx, ny, oy, tab, tab + npx, k); break;
+        OPH_FFT_LENGTHS(OPH_CF)
+#undef OPH_CF
+        default: return oph_fail(ctx, OPH_ENOTSUP, "ASM grid height %d unsupported", npy);
+    }
+    if (rc) return rc;
+    const float scale = (float)(1.0 / ((double)npx * npy));
+    switch (npx) {
+#define OPH_RI(n) case n: rc = launch_rows_inv<n>(ctx, work, nx, ny, ox, oy, scale, field); break;
+        OPH_FFT_LENGTHS(OPH_RI)
+#undef OPH_RI
+    }
+    return rc;
+}
+
+extern "C" int oph_asm_propagate(oph_ctx* ctx, oph_c64* field, int nx, int ny, double px, double py,
+                                 double lambda, double z, int pad, int band_limit, int flags) {
+    int rc = oph_enter(ctx);
+    if (rc) return rc;
+    if (!field) return oph_fail(ctx, OPH_EINVAL, "null field");
+    if (pad != 1 && pad != 2) return oph_fail(ctx, OPH_EINVAL, "pad must be 1 or 2");
+    if (!(px > 0) || !(py > 0) || !(lambda > 0)) return oph_fail(ctx, OPH_EINVAL, "bad pitch or wavelength");
+    const int npx = nx * pad, npy = ny * pad;
+    if (!fftk::supported(npx) || !fftk::supported(npy))
+        return oph_fail(ctx, OPH_ENOTSUP, "padded grid %dx%d: sides must be powers of two in 16..8192", npx, npy);
+    if ((flags & OPH_ASYNC) && !(flags & OPH_DEVICE_PTR))
+        return oph_fail(ctx, OPH_EINVAL, "OPH_ASYNC needs device pointers");
+    const size_t fbytes = (size_t)nx * ny * sizeof(float2);
+    float2* f = reinterpret_cast<float2*>(field);
+    if (!(flags & OPH_DEVICE_PTR)) {
+        f = static_cast<float2*>(oph_scratch(ctx, SLOT_IN, fbytes));
+        if (!f) return OPH_ENOMEM;
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync(f, field, fbytes, hipMemcpyHostToDevice, ctx->stream));
+    }
+    float2* work = static_cast<float2*>(oph_scratch(ctx, SLOT_WORK1, (size_t)npx * npy * sizeof(float2)));
+    if (!work) return OPH_ENOMEM;
+    if ((rc = oph_asm_device(ctx, f, nx, ny, npx, npy, px, py, lambda, z, band_limit, work))) return rc;
+    if (!(flags & OPH_DEVICE_PTR))
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync(field, f, fbytes, hipMemcpyDeviceToHost, ctx->stream));
+    return oph_finish(ctx, flags);
+}

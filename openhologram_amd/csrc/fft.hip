@@ -1,0 +1,191 @@
+// fft.hip -- 2-D complex64 FFT for gfx950 (replaces FFTW / cuFFT Z2Z of the
+// reference's fftwShift / cudaFFT, DM/Hologram/src/HologramGenerator_CPU.cpp:
+// 285-322, DM/Hologram/src/HologramKernel.cu:189-225).
+//
+// Row pass: one LDS line per L/16 threads, several lines per workgroup for
+// short rows; rows are read and written with unit stride.
+// Column pass: a workgroup stages B neighbouring columns (B*8-byte runs per
+// row) in LDS and transforms them together; workgroup ids are remapped so
+// neighbouring column groups run on the same XCD and share its L2.
+// Twiddle tables are computed once per length in FP64 and cached in the
+// context.  The plan is implicit (length -> kernel instance); nothing is
+// created or destroyed per call.
+#include <cmath>
+#include <vector>
+
+#include "fft_engine.h"
+#include "fft_kernels.h"
+
+const float2* oph_twiddles(oph_ctx* ctx, int L) {
+    int lg = 0;
+    while ((1 << lg) < L) ++lg;
+    if ((1 << lg) != L || lg >= 24) {
+        oph_fail(ctx, OPH_EINVAL, "FFT length %d is not a power of two", L);
+        return nullptr;
+    }
+    if (ctx->tw[lg]) return static_cast<const float2*>(ctx->tw[lg]);
+    std::vector<float2> h(L);
+    for (int m = 0; m < L; ++m) {
+        const double a = -2.0 * M_PI * (double)m / (double)L;
+        h[m] = make_float2((float)cos(a), (float)sin(a));
+    }
+    void* d = nullptr;
+    if (hipMalloc(&d, sizeof(float2) * L) != hipSuccess) {
+        hipGetLastError();
+        oph_fail(ctx, OPH_ENOMEM, "twiddle table allocation failed");
+        return nullptr;
+    }
+    if (hipMemcpy(d, h.data(), sizeof(float2) * L, hipMemcpyHostToDevice) != hipSuccess) {
+        hipGetLastError();
+        hipFree(d);
+        oph_fail(ctx, OPH_EDEVICE, "twiddle table upload failed");
+        return nullptr;
+    }
+    ctx->tw[lg] = d;
+    return static_cast<const float2*>(d);
+}
+
+namespace {
+
+// ---- plain row / column kernels --------------------------------------------
+
+template <int L, int SIGN>
+__global__ __launch_bounds__(512) void fft_rows_kernel(float2* data, int rows, int stride, float scale,
+                                                       const float2* __restrict__ tw) {
+    constexpr int TPL = L / 16;
+    constexpr int LPW = fftk::row_lines(L);
+    constexpr int PL = fft::padded_len(L);
+    __shared__ float2 lds[LPW * PL];
+    const int sub = threadIdx.x / TPL, t = threadIdx.x % TPL;
+    const int row = blockIdx.x * LPW + sub;
+    float2* buf = lds + sub * PL;
+    const bool ok = row < rows;
+    float2* src = data + (size_t)(ok ? row : 0) * stride;
+#pragma unroll
+    for (int i = t; i < L; i += TPL) buf[fft::pad16(i)] = ok ? src[i] : make_float2(0.f, 0.f);
+    __syncthreads();
+    fft::line<L, SIGN>(buf, tw, t);
+    if (ok) {
+#pragma unroll
+        for (int i = t; i < L; i += TPL) {
+            const float2 v = buf[fft::pad16(i)];
+            src[i] = make_float2(v.x * scale, v.y * scale);
+        }
+    }
+}
+
+template <int L, int SIGN, int B>
+__global__ __launch_bounds__(1024) void fft_cols_kernel(float2* data, int ncols, int stride, float scale,
+                                                        const float2* __restrict__ tw) {
+    constexpr int TPL = L / 16;
+    constexpr int PL = fft::padded_len(L);
+    extern __shared__ float2 lds[];
+    const int nthr = TPL * B;
+    const int g = xcd_remap(blockIdx.x, gridDim.x);
+    const int c0 = g * B;
+    for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
+        const int b = idx % B, i = idx / B;
+        const int c = c0 + b;
+        lds[b * PL + fft::pad16(i)] = c < ncols ? data[(size_t)i * stride + c] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
+    fft::line<L, SIGN>(lds + b * PL, tw, t);
+    for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
+        const int bb = idx % B, i = idx / B;
+        const int c = c0 + bb;
+        if (c < ncols) {
+            const float2 v = lds[bb * PL + fft::pad16(i)];
+            data[(size_t)i * stride + c] = make_float2(v.x * scale, v.y * scale);
+        }
+    }
+}
+
+template <int L, int SIGN>
+int launch_rows(oph_ctx* ctx, float2* data, int rows, int stride, float scale) {
+    const float2* tw = oph_twiddles(ctx, L);
+    if (!tw) return OPH_ENOMEM;
+    constexpr int LPW = fftk::row_lines(L);
+    const int grid = (rows + LPW - 1) / LPW;
+    OPH_LAUNCH(ctx, "fft_rows", (fft_rows_kernel<L, SIGN>), dim3(grid), dim3(LPW * (L / 16)), 0, data, rows,
+               stride, scale, tw);
+    return OPH_OK;
+}
+
+template <int L, int SIGN>
+int launch_cols(oph_ctx* ctx, float2* data, int ncols, int stride, float scale) {
+    const float2* tw = oph_twiddles(ctx, L);
+    if (!tw) return OPH_ENOMEM;
+    constexpr int B = fftk::col_lines(L);
+    const size_t shm = (size_t)B * fft::padded_len(L) * sizeof(float2);
+    static bool attr = false;
+    if (!attr) {
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_cols_kernel<L, SIGN, B>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipGetLastError();
+        attr = true;
+    }
+    const int grid = (ncols + B - 1) / B;
+    OPH_LAUNCH(ctx, "fft_cols", (fft_cols_kernel<L, SIGN, B>), dim3(grid), dim3(B * (L / 16)), shm, data,
+               ncols, stride, scale, tw);
+    return OPH_OK;
+}
+
+template <int SIGN>
+int rows_dispatch(oph_ctx* ctx, int L, float2* data, int rows, int stride, float scale) {
+    switch (L) {
+#define OPH_ROW_CASE(n) \
+    case n: return launch_rows<n, SIGN>(ctx, data, rows, stride, scale);
+        OPH_FFT_LENGTHS(OPH_ROW_CASE)
+#undef OPH_ROW_CASE
+    }
+    return oph_fail(ctx, OPH_ENOTSUP, "FFT length %d not supported (16..8192)", L);
+}
+
+template <int SIGN>
+int cols_dispatch(oph_ctx* ctx, int L, float2* data, int ncols, int stride, float scale) {
+    switch (L) {
+#define OPH_COL_CASE(n) \
+    case n: return launch_cols<n, SIGN>(ctx, data, ncols, stride, scale);
+        OPH_FFT_LENGTHS(OPH_COL_CASE)
+#undef OPH_COL_CASE
+    }
+    return oph_fail(ctx, OPH_ENOTSUP, "FFT length %d not supported (16..8192)", L);
+}
+
+}  // namespace
+
+int oph_fft_rows(oph_ctx* ctx, int L, int sign, float2* data, int rows, int stride, float scale) {
+    return sign < 0 ? rows_dispatch<-1>(ctx, L, data, rows, stride, scale)
+                    : rows_dispatch<+1>(ctx, L, data, rows, stride, scale);
+}
+
+int oph_fft_cols(oph_ctx* ctx, int L, int sign, float2* data, int ncols, int stride, float scale) {
+    return sign < 0 ? cols_dispatch<-1>(ctx, L, data, ncols, stride, scale)
+                    : cols_dispatch<+1>(ctx, L, data, ncols, stride, scale);
+}
+
+extern "C" int oph_fft2d(oph_ctx* ctx, oph_c64* data, int nx, int ny, int direction, int normalize,
+                         int flags) {
+    int rc = oph_enter(ctx);
+    if (rc) return rc;
+    if (!data) return oph_fail(ctx, OPH_EINVAL, "null data");
+    if (direction != -1 && direction != 1) return oph_fail(ctx, OPH_EINVAL, "direction must be -1 or +1");
+    if (!fftk::supported(nx) || !fftk::supported(ny))
+        return oph_fail(ctx, OPH_ENOTSUP, "FFT size %dx%d: each side must be a power of two in 16..8192", nx, ny);
+    if ((flags & OPH_ASYNC) && !(flags & OPH_DEVICE_PTR))
+        return oph_fail(ctx, OPH_EINVAL, "OPH_ASYNC needs device pointers");
+    const size_t bytes = (size_t)nx * ny * sizeof(float2);
+    float2* d = reinterpret_cast<float2*>(data);
+    if (!(flags & OPH_DEVICE_PTR)) {
+        d = static_cast<float2*>(oph_scratch(ctx, SLOT_WORK0, bytes));
+        if (!d) return OPH_ENOMEM;
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync(d, data, bytes, hipMemcpyHostToDevice, ctx->stream));
+    }
+    const float scale = normalize ? (float)(1.0 / ((double)nx * ny)) : 1.f;
+    if ((rc = oph_fft_rows(ctx, nx, direction, d, ny, nx, 1.f))) return rc;
+    if ((rc = oph_fft_cols(ctx, ny, direction, d, nx, nx, scale))) return rc;
+    if (!(flags & OPH_DEVICE_PTR))
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync(data, d, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    return oph_finish(ctx, flags);
+}

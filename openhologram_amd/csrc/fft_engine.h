@@ -1,0 +1,193 @@
+// fft_engine.h -- LDS-resident Stockham FFT building blocks for gfx950.
+//
+// A line of L = 16^a * r complex64 values (r in {1, 2, 4, 8}) sits in LDS in
+// natural order.  L/16 threads transform it: a radix-16 stage per factor of
+// 16 plus one radix-r stage, each stage reading its R inputs at stride L/R,
+// applying twiddles from a table tw[m] = exp(-2 pi i m / L) computed in FP64,
+// doing the radix-R DFT in registers and writing back in Stockham (self-
+// sorting) order, so the output is again in natural order.  Stages are
+// separated by workgroup barriers and work in place (all reads of a stage
+// complete before its writes).
+//
+// SIGN = -1 is the forward transform (exp(-i ...), FFTW_FORWARD / cuFFT
+// FORWARD), SIGN = +1 the unnormalised inverse.
+#pragma once
+
+#include "oph_internal.h"
+
+namespace fft {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+    return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -a.x * b.y));
+}
+// multiply by SIGN*i:  (x + iy) * (s i) = -s y + i s x
+template <int SIGN>
+__device__ __forceinline__ float2 mul_si(float2 a) {
+    return SIGN < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+    const float2 t = a;
+    a = cadd(t, b);
+    b = csub(t, b);
+}
+
+template <int SIGN>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+    const float2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
+    const float2 s13 = cadd(a1, a3), d13 = mul_si<SIGN>(csub(a1, a3));
+    a0 = cadd(s02, s13);
+    a2 = csub(s02, s13);
+    a1 = cadd(d02, d13);
+    a3 = csub(d02, d13);
+}
+
+template <int R, int SIGN>
+struct Dft;
+
+template <int SIGN>
+struct Dft<1, SIGN> {
+    __device__ static void run(float2*) {}
+};
+template <int SIGN>
+struct Dft<2, SIGN> {
+    __device__ static void run(float2* v) { dft2<SIGN>(v[0], v[1]); }
+};
+template <int SIGN>
+struct Dft<4, SIGN> {
+    __device__ static void run(float2* v) { dft4<SIGN>(v[0], v[1], v[2], v[3]); }
+};
+template <int SIGN>
+struct Dft<8, SIGN> {
+    __device__ static void run(float2* v) {
+        // radix 2 x 4: v[n], n = 4 n1 + n2?  use decimation: even/odd split
+        constexpr float h = 0.70710678118654752f;
+        dft4<SIGN>(v[0], v[2], v[4], v[6]);
+        dft4<SIGN>(v[1], v[3], v[5], v[7]);
+        // twiddle odd part by W8^k, k = 0..3 (W8 = exp(SIGN i 2pi/8))
+        v[3] = make_float2(h * (v[3].x - SIGN * v[3].y), h * (v[3].y + SIGN * v[3].x));
+        v[5] = mul_si<SIGN>(v[5]);
+        v[7] = make_float2(h * (-v[7].x - SIGN * v[7].y), h * (-v[7].y + SIGN * v[7].x));
+        float2 o[8];
+        o[0] = cadd(v[0], v[1]);
+        o[4] = csub(v[0], v[1]);
+        o[1] = cadd(v[2], v[3]);
+        o[5] = csub(v[2], v[3]);
+        o[2] = cadd(v[4], v[5]);
+        o[6] = csub(v[4], v[5]);
+        o[3] = cadd(v[6], v[7]);
+        o[7] = csub(v[6], v[7]);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = o[i];
+    }
+};
+template <int SIGN>
+struct Dft<16, SIGN> {
+    __device__ static void run(float2* v) {
+        // 4 x 4: X[k1 + 4 k2] = sum_{n2} W4^{n2 k2} W16^{n2 k1} sum_{n1} W4^{n1 k1} x[4 n1 + n2]
+#pragma unroll
+        for (int n2 = 0; n2 < 4; ++n2) dft4<SIGN>(v[n2], v[n2 + 4], v[n2 + 8], v[n2 + 12]);
+        // now v[n2 + 4 k1] holds inner DFT k1 of column n2; twiddle W16^{n2 k1}
+        constexpr float c1 = 0.92387953251128676f, s1 = 0.38268343236508977f, h = 0.70710678118654752f;
+        const float2 w1 = make_float2(c1, SIGN * s1), w2 = make_float2(h, SIGN * h),
+                     w3 = make_float2(s1, SIGN * c1), w6 = make_float2(-h, SIGN * h),
+                     w9 = make_float2(-c1, -SIGN * s1);
+        v[1 + 4] = cmul(v[1 + 4], w1);
+        v[1 + 8] = cmul(v[1 + 8], w2);
+        v[1 + 12] = cmul(v[1 + 12], w3);
+        v[2 + 4] = cmul(v[2 + 4], w2);
+        v[2 + 8] = mul_si<SIGN>(v[2 + 8]);
+        v[2 + 12] = cmul(v[2 + 12], w6);
+        v[3 + 4] = cmul(v[3 + 4], w3);
+        v[3 + 8] = cmul(v[3 + 8], w6);
+        v[3 + 12] = cmul(v[3 + 12], w9);
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1) dft4<SIGN>(v[4 * k1], v[4 * k1 + 1], v[4 * k1 + 2], v[4 * k1 + 3]);
+        // v[4 k1 + k2] holds X[k1 + 4 k2]; transpose into natural order
+        float2 o[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 4; ++k1)
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) o[k1 + 4 * k2] = v[4 * k1 + k2];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) v[i] = o[i];
+    }
+};
+
+// Radix of the last (non-16) stage of a length-L transform.
+__host__ __device__ constexpr int last_radix(int L) {
+    int r = L;
+    while (r >= 16 && r % 16 == 0) r /= 16;
+    return r;  // 1, 2, 4 or 8 for powers of two
+}
+__host__ __device__ constexpr int n16(int L) {
+    int n = 0;
+    while (L >= 16 && L % 16 == 0) { L /= 16; ++n; }
+    return n;
+}
+
+// LDS index with one float2 of padding per 16 elements (spreads the strided
+// Stockham writes over the banks).
+__device__ __forceinline__ int pad16(int i) { return i + (i >> 4); }
+__host__ __device__ constexpr int padded_len(int L) { return L + L / 16; }
+
+// One Stockham stage of radix R over a line in LDS (padded indexing).
+template <int L, int R, int SIGN, int TPL>
+__device__ __forceinline__ void stage(float2* buf, const float2* __restrict__ tw, int Ns, int t) {
+    constexpr int NB = L / R;
+    constexpr int PER = NB / TPL;  // butterflies per thread (TPL divides NB)
+    float2 v[PER][R];
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int j = t + p * TPL;
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[p][r] = buf[pad16(j + r * NB)];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PER; ++p) {
+        const int j = t + p * TPL;
+        const int k = j & (Ns - 1);
+        if (Ns > 1) {
+            const int m = k * (L / (Ns * R));
+#pragma unroll
+            for (int r = 1; r < R; ++r) {
+                const float2 w = tw[m * r];
+                v[p][r] = SIGN < 0 ? cmul(v[p][r], w) : cmulc(v[p][r], w);
+            }
+        }
+        Dft<R, SIGN>::run(v[p]);
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) buf[pad16(base + r * Ns)] = v[p][r];
+    }
+    __syncthreads();
+}
+
+// Full transform of one LDS line by TPL = L/16 threads (t in [0, TPL)).
+// Caller syncs before (line loaded) and the last stage ends with a barrier.
+template <int L, int SIGN>
+__device__ __forceinline__ void line(float2* buf, const float2* __restrict__ tw, int t) {
+    constexpr int TPL = L / 16;
+    constexpr int A = n16(L);
+    constexpr int RL = last_radix(L);
+    int Ns = 1;
+#pragma unroll
+    for (int s = 0; s < A; ++s) {
+        stage<L, 16, SIGN, TPL>(buf, tw, Ns, t);
+        Ns *= 16;
+    }
+    if (RL > 1) stage<L, RL, SIGN, TPL>(buf, tw, Ns, t);
+}
+
+}  // namespace fft
+
+// Device twiddle table of length L (exp(-2 pi i m / L), FP64-computed), cached
+// per length in the context.  Returns nullptr on failure (error recorded).
+const float2* oph_twiddles(oph_ctx* ctx, int L);

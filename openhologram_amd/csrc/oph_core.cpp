@@ -139,6 +139,8 @@ void oph_destroy(oph_ctx* ctx) {
     if (ctx->stream) hipStreamSynchronize(ctx->stream);
     for (int i = 0; i < SLOT_COUNT; ++i)
         if (ctx->slot_ptr[i]) hipFree(ctx->slot_ptr[i]);
+    for (void* t : ctx->tw)
+        if (t) hipFree(t);
     for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
     if (ctx->ev_begin) hipEventDestroy(ctx->ev_begin);
     if (ctx->ev_end) hipEventDestroy(ctx->ev_end);
@@ -152,6 +154,13 @@ int oph_sync(oph_ctx* ctx) {
     int rc = oph_enter(ctx);
     if (rc) return rc;
     OPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->pending_overflow) {
+        unsigned f = 0;
+        OPH_HIP_CHECK(ctx, hipMemcpy(&f, ctx->pending_overflow, sizeof f, hipMemcpyDeviceToHost));
+        ctx->pending_overflow = nullptr;
+        if (f) return oph_fail(ctx, OPH_ENOMEM, "an asynchronous WRP call overflowed its tile lists; "
+                                                "repeat it synchronously");
+    }
     return OPH_OK;
 }
 

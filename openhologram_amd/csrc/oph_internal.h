@@ -49,6 +49,8 @@ struct oph_ctx {
     size_t slot_size[SLOT_COUNT] = {};
     std::string err;
     int cu_count = 256;
+    void* tw[24] = {};  // FFT twiddle tables by log2(length)
+    unsigned* pending_overflow = nullptr;  // device flag left by an async WRP call
 };
 
 // Records an error message on the context and returns code.

@@ -243,11 +243,12 @@ extern "C" int oph_pc_rs(oph_ctx* ctx, const oph_spec* spec, const oph_points* p
     RsArgs a{};
     a.n = n;
     if (!(flags & OPH_DEVICE_PTR) && !oph_scratch(ctx, SLOT_PTS, 5 * pbytes)) return OPH_ENOMEM;
-    a.x = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->x, pbytes, flags, 0 * pbytes));
-    a.y = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->y, pbytes, flags, 1 * pbytes));
-    a.z = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->z, pbytes, flags, 2 * pbytes));
-    a.amp = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->amp, pbytes, flags, 3 * pbytes));
-    a.phase = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->phase, pbytes, flags, 4 * pbytes));
+    const int sflags = n > 0 ? flags : (flags | OPH_DEVICE_PTR);  // nothing to copy when n == 0
+    a.x = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->x, pbytes, sflags, 0 * pbytes));
+    a.y = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->y, pbytes, sflags, 1 * pbytes));
+    a.z = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->z, pbytes, sflags, 2 * pbytes));
+    a.amp = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->amp, pbytes, sflags, 3 * pbytes));
+    a.phase = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->phase, pbytes, sflags, 4 * pbytes));
     if (n > 0 && (!a.x || !a.y || !a.z)) return OPH_ENOMEM;
     if (n > 0 && ((pts->amp && !a.amp) || (pts->phase && !a.phase))) return OPH_ENOMEM;
 

@@ -1,0 +1,423 @@
+// wrp.hip -- wavefront-recording-plane accumulation on gfx950, and the full
+// WRP hologram pipeline (WRP, then angular-spectrum propagation).
+//
+// Replaces OPHWRP::SingleWRP (W/openwrp/WRP.cpp:75-142, with CWO::AddPixel,
+// CWO/cwo.cpp:4298-4309).  Per point n, with dz = z_wrp - z_n:
+//   centre  tx = (int)(x_n / p) + Nx/2, ty = (int)(y_n / p) + Ny/2  (:114-115)
+//   window  wx, wy in [-w, w)                                      (:119-120)
+//   r = sign(dz) sqrt((wx p)^2 + (wy p)^2 + dz^2)                   (:122-128)
+//   wrp[ty+wy][tx+wx] += a_n exp(i k r) / (r + 0.05)                (:131-136)
+// Half-width w: mode 0 the grating-angle radius (int)(|dz| tan(asin(lambda/
+// (2p)))/p) (CWO/cwoPLS.cpp:613), mode 1 the reference's
+// (int)|lambda dz/(2p^2) + 0.5| * 2 - 1 (WRP.cpp:111).
+//
+// The reference scatters with a racing AddPixel; here the plane is gathered
+// per 64x64 tile:
+//   1. wrp_prep   one thread per point: centre, window, FP64-reduced phase
+//                 constants; counts the tiles each window touches;
+//   2. scan       tile offsets;
+//   3. wrp_fill   point indices into per-tile lists;
+//   4. wrp_gather one workgroup per tile: each wave takes points of the list
+//                 and spreads the window-tile intersection over its lanes;
+//                 contributions are summed in LDS as 64-bit fixed point
+//                 (integer adds are associative, so the result does not
+//                 depend on the list order: deterministic without sorting).
+// Per pixel, with u = (wx^2 + wy^2) p^2 / dz^2 (<= 2 tan^2 ~ 2.2e-3 for the
+// grating-angle window):
+//   (r - |dz|)/|dz| = P(u) = u/2 - u^2/8 + u^3/16 - 5u^4/128,
+//   phase (turns) = frac(sign |dz|/lambda + phi) + sign |dz|/lambda P(u),
+//   1/(r + 0.05)  = 1/D (1 - sign |dz| P/D),  D = sign |dz| + 0.05,
+// so no square root or division per pixel; points whose window makes the
+// truncated series worse than 1e-7 turns use an exact FP64 evaluation.
+#include <cmath>
+
+#include "oph_internal.h"
+#include "pipelines.h"
+
+namespace {
+
+constexpr int TILE = 64;
+constexpr int GATHER_THREADS = 256;
+
+struct WrpPoint {
+    int tx, ty, w, exact;
+    float f0;     // frac(sign |dz| / lambda + phase/2pi), turns
+    float ks;     // sign |dz| / lambda
+    float p2dz2;  // p^2 / dz^2
+    float ampd;   // a / D
+    float cd;     // -sign |dz| / D
+    float pad0, pad1, pad2;
+    double dz;    // for the exact path
+    double pad3;
+};
+
+struct WrpArgs {
+    const double* x;
+    const double* y;
+    const double* z;
+    const double* amp;
+    int64_t n;
+    int nx, ny, ntx, nty;
+    double pitch, inv_lambda, z_wrp, grating, lambda;
+    int mode;
+    WrpPoint* pts;
+    unsigned* counts;     // [ntiles]
+    unsigned* offsets;    // [ntiles + 1]
+    unsigned* cursor;     // [ntiles]
+    unsigned* list;
+    unsigned capacity;
+    unsigned* overflow;
+    unsigned* maxbits;    // float bits of max |a/D|
+};
+
+__global__ void wrp_zero(unsigned* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+__device__ __forceinline__ void tile_range(const WrpPoint& q, const WrpArgs& a, int& x0, int& x1, int& y0,
+                                           int& y1) {
+    int lo = q.tx - q.w, hi = q.tx + q.w - 1;
+    if (lo < 0) lo = 0;
+    if (hi > a.nx - 1) hi = a.nx - 1;
+    x0 = lo / TILE;
+    x1 = hi / TILE;
+    if (lo > hi) x1 = x0 - 1;
+    lo = q.ty - q.w;
+    hi = q.ty + q.w - 1;
+    if (lo < 0) lo = 0;
+    if (hi > a.ny - 1) hi = a.ny - 1;
+    y0 = lo / TILE;
+    y1 = hi / TILE;
+    if (lo > hi) y1 = y0 - 1;
+}
+
+__global__ void wrp_prep(WrpArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const double dz = a.z_wrp - a.z[i];
+    WrpPoint q;
+    q.tx = (int)(a.x[i] / a.pitch) + (a.nx >> 1);
+    q.ty = (int)(a.y[i] / a.pitch) + (a.ny >> 1);
+    if (a.mode == 0)
+        q.w = (int)(fabs(dz) * a.grating / a.pitch);
+    else
+        q.w = (int)fabs(a.lambda * dz / a.pitch / a.pitch / 2 + 0.5) * 2 - 1;
+    if (q.w < 0) q.w = 0;
+    const double sgn = dz > 0.0 ? 1.0 : -1.0;
+    const double adz = fabs(dz);
+    const double am = a.amp ? a.amp[i] : 1.0;
+    double t = sgn * adz * a.inv_lambda;
+    t -= rint(t);
+    q.f0 = (float)t;
+    q.ks = (float)(sgn * adz * a.inv_lambda);
+    q.p2dz2 = adz > 0.0 ? (float)(a.pitch * a.pitch / (dz * dz)) : 0.f;
+    const double D = sgn * adz + 0.05;
+    q.ampd = (float)(am / D);
+    q.cd = (float)(-sgn * adz / D);
+    q.pad0 = q.pad1 = q.pad2 = 0.f;
+    q.dz = dz;
+    q.pad3 = 0.0;
+    // series error bound at the window corner: (|dz|/lambda) * 7/256 u^5
+    const double umax = 2.0 * (double)q.w * q.w * (adz > 0.0 ? a.pitch * a.pitch / (dz * dz) : 0.0);
+    q.exact = (umax > 0.05 || adz * a.inv_lambda * 7.0 / 256.0 * pow(umax, 5) > 1e-7 ||
+               fabs(q.cd) * 0.5 * umax > 1e-3) ? 1 : 0;
+    a.pts[i] = q;
+    if (q.w > 0) {
+        const float mag = fabsf(q.ampd);
+        atomicMax(a.maxbits, __float_as_uint(mag));
+        int x0, x1, y0, y1;
+        tile_range(q, a, x0, x1, y0, y1);
+        for (int ty = y0; ty <= y1; ++ty)
+            for (int tx = x0; tx <= x1; ++tx) atomicAdd(&a.counts[ty * a.ntx + tx], 1u);
+    }
+}
+
+// exclusive scan of counts into offsets[0..n], one workgroup of 1024 threads
+__global__ __launch_bounds__(1024) void wrp_scan(const unsigned* counts, unsigned* offsets, int n) {
+    __shared__ unsigned part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int b = t * per, e = min(n, b + per);
+    unsigned s = 0;
+    for (int i = b; i < e; ++i) s += counts[i];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const unsigned v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    unsigned run = part[t] - s;  // exclusive prefix of this chunk
+    for (int i = b; i < e; ++i) {
+        offsets[i] = run;
+        run += counts[i];
+    }
+    if (t == 1023) offsets[n] = part[1023];
+}
+
+__global__ void wrp_fill(WrpArgs a) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const WrpPoint q = a.pts[i];
+    if (q.w <= 0) return;
+    int x0, x1, y0, y1;
+    tile_range(q, a, x0, x1, y0, y1);
+    for (int ty = y0; ty <= y1; ++ty)
+        for (int tx = x0; tx <= x1; ++tx) {
+            const int tile = ty * a.ntx + tx;
+            const unsigned slot = a.offsets[tile] + atomicAdd(&a.cursor[tile], 1u);
+            if (slot < a.capacity)
+                a.list[slot] = (unsigned)i;
+            else
+                atomicOr(a.overflow, 1u);
+        }
+}
+
+struct GatherArgs {
+    const WrpPoint* pts;
+    const unsigned* offsets;
+    const unsigned* list;
+    const unsigned* maxbits;
+    unsigned capacity;
+    int nx, ny, ntx;
+    double pitch, inv_lambda;
+    float2* wrp;
+    int accumulate;
+};
+
+__global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
+    __shared__ unsigned long long acc[2 * TILE * TILE];
+    const int tile = blockIdx.x;
+    const int tyi = tile / g.ntx, txi = tile % g.ntx;
+    const int X0 = txi * TILE, Y0 = tyi * TILE;
+    for (int i = threadIdx.x; i < 2 * TILE * TILE; i += GATHER_THREADS) acc[i] = 0ull;
+    // fixed-point scale: |contribution| * 2^S <= 2^30 (int32 per contribution)
+    const float mx = __uint_as_float(*g.maxbits);
+    const int S = mx > 0.f ? max(-60, min(60, 30 - (int)ceilf(log2f(mx)))) : 30;
+    const float scale = ldexpf(1.f, S);
+    __syncthreads();
+    unsigned b = g.offsets[tile], e = g.offsets[tile + 1];
+    if (e > g.capacity) e = g.capacity;
+    if (b > e) b = e;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    constexpr int NW = GATHER_THREADS / 64;
+    for (unsigned k = b + wave; k < e; k += NW) {
+        const WrpPoint q = g.pts[g.list[k]];
+        const int cx0 = max(q.tx - q.w, max(X0, 0)), cx1 = min(q.tx + q.w, min(X0 + TILE, g.nx));
+        const int cy0 = max(q.ty - q.w, max(Y0, 0)), cy1 = min(q.ty + q.w, min(Y0 + TILE, g.ny));
+        const int wd = cx1 - cx0, ht = cy1 - cy0;
+        if (wd <= 0 || ht <= 0) continue;
+        const int cnt = wd * ht;
+        const float inv_wd = 1.f / (float)wd;
+        for (int qq = lane; qq < cnt; qq += 64) {
+            const int row = (int)(((float)qq + 0.5f) * inv_wd);
+            const int col = qq - row * wd;
+            const int wx = cx0 + col - q.tx, wy = cy0 + row - q.ty;
+            float re, im;
+            if (!q.exact) {
+                const float u = (float)(wx * wx + wy * wy) * q.p2dz2;
+                const float p = u * fmaf(u, fmaf(u, fmaf(u, -5.f / 128.f, 1.f / 16.f), -0.125f), 0.5f);
+                const float ph = fmaf(q.ks, p, q.f0);
+                const float am = q.ampd * fmaf(q.cd, p, 1.f);
+                re = am * cos_turns(ph);
+                im = am * sin_turns(ph);
+            } else {
+                const double dx = wx * g.pitch, dy = wy * g.pitch;
+                const double sgn = q.dz > 0.0 ? 1.0 : -1.0;
+                const double r = sgn * sqrt(dx * dx + dy * dy + q.dz * q.dz);
+                double t = r * g.inv_lambda;
+                t -= rint(t);
+                // amplitude a/(r + 0.05) = (a/D) * D/(r + 0.05), D = sign|dz| + 0.05
+                const double D = sgn * fabs(q.dz) + 0.05;
+                const float am = (float)((double)q.ampd * D / (r + 0.05));
+                re = am * cos_turns((float)t);
+                im = am * sin_turns((float)t);
+            }
+            const int pix = (cy0 + row - Y0) * TILE + (cx0 + col - X0);
+            const long long ire = (long long)(int)rintf(re * scale);
+            const long long iim = (long long)(int)rintf(im * scale);
+            atomicAdd(&acc[2 * pix], (unsigned long long)ire);
+            atomicAdd(&acc[2 * pix + 1], (unsigned long long)iim);
+        }
+    }
+    __syncthreads();
+    const double inv = ldexp(1.0, -S);
+    for (int p = threadIdx.x; p < TILE * TILE; p += GATHER_THREADS) {
+        const int row = Y0 + p / TILE, col = X0 + p % TILE;
+        if (row >= g.ny || col >= g.nx) continue;
+        const double re = (double)(long long)acc[2 * p] * inv;
+        const double im = (double)(long long)acc[2 * p + 1] * inv;
+        float2* o = g.wrp + (size_t)row * g.nx + col;
+        if (g.accumulate) {
+            const float2 v = *o;
+            *o = make_float2((float)(re + v.x), (float)(im + v.y));
+        } else {
+            *o = make_float2((float)re, (float)im);
+        }
+    }
+}
+
+}  // namespace
+
+int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z_wrp, int mode,
+                   const double* x, const double* y, const double* z, const double* amp, int64_t n,
+                   float2* wrp, int accumulate, int check) {
+    WrpArgs a{};
+    a.x = x;
+    a.y = y;
+    a.z = z;
+    a.amp = amp;
+    a.n = n;
+    a.nx = nx;
+    a.ny = ny;
+    a.ntx = (nx + TILE - 1) / TILE;
+    a.nty = (ny + TILE - 1) / TILE;
+    a.pitch = pitch;
+    a.lambda = lambda;
+    a.inv_lambda = 1.0 / lambda;
+    a.z_wrp = z_wrp;
+    a.grating = tan(asin(lambda / (2.0 * pitch)));
+    a.mode = mode;
+    const int ntiles = a.ntx * a.nty;
+    const int64_t np = n > 0 ? n : 1;
+    a.pts = static_cast<WrpPoint*>(oph_scratch(ctx, SLOT_AUX0, sizeof(WrpPoint) * np));
+    // bookkeeping: counts, cursor, offsets (ntiles + 1), overflow, maxbits
+    const size_t book = (size_t)3 * ntiles + 3;
+    unsigned* bk = static_cast<unsigned*>(oph_scratch(ctx, SLOT_AUX1, sizeof(unsigned) * book));
+    if (!a.pts || !bk) return OPH_ENOMEM;
+    a.counts = bk;
+    a.cursor = bk + ntiles;
+    a.offsets = bk + 2 * ntiles;
+    a.overflow = bk + 3 * ntiles + 1;
+    a.maxbits = bk + 3 * ntiles + 2;
+    size_t cap = ctx->slot_size[SLOT_TABLE2] / sizeof(unsigned);
+    const size_t want = (size_t)24 * np + 65536;
+    if (cap < want) cap = want;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        a.list = static_cast<unsigned*>(oph_scratch(ctx, SLOT_TABLE2, cap * sizeof(unsigned)));
+        if (!a.list) return OPH_ENOMEM;
+        a.capacity = (unsigned)(cap < 0xFFFFFFFFull ? cap : 0xFFFFFFFFull);
+        OPH_LAUNCH(ctx, "wrp_zero", wrp_zero, dim3((unsigned)((book + 255) / 256)), dim3(256), 0, bk, (int)book);
+        const unsigned pg = (unsigned)((np + 255) / 256);
+        OPH_LAUNCH(ctx, "wrp_prep", wrp_prep, dim3(pg), dim3(256), 0, a);
+        OPH_LAUNCH(ctx, "wrp_scan", wrp_scan, dim3(1), dim3(1024), 0, a.counts, a.offsets, ntiles);
+        OPH_LAUNCH(ctx, "wrp_fill", wrp_fill, dim3(pg), dim3(256), 0, a);
+        GatherArgs g{};
+        g.pts = a.pts;
+        g.offsets = a.offsets;
+        g.list = a.list;
+        g.maxbits = a.maxbits;
+        g.capacity = a.capacity;
+        g.nx = nx;
+        g.ny = ny;
+        g.ntx = a.ntx;
+        g.pitch = pitch;
+        g.inv_lambda = a.inv_lambda;
+        g.wrp = wrp;
+        g.accumulate = accumulate;
+        OPH_LAUNCH(ctx, "wrp_gather", wrp_gather, dim3(ntiles), dim3(GATHER_THREADS), 0, g);
+        // the list size is only known on the device; a capacity overflow is
+        // detected after the fact and the pass is redone with the exact size.
+        // Asynchronous calls skip the read-back (the capacity is 24 entries
+        // per point, far above what grating-angle windows need) and leave the
+        // flag for oph_sync() to report.
+        if (!check) {
+            ctx->pending_overflow = a.overflow;
+            break;
+        }
+        unsigned hdr[2] = {0, 0};
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync(hdr, a.offsets + ntiles, 2 * sizeof(unsigned), hipMemcpyDeviceToHost,
+                                          ctx->stream));
+        OPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+        if (!hdr[1] || accumulate) {
+            if (hdr[1]) return oph_fail(ctx, OPH_ENOMEM, "WRP tile lists overflowed with accumulate set");
+            break;
+        }
+        cap = (size_t)hdr[0] + 65536;
+    }
+    return OPH_OK;
+}
+
+extern "C" int oph_wrp(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z_wrp, int mode,
+                       const oph_points* pts, oph_c64* wrp, int flags) {
+    int rc = oph_enter(ctx);
+    if (rc) return rc;
+    if (!pts || !wrp) return oph_fail(ctx, OPH_EINVAL, "null argument");
+    if (nx <= 0 || ny <= 0 || nx > 65536 || ny > 65536) return oph_fail(ctx, OPH_EINVAL, "bad WRP size");
+    if (!(pitch > 0) || !(lambda > 0) || lambda >= 2 * pitch)
+        return oph_fail(ctx, OPH_EINVAL, "pitch and wavelength must be positive with lambda < 2 pitch");
+    if (mode != 0 && mode != 1) return oph_fail(ctx, OPH_EINVAL, "mode must be 0 or 1");
+    if (pts->n < 0 || (pts->n > 0 && (!pts->x || !pts->y || !pts->z)))
+        return oph_fail(ctx, OPH_EINVAL, "bad point arrays");
+    const int64_t n = pts->n;
+    const size_t pb = (size_t)(n > 0 ? n : 1) * sizeof(double);
+    const int sf = n > 0 ? flags : (flags | OPH_DEVICE_PTR);
+    if (!(flags & OPH_DEVICE_PTR) && !oph_scratch(ctx, SLOT_PTS, 4 * pb)) return OPH_ENOMEM;
+    const double* dx = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->x, pb, sf, 0));
+    const double* dy = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->y, pb, sf, pb));
+    const double* dz = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->z, pb, sf, 2 * pb));
+    const double* da = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->amp, pb, sf, 3 * pb));
+    if (n > 0 && (!dx || !dy || !dz || (pts->amp && !da))) return OPH_ENOMEM;
+    const size_t ob = (size_t)nx * ny * sizeof(float2);
+    float2* out = reinterpret_cast<float2*>(wrp);
+    if (!(flags & OPH_DEVICE_PTR)) {
+        out = static_cast<float2*>(oph_scratch(ctx, SLOT_OUT, ob));
+        if (!out) return OPH_ENOMEM;
+        if (flags & OPH_ACCUMULATE)
+            OPH_HIP_CHECK(ctx, hipMemcpyAsync(out, wrp, ob, hipMemcpyHostToDevice, ctx->stream));
+    }
+    if ((rc = oph_wrp_device(ctx, nx, ny, pitch, lambda, z_wrp, mode, dx, dy, dz, da, n, out,
+                             (flags & OPH_ACCUMULATE) ? 1 : 0, !(flags & OPH_ASYNC))))
+        return rc;
+    if (!(flags & OPH_DEVICE_PTR))
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync(wrp, out, ob, hipMemcpyDeviceToHost, ctx->stream));
+    return oph_finish(ctx, flags);
+}
+
+extern "C" int oph_wrp_hologram(oph_ctx* ctx, int n_side, double pitch, double lambda, double z_wrp,
+                                double distance, int pad, int band_limit, int mode, const oph_points* pts,
+                                oph_c64* out, int flags) {
+    int rc = oph_enter(ctx);
+    if (rc) return rc;
+    if (pad != 1 && pad != 2) return oph_fail(ctx, OPH_EINVAL, "pad must be 1 or 2");
+    const int np = n_side * pad;
+    if (n_side <= 0 || (np & (np - 1)) || np < 16 || np > 8192)
+        return oph_fail(ctx, OPH_ENOTSUP, "padded grid %d must be a power of two in 16..8192", np);
+    if ((flags & OPH_ASYNC) && !(flags & OPH_DEVICE_PTR))
+        return oph_fail(ctx, OPH_EINVAL, "OPH_ASYNC needs device pointers");
+    // WRP into the output buffer (device or staged), then ASM in place
+    if (!(flags & OPH_DEVICE_PTR)) {
+        // host buffers: run the device core on staged copies
+        const int64_t n = pts->n;
+        const size_t pb = (size_t)(n > 0 ? n : 1) * sizeof(double);
+        const int sf = n > 0 ? flags : (flags | OPH_DEVICE_PTR);
+        if (!oph_scratch(ctx, SLOT_PTS, 4 * pb)) return OPH_ENOMEM;
+        const double* dx = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->x, pb, sf, 0));
+        const double* dy = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->y, pb, sf, pb));
+        const double* dz = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->z, pb, sf, 2 * pb));
+        const double* da = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->amp, pb, sf, 3 * pb));
+        if (n > 0 && (!dx || !dy || !dz || (pts->amp && !da))) return OPH_ENOMEM;
+        const size_t ob = (size_t)n_side * n_side * sizeof(float2);
+        float2* d = static_cast<float2*>(oph_scratch(ctx, SLOT_OUT, ob));
+        float2* work = static_cast<float2*>(oph_scratch(ctx, SLOT_WORK1, (size_t)np * np * sizeof(float2)));
+        if (!d || !work) return OPH_ENOMEM;
+        if ((rc = oph_wrp_device(ctx, n_side, n_side, pitch, lambda, z_wrp, mode, dx, dy, dz, da, n, d, 0, 1)))
+            return rc;
+        if ((rc = oph_asm_device(ctx, d, n_side, n_side, np, np, pitch, pitch, lambda, distance, band_limit,
+                                 work)))
+            return rc;
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync(out, d, ob, hipMemcpyDeviceToHost, ctx->stream));
+        return oph_finish(ctx, flags);
+    }
+    float2* d = reinterpret_cast<float2*>(out);
+    float2* work = static_cast<float2*>(oph_scratch(ctx, SLOT_WORK1, (size_t)np * np * sizeof(float2)));
+    if (!work) return OPH_ENOMEM;
+    if ((rc = oph_wrp_device(ctx, n_side, n_side, pitch, lambda, z_wrp, mode, pts->x, pts->y, pts->z, pts->amp,
+                             pts->n, d, 0, !(flags & OPH_ASYNC))))
+        return rc;
+    if ((rc = oph_asm_device(ctx, d, n_side, n_side, np, np, pitch, pitch, lambda, distance, band_limit, work)))
+        return rc;
+    return oph_finish(ctx, flags);
+}

@@ -71,6 +71,10 @@ def load():
         "oph_spec_from_keti_config": (I, [C.c_char_p, C.POINTER(Spec)]),
         "oph_load_keti_points": (I, [C.c_char_p, P, I64, C.POINTER(I64)]),
         "oph_pc_rs": (I, [P, C.POINTER(Spec), C.POINTER(Points), P, I]),
+        "oph_fft2d": (I, [P, P, I, I, I, I, I]),
+        "oph_asm_propagate": (I, [P, P, I, I, D, D, D, D, I, I, I]),
+        "oph_wrp": (I, [P, I, I, D, D, D, I, C.POINTER(Points), P, I]),
+        "oph_wrp_hologram": (I, [P, I, D, D, D, D, I, I, I, C.POINTER(Points), P, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -173,6 +177,53 @@ class Context:
         flags = fmt | (ACCUMULATE if accumulate else 0) | (DEVICE_PTR if device else 0) | \
             (ASYNC if asynchronous else 0)
         self.check(self.lib.oph_pc_rs(self.h, C.byref(s), C.byref(pts), ptr(out), flags))
+        del keep
+        return out
+
+
+    def fft2d(self, data, direction=-1, normalize=False, device=False, asynchronous=False):
+        """In-place complex64 2-D FFT (oph_fft2d).  Host arrays are copied."""
+        if not device:
+            data = np.ascontiguousarray(data, np.complex64).copy()
+        ny, nx = data.shape
+        flags = (DEVICE_PTR if device else 0) | (ASYNC if asynchronous else 0)
+        self.check(self.lib.oph_fft2d(self.h, ptr(data), nx, ny, direction, int(normalize), flags))
+        return data
+
+    def asm(self, field, pitch, lambda_, z, pad=2, band_limit=False, device=False, asynchronous=False):
+        """Angular-spectrum propagation in place (oph_asm_propagate)."""
+        px, py = pitch if hasattr(pitch, "__len__") else (pitch, pitch)
+        if not device:
+            field = np.ascontiguousarray(field, np.complex64).copy()
+        ny, nx = field.shape
+        flags = (DEVICE_PTR if device else 0) | (ASYNC if asynchronous else 0)
+        self.check(self.lib.oph_asm_propagate(self.h, ptr(field), nx, ny, px, py, lambda_, z, pad,
+                                              int(band_limit), flags))
+        return field
+
+    def wrp(self, nx, ny, pitch, lambda_, z_wrp, x, y, z, amp=None, mode=0, out=None, accumulate=False,
+            device=False, asynchronous=False):
+        """WRP plane (oph_wrp), (ny, nx) complex64."""
+        pts, keep = _points(x, y, z, amp, None, device)
+        if out is None:
+            out = self.empty((ny, nx), np.complex64) if device else np.zeros((ny, nx), np.complex64)
+        flags = (DEVICE_PTR if device else 0) | (ASYNC if asynchronous else 0) | \
+            (ACCUMULATE if accumulate else 0)
+        self.check(self.lib.oph_wrp(self.h, nx, ny, pitch, lambda_, z_wrp, mode, C.byref(pts), ptr(out),
+                                    flags))
+        del keep
+        return out
+
+    def wrp_hologram(self, n_side, pitch, lambda_, z_wrp, distance, x, y, z, amp=None, pad=2,
+                     band_limit=False, mode=0, out=None, device=False, asynchronous=False):
+        """WRP + ASM hologram (oph_wrp_hologram), (n, n) complex64."""
+        pts, keep = _points(x, y, z, amp, None, device)
+        if out is None:
+            out = self.empty((n_side, n_side), np.complex64) if device else \
+                np.zeros((n_side, n_side), np.complex64)
+        flags = (DEVICE_PTR if device else 0) | (ASYNC if asynchronous else 0)
+        self.check(self.lib.oph_wrp_hologram(self.h, n_side, pitch, lambda_, z_wrp, distance, pad,
+                                             int(band_limit), mode, C.byref(pts), ptr(out), flags))
         del keep
         return out
 

@@ -1,0 +1,115 @@
+"""GPU parity of the FFT, the padded angular-spectrum propagation and the WRP
+accumulation against the FP64 oracle (and numpy's FFT for the transform)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import oracle as O
+from openhologram_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("ny,nx", [(16, 64), (32, 32), (256, 128), (1024, 512), (2048, 4096), (4096, 4096),
+                                   (64, 8192)])
+def test_fft2d_vs_numpy(ctx, ny, nx):
+    print(f"case: fft2d {ny}x{nx}", flush=True)
+    rng = np.random.default_rng(ny + nx)
+    a = (rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))).astype(np.complex64)
+    f = ctx.fft2d(a, -1)
+    ref = np.fft.fft2(a.astype(np.complex128))
+    err = rel_l2(f, ref)
+    print("forward rel-L2", err)
+    assert err < 2e-6
+    b = ctx.fft2d(f, +1, normalize=True)
+    assert rel_l2(b, a) < 4e-6
+
+
+def test_fft2d_delta_and_linearity(ctx):
+    a = np.zeros((64, 128), np.complex64)
+    a[3, 5] = 1.0
+    f = ctx.fft2d(a)
+    ky, kx = np.mgrid[0:64, 0:128]
+    assert np.abs(f - np.exp(-2j * np.pi * (3 * ky / 64 + 5 * kx / 128))).max() < 1e-5
+
+
+@pytest.mark.parametrize("ny,nx,pitch,lam,z,band", [
+    (128, 256, 8e-6, 532e-9, 0.05, False),
+    (256, 256, 8e-6, 633e-9, 0.1, True),
+    (64, 128, 1e-6, 532e-9, 0.002, False),     # lambda/p large: FP64 transfer path
+])
+def test_asm_vs_oracle(ctx, ny, nx, pitch, lam, z, band):
+    print(f"case: asm {ny}x{nx} p={pitch} z={z} band={band}", flush=True)
+    rng = np.random.default_rng(7)
+    u = (rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))).astype(np.complex64)
+    out = ctx.asm(u, pitch, lam, z, pad=2, band_limit=band)
+    ref = O.asm(u.astype(np.complex128), pitch, pitch, lam, z, pad=2, band_limit=band)
+    err = rel_l2(out, ref)
+    print("rel-L2", err)
+    assert err < 1e-5
+
+
+def test_asm_pad1_and_errors(ctx):
+    rng = np.random.default_rng(8)
+    u = (rng.standard_normal((64, 64)) + 1j * rng.standard_normal((64, 64))).astype(np.complex64)
+    out = ctx.asm(u, 8e-6, 532e-9, 0.02, pad=1)
+    assert rel_l2(out, O.asm(u, 8e-6, 8e-6, 532e-9, 0.02, pad=1)) < 1e-5
+    import openhologram_amd as oph
+    with pytest.raises(oph.OphError):
+        ctx.asm(np.zeros((48, 64), np.complex64), 8e-6, 532e-9, 0.02, pad=2)   # 96 not a power of two
+    with pytest.raises(oph.OphError):
+        ctx.asm(u, 8e-6, 532e-9, 0.02, pad=3)
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_wrp_vs_oracle(ctx, mode):
+    print(f"case: wrp 256x256 mode {mode}", flush=True)
+    x, y, z, a, zw = scenes.wrp_points(3000, seed=5)
+    x, y = x * 0.1, y * 0.1   # keep most windows inside a 2 mm plane
+    out = ctx.wrp(256, 256, 8e-6, 532e-9, zw, x, y, z, a, mode=mode)
+    ref = O.wrp(256, 256, 8e-6, 532e-9, zw, x, y, z, a, mode=mode)
+    err = rel_l2(out, ref)
+    print("rel-L2", err)
+    assert err < 1e-5
+    # deterministic: a second call is bit-identical
+    out2 = ctx.wrp(256, 256, 8e-6, 532e-9, zw, x, y, z, a, mode=mode)
+    assert np.array_equal(out, out2)
+
+
+def test_wrp_edge_cases(ctx):
+    # empty set, a point outside the plane, unit amplitude, accumulate, ragged size
+    e = np.zeros(0)
+    assert np.all(ctx.wrp(64, 64, 8e-6, 532e-9, 0.0, e, e, e) == 0)
+    x, y, z = np.array([1.0, 2e-5]), np.array([1.0, -3e-5]), np.array([-0.003, 0.002])
+    out = ctx.wrp(100, 70, 8e-6, 532e-9, 0.0, x, y, z)
+    ref = O.wrp(100, 70, 8e-6, 532e-9, 0.0, x, y, z)
+    assert rel_l2(out, ref) < 1e-5
+    acc = ctx.wrp(100, 70, 8e-6, 532e-9, 0.0, x, y, z, out=out.copy(), accumulate=True)
+    assert rel_l2(acc, 2 * ref) < 1e-5
+
+
+def test_wrp_hologram_small(ctx):
+    """WRP + padded ASM at reduced size against the oracle chain."""
+    print("case: wrp_hologram 512 P=5000", flush=True)
+    x, y, z, a, zw = scenes.wrp_points(5000, seed=6)
+    x, y = x * 0.2, y * 0.2
+    d = -zw
+    out = ctx.wrp_hologram(512, 8e-6, 532e-9, zw, d, x, y, z, a, pad=2)
+    w = O.wrp(512, 512, 8e-6, 532e-9, zw, x, y, z, a)
+    ref = O.asm(w, 8e-6, 8e-6, 532e-9, d, pad=2)
+    err = rel_l2(out, ref)
+    print("rel-L2", err)
+    assert err < 1e-5
+
+
+def test_wrp_hologram_config2_full(ctx):
+    """The full WRP workload: 100,000 points, 2048^2 WRP, 100 mm ASM on a
+    4096^2 padded grid, complex64 out; rel-L2 vs the FP64 oracle <= 1e-3."""
+    print("case: wrp_hologram config2 full", flush=True)
+    x, y, z, a, zw = scenes.wrp_points(100_000, seed=2)
+    out = ctx.wrp_hologram(2048, 8e-6, 532e-9, zw, -zw, x, y, z, a, pad=2)
+    w = O.wrp(2048, 2048, 8e-6, 532e-9, zw, x, y, z, a)
+    ref = O.asm(w, 8e-6, 8e-6, 532e-9, -zw, pad=2)
+    err = rel_l2(out, ref)
+    print("rel-L2 config2", err)
+    assert err < 1e-3

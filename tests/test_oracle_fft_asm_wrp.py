@@ -1,0 +1,120 @@
+"""CPU: pins of the oracle's FFT, angular-spectrum propagation and WRP
+accumulation by mathematics (the reference's FFTW/cuFFT and CWO kernels are
+third-party or closed; see the oracle header)."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from oracle import oracle as O
+
+
+def test_fft_vs_numpy_and_dft():
+    rng = np.random.default_rng(1)
+    for ny, nx in [(32, 64), (128, 16), (256, 512)]:
+        a = rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))
+        assert rel_l2(O.fft2d(a, -1), np.fft.fft2(a)) < 1e-13
+        assert rel_l2(O.fft2d(a, +1, normalize=True), np.fft.ifft2(a)) < 1e-13
+    # brute-force O(N^2) DFT, independent of numpy's FFT
+    a = rng.standard_normal((8, 16)) + 1j * rng.standard_normal((8, 16))
+    ky, kx = np.arange(8)[:, None, None, None], np.arange(16)[None, :, None, None]
+    y, x = np.arange(8)[None, None, :, None], np.arange(16)[None, None, None, :]
+    W = np.exp(-2j * np.pi * (ky * y / 8 + kx * x / 16))
+    dft = (W * a[None, None]).sum(axis=(2, 3))
+    assert rel_l2(O.fft2d(a, -1), dft) < 1e-13
+    # Parseval
+    b = O.fft2d(a, -1)
+    assert abs(np.sum(abs(b) ** 2) / a.size - np.sum(abs(a) ** 2)) < 1e-9
+
+
+def test_asm_plane_wave_eigenfunction():
+    """Without padding a grid-periodic plane wave is an eigenfunction: the
+    output is H(fx, fy) times the input, H = exp(i 2 pi z sqrt(1/l^2 - f^2))."""
+    n, p, lam, z = 64, 8e-6, 532e-9, 0.05
+    kx0, ky0 = 5, -3
+    yy, xx = np.mgrid[0:n, 0:n]
+    u = np.exp(2j * np.pi * (kx0 * xx + ky0 * yy) / n)
+    out = O.asm(u, p, p, lam, z, pad=1)
+    fx, fy = kx0 / (n * p), ky0 / (n * p)
+    H = np.exp(2j * np.pi * z * np.sqrt(1 / lam ** 2 - fx ** 2 - fy ** 2))
+    assert np.abs(out - H * u).max() < 1e-9
+
+
+def test_asm_forward_back_identity_and_evanescent():
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal((64, 32)) + 1j * rng.standard_normal((64, 32))
+    v = O.asm(O.asm(u, 8e-6, 8e-6, 532e-9, 0.03, pad=1), 8e-6, 8e-6, 532e-9, -0.03, pad=1)
+    assert rel_l2(v, u) < 1e-12
+    # pitch below lambda/2: high frequencies are evanescent and removed
+    w = O.asm(u, 0.2e-6, 0.2e-6, 532e-9, 1e-6, pad=1)
+    spec = np.fft.fft2(w)
+    f = np.fft.fftfreq(64, 0.2e-6)[:, None] ** 2 + np.fft.fftfreq(32, 0.2e-6)[None, :] ** 2
+    assert np.abs(spec[f >= 1 / 532e-9 ** 2]).max() < 1e-9
+
+
+def test_asm_spherical_wave_propagation():
+    """Padded ASM carries the sampled spherical wave exp(ikr)/r of a point at
+    z1 to the field of the same point at z1 + dz (central region)."""
+    n, p, lam, z1, dz = 256, 8e-6, 532e-9, 0.1, 0.004
+    c = (np.arange(n) - n / 2 + 0.5) * p
+    X, Y = np.meshgrid(c, c)
+
+    def sph(z):
+        r = np.sqrt(X ** 2 + Y ** 2 + z ** 2)
+        t = (r / lam) % 1.0
+        return np.exp(2j * np.pi * t) / r
+    out = O.asm(sph(z1), p, p, lam, dz, pad=2)
+    ref = sph(z1 + dz)
+    m = slice(n // 4, 3 * n // 4)
+    assert rel_l2(out[m, m], ref[m, m]) < 2e-2
+    # a wrong sign of z would go the other way
+    bad = O.asm(sph(z1), p, p, lam, -dz, pad=2)
+    assert rel_l2(bad[m, m], ref[m, m]) > 0.5
+
+
+def test_wrp_single_point_closed_form():
+    n, p, lam = 64, 8e-6, 532e-9
+    x0, y0, z0, zw = 3.3e-5, -4.1e-5, -0.103, -0.1
+    out = O.wrp(n, n, p, lam, zw, [x0], [y0], [z0])
+    dz = zw - z0
+    w = int(abs(dz) * np.tan(np.arcsin(lam / (2 * p))) / p)
+    assert w == 12
+    tx, ty = int(x0 / p) + n // 2, int(y0 / p) + n // 2
+    nz = np.argwhere(out != 0)
+    assert nz[:, 0].min() == ty - w and nz[:, 0].max() == ty + w - 1
+    assert nz[:, 1].min() == tx - w and nz[:, 1].max() == tx + w - 1
+    L = np.longdouble
+    wy, wx = np.mgrid[-w:w, -w:w]
+    r = np.sqrt((wx * L(p)) ** 2 + (wy * L(p)) ** 2 + L(dz) ** 2)  # dz > 0: sign +
+    t = (r / L(lam)) % 1
+    ref = np.exp(2j * np.pi * t.astype(np.float64)) / (r.astype(np.float64) + 0.05)
+    assert np.abs(out[ty - w:ty + w, tx - w:tx + w] - ref).max() < 1e-9  # |v| ~ 20
+
+
+def test_wrp_modes_sign_and_clipping():
+    n, p, lam = 48, 8e-6, 532e-9
+    # mode 1 half-width of the reference, WRP.cpp:111
+    dz = 0.002
+    w1 = int(abs(lam * dz / p / p / 2 + 0.5)) * 2 - 1
+    out = O.wrp(n, n, p, lam, 0.0, [0.0], [0.0], [-dz], mode=1)
+    nz = np.argwhere(out != 0)
+    assert nz[:, 0].max() - nz[:, 0].min() + 1 == min(2 * w1, n)
+    # negative dz: r is negative -> exp(-ik|r|)/(0.05 - |r|)
+    o2 = O.wrp(n, n, p, lam, 0.0, [0.0], [0.0], [+0.001])
+    v = o2[n // 2, n // 2]
+    t = (-0.001 / lam) % 1
+    assert abs(v - np.exp(2j * np.pi * t) / (0.05 - 0.001)) < 1e-9
+    # a point near the corner is clipped, not wrapped
+    o3 = O.wrp(n, n, p, lam, 0.0, [-(n // 2 - 2) * p], [-(n // 2 - 2) * p], [-0.004])
+    nz = np.argwhere(o3 != 0)
+    assert nz.min() == 0 and nz[:, 0].max() < n // 2
+
+
+def test_wrp_linearity():
+    rng = np.random.default_rng(3)
+    x, y = rng.uniform(-1e-4, 1e-4, (2, 30))
+    z = rng.uniform(-0.106, -0.101, 30)
+    a = rng.uniform(0, 1, 30)
+    f = O.wrp(64, 64, 8e-6, 532e-9, -0.1, x, y, z, a)
+    g = O.wrp(64, 64, 8e-6, 532e-9, -0.1, x[:11], y[:11], z[:11], a[:11]) + \
+        O.wrp(64, 64, 8e-6, 532e-9, -0.1, x[11:], y[11:], z[11:], a[11:])
+    assert rel_l2(f, g) < 1e-13
