@@ -122,7 +122,7 @@ def _time_steps(ctx, step, args, kernels):
     return min(wall, wall_noprof), per
 
 
-WRP_KERNELS = ["wrp_zero", "wrp_prep", "wrp_scan", "wrp_fill", "wrp_gather", "asm_axis_table",
+WRP_KERNELS = ["wrp_bin_count", "wrp_bin_colscan", "wrp_scan", "wrp_bin_fill", "wrp_gather", "asm_axis_table",
                "asm_rows_fwd", "asm_cols", "asm_rows_inv"]
 
 
@@ -161,6 +161,18 @@ def bench_wrp(ctx, args):
                         "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": None,
                         "algorithmic_bytes": byt[dom]}
     pairs = 0
+    # a plain device-to-device copy of the padded work-buffer size: the
+    # achievable copy rate on this device (and the calibration kernel for the
+    # rocprof FETCH_SIZE / WRITE_SIZE counters)
+    src = ctx.empty((n_side * pad, n_side * pad), np.complex64)
+    dst = ctx.empty((n_side * pad, n_side * pad), np.complex64)
+    ctx.copy(dst, src)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        ctx.copy(dst, src)
+    tc = (time.perf_counter() - t0) / 5
+    line["d2d_copy_gbs"] = 2 * src.nbytes / tc / 1e9
+    del src, dst
     line["hbm_gbs_asm_total"] = sum(byt.values()) / (sum(per[k]["avg_us"] for k in byt if k in per) * 1e-6) / 1e9
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_wrp(x, y, z, a, zw, n_side, pitch, lam, dist, pad)

@@ -68,6 +68,8 @@ int oph_device_free(oph_ctx* ctx, void* dptr);
 int oph_copy_to_device(oph_ctx* ctx, void* dst, const void* src, size_t bytes);
 int oph_copy_to_host(oph_ctx* ctx, void* dst, const void* src, size_t bytes);
 int oph_device_zero(oph_ctx* ctx, void* dptr, size_t bytes);
+/* Device-to-device copy on the context stream (synchronous). */
+int oph_device_copy(oph_ctx* ctx, void* dst, const void* src, size_t bytes);
 /* HIP events recorded on the context's stream around whatever the caller
  * launches between begin and end; end waits and returns milliseconds. */
 int oph_timer_begin(oph_ctx* ctx);

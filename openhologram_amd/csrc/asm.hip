@@ -94,15 +94,17 @@ __global__ __launch_bounds__(512) void asm_rows_fwd(const float2* __restrict__ i
     float2* buf = lds + sub * PL;
     const bool ok = r < ny;
     const float2* src = in + (size_t)(ok ? r : 0) * nx;
-    for (int i = t; i < L; i += TPL) {
-        const int c = i - ox;
-        buf[fft::pad16(i)] = (ok && c >= 0 && c < nx) ? src[c] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int c = t + m * TPL - ox;
+        buf[fft::lane_pad<L>(t, m)] = (ok && c >= 0 && c < nx) ? src[c] : make_float2(0.f, 0.f);
     }
     __syncthreads();
     fft::line<L, -1>(buf, tw, t);
     if (ok) {
         float2* dst = work + (size_t)(oy + r) * L;
-        for (int i = t; i < L; i += TPL) dst[i] = buf[fft::pad16(i)];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) dst[t + m * TPL] = buf[fft::lane_pad<L>(t, m)];
     }
 }
 
@@ -130,9 +132,12 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
     fft::line<L, -1>(line, tw, t);
     const int c = cbase + b;
     const float4 ax = tabx[c < npx ? c : 0];
-    for (int i = t; i < L; i += TPL) {
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int i = t + m * TPL;
         const float2 h = transfer<EXACT>(ax, taby[i], c, i, k);
-        line[fft::pad16(i)] = fft::cmul(line[fft::pad16(i)], h);
+        float2* e = line + fft::lane_pad<L>(t, m);
+        *e = fft::cmul(*e, h);
     }
     __syncthreads();
     fft::line<L, +1>(line, tw, t);
@@ -156,7 +161,8 @@ __global__ __launch_bounds__(512) void asm_rows_inv(const float2* __restrict__ w
     float2* buf = lds + sub * PL;
     const bool ok = r < ny;
     const float2* src = work + (size_t)(oy + (ok ? r : 0)) * L;
-    for (int i = t; i < L; i += TPL) buf[fft::pad16(i)] = ok ? src[i] : make_float2(0.f, 0.f);
+#pragma unroll
+    for (int m = 0; m < 16; ++m) buf[fft::lane_pad<L>(t, m)] = ok ? src[t + m * TPL] : make_float2(0.f, 0.f);
     __syncthreads();
     fft::line<L, +1>(buf, tw, t);
     if (ok) {
@@ -188,12 +194,9 @@ int launch_rows_inv(oph_ctx* ctx, const float2* work, int nx, int ny, int ox, in
     return OPH_OK;
 }
 
-template <int LY, bool EXACT>
-int launch_cols_ex(oph_ctx* ctx, float2* work, int npx, int ny, int oy, const float4* tx, const float4* ty,
-                   const AsmConst& k) {
-    const float2* tw = oph_twiddles(ctx, LY);
-    if (!tw) return OPH_ENOMEM;
-    constexpr int B = fftk::col_lines(LY);
+template <int LY, int B, bool EXACT>
+int launch_cols_b(oph_ctx* ctx, float2* work, int npx, int ny, int oy, const float4* tx, const float4* ty,
+                  const AsmConst& k, const float2* tw) {
     const size_t shm = (size_t)B * fft::padded_len(LY) * sizeof(float2);
     static bool attr = false;
     if (!attr) {
@@ -205,6 +208,31 @@ int launch_cols_ex(oph_ctx* ctx, float2* work, int npx, int ny, int oy, const fl
     OPH_LAUNCH(ctx, "asm_cols", (asm_cols<LY, B, EXACT>), dim3((npx + B - 1) / B), dim3(B * (LY / 16)), shm,
                work, npx, ny, oy, tx, ty, k, tw);
     return OPH_OK;
+}
+
+template <int LY, bool EXACT>
+int launch_cols_ex(oph_ctx* ctx, float2* work, int npx, int ny, int oy, const float4* tx, const float4* ty,
+                   const AsmConst& k) {
+    const float2* tw = oph_twiddles(ctx, LY);
+    if (!tw) return OPH_ENOMEM;
+    constexpr int BMAX = fftk::col_lines(LY);
+    int b = fftk::col_width(LY, BMAX);
+    switch (b) {
+        case 1: return launch_cols_b<LY, 1, EXACT>(ctx, work, npx, ny, oy, tx, ty, k, tw);
+        case 2:
+            if constexpr (BMAX >= 2) return launch_cols_b<LY, 2, EXACT>(ctx, work, npx, ny, oy, tx, ty, k, tw);
+            break;
+        case 4:
+            if constexpr (BMAX >= 4) return launch_cols_b<LY, 4, EXACT>(ctx, work, npx, ny, oy, tx, ty, k, tw);
+            break;
+        case 8:
+            if constexpr (BMAX >= 8) return launch_cols_b<LY, 8, EXACT>(ctx, work, npx, ny, oy, tx, ty, k, tw);
+            break;
+        default:
+            if constexpr (BMAX >= 16) return launch_cols_b<LY, 16, EXACT>(ctx, work, npx, ny, oy, tx, ty, k, tw);
+            break;
+    }
+    return launch_cols_b<LY, BMAX, EXACT>(ctx, work, npx, ny, oy, tx, ty, k, tw);
 }
 
 template <int LY>

@@ -11,6 +11,7 @@
 // context.  The plan is implicit (length -> kernel instance); nothing is
 // created or destroyed per call.
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "fft_engine.h"
@@ -24,18 +25,28 @@ const float2* oph_twiddles(oph_ctx* ctx, int L) {
         return nullptr;
     }
     if (ctx->tw[lg]) return static_cast<const float2*>(ctx->tw[lg]);
-    std::vector<float2> h(L);
-    for (int m = 0; m < L; ++m) {
-        const double a = -2.0 * M_PI * (double)m / (double)L;
-        h[m] = make_float2((float)cos(a), (float)sin(a));
+    // per-stage tables in the order fft::line walks the stages
+    std::vector<float2> h;
+    int Ns = 1, rem = L;
+    while (rem > 1) {
+        const int R = (rem % 16 == 0) ? 16 : rem;
+        if (Ns > 1)
+            for (int r = 1; r < R; ++r)
+                for (int k = 0; k < Ns; ++k) {
+                    const double a = -2.0 * M_PI * (double)k * r / ((double)Ns * R);
+                    h.push_back(make_float2((float)cos(a), (float)sin(a)));
+                }
+        Ns *= R;
+        rem /= R;
     }
+    if (h.empty()) h.push_back(make_float2(1.f, 0.f));
     void* d = nullptr;
-    if (hipMalloc(&d, sizeof(float2) * L) != hipSuccess) {
+    if (hipMalloc(&d, sizeof(float2) * h.size()) != hipSuccess) {
         hipGetLastError();
         oph_fail(ctx, OPH_ENOMEM, "twiddle table allocation failed");
         return nullptr;
     }
-    if (hipMemcpy(d, h.data(), sizeof(float2) * L, hipMemcpyHostToDevice) != hipSuccess) {
+    if (hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
         hipGetLastError();
         hipFree(d);
         oph_fail(ctx, OPH_EDEVICE, "twiddle table upload failed");
@@ -43,6 +54,18 @@ const float2* oph_twiddles(oph_ctx* ctx, int L) {
     }
     ctx->tw[lg] = d;
     return static_cast<const float2*>(d);
+}
+
+int fftk::col_width(int L, int bmax) {
+    static int env = -1;
+    if (env < 0) {
+        const char* e = getenv("OPH_FFT_COLUMNS");
+        env = e ? atoi(e) : 0;
+    }
+    int b = env > 0 ? env : (L >= 4096 ? 2 : (L >= 2048 ? 4 : 8));
+    if (b > bmax) b = bmax;
+    if (b < 1) b = 1;
+    return b;
 }
 
 namespace {
@@ -62,14 +85,14 @@ __global__ __launch_bounds__(512) void fft_rows_kernel(float2* data, int rows, i
     const bool ok = row < rows;
     float2* src = data + (size_t)(ok ? row : 0) * stride;
 #pragma unroll
-    for (int i = t; i < L; i += TPL) buf[fft::pad16(i)] = ok ? src[i] : make_float2(0.f, 0.f);
+    for (int m = 0; m < 16; ++m) buf[fft::lane_pad<L>(t, m)] = ok ? src[t + m * TPL] : make_float2(0.f, 0.f);
     __syncthreads();
     fft::line<L, SIGN>(buf, tw, t);
     if (ok) {
 #pragma unroll
-        for (int i = t; i < L; i += TPL) {
-            const float2 v = buf[fft::pad16(i)];
-            src[i] = make_float2(v.x * scale, v.y * scale);
+        for (int m = 0; m < 16; ++m) {
+            const float2 v = buf[fft::lane_pad<L>(t, m)];
+            src[t + m * TPL] = make_float2(v.x * scale, v.y * scale);
         }
     }
 }

@@ -137,17 +137,42 @@ __host__ __device__ constexpr int n16(int L) {
 __device__ __forceinline__ int pad16(int i) { return i + (i >> 4); }
 __host__ __device__ constexpr int padded_len(int L) { return L + L / 16; }
 
-// One Stockham stage of radix R over a line in LDS (padded indexing).
+// Twiddles of stage s (Ns = product of the earlier radices, radix R) are
+// stored as tws[(r-1) * Ns + k] = exp(-2 pi i k r / (Ns R)), k < Ns,
+// r = 1..R-1, so that the lanes of a wave, which hold consecutive k, read
+// consecutive entries.  The tables of all stages of one length are
+// concatenated; stage 0 (Ns = 1) has none.
+
+// One Stockham stage of radix R over a line in LDS (padded indexing).  The
+// stage's twiddles are loaded first so that their latency overlaps the LDS
+// reads and the barrier.
 template <int L, int R, int SIGN, int TPL>
-__device__ __forceinline__ void stage(float2* buf, const float2* __restrict__ tw, int Ns, int t) {
+__device__ __forceinline__ void stage(float2* buf, const float2* __restrict__ tws, int Ns, int t) {
     constexpr int NB = L / R;
     constexpr int PER = NB / TPL;  // butterflies per thread (TPL divides NB)
+    float2 w[PER][R];
+    if (Ns > 1) {
+#pragma unroll
+        for (int p = 0; p < PER; ++p) {
+            const int k = (t + p * TPL) & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < R; ++r) w[p][r] = tws[(r - 1) * Ns + k];
+        }
+    }
     float2 v[PER][R];
+    // padded index of j + r*NB is pad16(j) + r*(NB + NB/16) when 16 | NB:
+    // the per-r part folds into the LDS instruction's immediate offset
 #pragma unroll
     for (int p = 0; p < PER; ++p) {
         const int j = t + p * TPL;
+        if constexpr (NB % 16 == 0) {
+            const float2* src = buf + pad16(j);
 #pragma unroll
-        for (int r = 0; r < R; ++r) v[p][r] = buf[pad16(j + r * NB)];
+            for (int r = 0; r < R; ++r) v[p][r] = src[r * (NB + NB / 16)];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[p][r] = buf[pad16(j + r * NB)];
+        }
     }
     __syncthreads();
 #pragma unroll
@@ -155,39 +180,59 @@ __device__ __forceinline__ void stage(float2* buf, const float2* __restrict__ tw
         const int j = t + p * TPL;
         const int k = j & (Ns - 1);
         if (Ns > 1) {
-            const int m = k * (L / (Ns * R));
 #pragma unroll
-            for (int r = 1; r < R; ++r) {
-                const float2 w = tw[m * r];
-                v[p][r] = SIGN < 0 ? cmul(v[p][r], w) : cmulc(v[p][r], w);
-            }
+            for (int r = 1; r < R; ++r)
+                v[p][r] = SIGN < 0 ? cmul(v[p][r], w[p][r]) : cmulc(v[p][r], w[p][r]);
         }
         Dft<R, SIGN>::run(v[p]);
         const int base = (j - k) * R + k;
+        if (Ns == 1) {
+            // base = R j: pad16(R j + r) = pad16(R j) + r for r < R <= 16
+            float2* dst = buf + pad16(base);
 #pragma unroll
-        for (int r = 0; r < R; ++r) buf[pad16(base + r * Ns)] = v[p][r];
+            for (int r = 0; r < R; ++r) dst[r] = v[p][r];
+        } else if ((Ns & 15) == 0) {
+            float2* dst = buf + pad16(base);
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r * (Ns + Ns / 16)] = v[p][r];
+        } else {
+#pragma unroll
+            for (int r = 0; r < R; ++r) buf[pad16(base + r * Ns)] = v[p][r];
+        }
     }
     __syncthreads();
 }
 
+// Element i = t + m*TPL of a line, m = 0..15, padded: pad16(t) + m*(TPL + TPL/16)
+// when 16 | TPL (used by the kernels' global<->LDS copies).
+template <int L>
+__device__ __forceinline__ int lane_pad(int t, int m) {
+    constexpr int TPL = L / 16;
+    if constexpr (TPL % 16 == 0) return pad16(t) + m * (TPL + TPL / 16);
+    else return pad16(t + m * TPL);
+}
+
 // Full transform of one LDS line by TPL = L/16 threads (t in [0, TPL)).
 // Caller syncs before (line loaded) and the last stage ends with a barrier.
+// tw: the concatenated per-stage tables of length L (oph_twiddles).
 template <int L, int SIGN>
 __device__ __forceinline__ void line(float2* buf, const float2* __restrict__ tw, int t) {
     constexpr int TPL = L / 16;
     constexpr int A = n16(L);
     constexpr int RL = last_radix(L);
-    int Ns = 1;
+    int Ns = 1, off = 0;
 #pragma unroll
     for (int s = 0; s < A; ++s) {
-        stage<L, 16, SIGN, TPL>(buf, tw, Ns, t);
+        stage<L, 16, SIGN, TPL>(buf, tw + off, Ns, t);
+        if (Ns > 1) off += Ns * 15;
         Ns *= 16;
     }
-    if (RL > 1) stage<L, RL, SIGN, TPL>(buf, tw, Ns, t);
+    if (RL > 1) stage<L, RL, SIGN, TPL>(buf, tw + off, Ns, t);
 }
 
 }  // namespace fft
 
-// Device twiddle table of length L (exp(-2 pi i m / L), FP64-computed), cached
-// per length in the context.  Returns nullptr on failure (error recorded).
+// Device twiddle tables of all stages of a length-L transform (layout above,
+// FP64-computed, rounded to float), cached per length in the context.
+// Returns nullptr on failure (error recorded).
 const float2* oph_twiddles(oph_ctx* ctx, int L);

@@ -21,6 +21,10 @@ __host__ __device__ constexpr int col_lines(int L) {
 
 inline bool supported(int n) { return n >= 16 && n <= 8192 && (n & (n - 1)) == 0; }
 
+// Columns per workgroup actually used for a length-L column pass: the
+// default below, or OPH_FFT_COLUMNS (a power of two) when set, capped at bmax.
+int col_width(int L, int bmax);
+
 }  // namespace fftk
 
 // In-place batched transforms on device data (row-major, `stride` elements

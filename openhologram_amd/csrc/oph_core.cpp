@@ -199,6 +199,14 @@ int oph_copy_to_host(oph_ctx* ctx, void* dst, const void* src, size_t bytes) {
     return OPH_OK;
 }
 
+int oph_device_copy(oph_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    int rc = oph_enter(ctx);
+    if (rc) return rc;
+    OPH_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    OPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return OPH_OK;
+}
+
 int oph_device_zero(oph_ctx* ctx, void* dptr, size_t bytes) {
     int rc = oph_enter(ctx);
     if (rc) return rc;

@@ -12,16 +12,21 @@
 // (int)|lambda dz/(2p^2) + 0.5| * 2 - 1 (WRP.cpp:111).
 //
 // The reference scatters with a racing AddPixel; here the plane is gathered
-// per 64x64 tile:
-//   1. wrp_prep   one thread per point: centre, window, FP64-reduced phase
-//                 constants; counts the tiles each window touches;
-//   2. scan       tile offsets;
-//   3. wrp_fill   point indices into per-tile lists;
-//   4. wrp_gather one workgroup per tile: each wave takes points of the list
-//                 and spreads the window-tile intersection over its lanes;
-//                 contributions are summed in LDS as 64-bit fixed point
-//                 (integer adds are associative, so the result does not
-//                 depend on the list order: deterministic without sorting).
+// per 64x64 tile, with no global atomics:
+//   1. wrp_bin_count   per chunk of 1024 points: the point records (centre,
+//                      window, FP64-reduced phase constants) and per-tile
+//                      counts in LDS;
+//   2. wrp_bin_colscan per tile: prefix of the counts over chunks;
+//   3. wrp_scan        tile offsets;
+//   4. wrp_bin_fill    per chunk: point indices into the tile lists;
+//   5. wrp_gather      one workgroup per tile: each wave takes points of the
+//                      list; a window value depends on (|wx|, |wy|) only, so
+//                      each is evaluated once and added to its mirrored
+//                      pixels; contributions are summed in LDS as 32-bit
+//                      fixed point scaled per tile so the sum cannot
+//                      overflow (integer adds are associative, so the result
+//                      does not depend on the list order: deterministic
+//                      without sorting).
 // Per pixel, with u = (wx^2 + wy^2) p^2 / dz^2 (<= 2 tan^2 ~ 2.2e-3 for the
 // grating-angle window):
 //   (r - |dz|)/|dz| = P(u) = u/2 - u^2/8 + u^3/16 - 5u^4/128,
@@ -57,23 +62,24 @@ struct WrpArgs {
     const double* z;
     const double* amp;
     int64_t n;
-    int nx, ny, ntx, nty;
+    int nx, ny, ntx, nty, ntiles;
     double pitch, inv_lambda, z_wrp, grating, lambda;
     int mode;
     WrpPoint* pts;
-    unsigned* counts;     // [ntiles]
-    unsigned* offsets;    // [ntiles + 1]
-    unsigned* cursor;     // [ntiles]
+    unsigned* chunk_counts;  // [ntiles][nchunks]: counts, then exclusive prefix over chunks
+    int nchunks;
+    unsigned* totals;        // [ntiles]
+    unsigned* offsets;       // [ntiles + 1]
+    float* chunk_max;        // [nchunks] max |a/D|
+    float* gmax;             // [1]
     unsigned* list;
     unsigned capacity;
     unsigned* overflow;
-    unsigned* maxbits;    // float bits of max |a/D|
 };
 
-__global__ void wrp_zero(unsigned* p, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0u;
-}
+constexpr int CHUNK = 1024;          // points per binning workgroup
+constexpr int BIN_THREADS = 256;
+constexpr int MAX_TILES = 16384;     // LDS counters of one binning workgroup
 
 __device__ __forceinline__ void tile_range(const WrpPoint& q, const WrpArgs& a, int& x0, int& x1, int& y0,
                                            int& y1) {
@@ -92,9 +98,7 @@ __device__ __forceinline__ void tile_range(const WrpPoint& q, const WrpArgs& a, 
     if (lo > hi) y1 = y0 - 1;
 }
 
-__global__ void wrp_prep(WrpArgs a) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
+__device__ WrpPoint make_point(const WrpArgs& a, int64_t i) {
     const double dz = a.z_wrp - a.z[i];
     WrpPoint q;
     q.tx = (int)(a.x[i] / a.pitch) + (a.nx >> 1);
@@ -118,19 +122,76 @@ __global__ void wrp_prep(WrpArgs a) {
     q.pad0 = q.pad1 = q.pad2 = 0.f;
     q.dz = dz;
     q.pad3 = 0.0;
-    // series error bound at the window corner: (|dz|/lambda) * 7/256 u^5
+    // series error at the window corner: (|dz|/lambda) 7/256 u^5 turns, and
+    // the amplitude's dropped second-order term (cd P)^2
     const double umax = 2.0 * (double)q.w * q.w * (adz > 0.0 ? a.pitch * a.pitch / (dz * dz) : 0.0);
-    q.exact = (umax > 0.05 || adz * a.inv_lambda * 7.0 / 256.0 * pow(umax, 5) > 1e-7 ||
-               fabs(q.cd) * 0.5 * umax > 1e-3) ? 1 : 0;
-    a.pts[i] = q;
-    if (q.w > 0) {
-        const float mag = fabsf(q.ampd);
-        atomicMax(a.maxbits, __float_as_uint(mag));
+    const double u5 = umax * umax * umax * umax * umax;
+    q.exact = (umax > 0.05 || adz * a.inv_lambda * (7.0 / 256.0) * u5 > 1e-7 || fabs(q.cd) * 0.5 * umax > 1e-3)
+                  ? 1
+                  : 0;
+    return q;
+}
+
+// pass 1: per chunk of points, the point records and per-tile counts
+__global__ __launch_bounds__(BIN_THREADS) void wrp_bin_count(WrpArgs a) {
+    __shared__ unsigned cnt[MAX_TILES];
+    __shared__ float wmax[BIN_THREADS / 64];
+    for (int t = threadIdx.x; t < a.ntiles; t += BIN_THREADS) cnt[t] = 0u;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * CHUNK;
+    float mx = 0.f;
+    for (int k = threadIdx.x; k < CHUNK; k += BIN_THREADS) {
+        const int64_t i = base + k;
+        if (i >= a.n) break;
+        const WrpPoint q = make_point(a, i);
+        a.pts[i] = q;
+        if (q.w <= 0) continue;
+        mx = fmaxf(mx, fabsf(q.ampd));
         int x0, x1, y0, y1;
         tile_range(q, a, x0, x1, y0, y1);
         for (int ty = y0; ty <= y1; ++ty)
-            for (int tx = x0; tx <= x1; ++tx) atomicAdd(&a.counts[ty * a.ntx + tx], 1u);
+            for (int tx = x0; tx <= x1; ++tx) atomicAdd(&cnt[ty * a.ntx + tx], 1u);
     }
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    for (int t = threadIdx.x; t < a.ntiles; t += BIN_THREADS)
+        a.chunk_counts[(size_t)t * a.nchunks + blockIdx.x] = cnt[t];
+    if (threadIdx.x == 0) {
+        float m = 0.f;
+        for (int w = 0; w < BIN_THREADS / 64; ++w) m = fmaxf(m, wmax[w]);
+        a.chunk_max[blockIdx.x] = m;
+    }
+}
+
+// pass 2: per tile (one wave each), exclusive prefix of the chunk counts and
+// the tile total; the first wave also reduces the chunk maxima
+__global__ __launch_bounds__(256) void wrp_bin_colscan(WrpArgs a) {
+    const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+    if (wave == 0) {
+        float m = 0.f;
+        for (int c = lane; c < a.nchunks; c += 64) m = fmaxf(m, a.chunk_max[c]);
+        for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off));
+        if (lane == 0) {
+            *a.gmax = m;
+            *a.overflow = 0u;
+        }
+    }
+    if (wave >= a.ntiles) return;
+    unsigned* p = a.chunk_counts + (size_t)wave * a.nchunks;
+    unsigned run = 0;
+    for (int c0 = 0; c0 < a.nchunks; c0 += 64) {
+        const int c = c0 + lane;
+        const unsigned v = c < a.nchunks ? p[c] : 0u;
+        unsigned incl = v;
+        for (int off = 1; off < 64; off <<= 1) {
+            const unsigned o = __shfl_up(incl, off);
+            if (lane >= off) incl += o;
+        }
+        if (c < a.nchunks) p[c] = run + incl - v;
+        run += __shfl(incl, 63);
+    }
+    if (lane == 0) a.totals[wave] = run;
 }
 
 // exclusive scan of counts into offsets[0..n], one workgroup of 1024 threads
@@ -157,29 +218,39 @@ __global__ __launch_bounds__(1024) void wrp_scan(const unsigned* counts, unsigne
     if (t == 1023) offsets[n] = part[1023];
 }
 
-__global__ void wrp_fill(WrpArgs a) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    const WrpPoint q = a.pts[i];
-    if (q.w <= 0) return;
-    int x0, x1, y0, y1;
-    tile_range(q, a, x0, x1, y0, y1);
-    for (int ty = y0; ty <= y1; ++ty)
-        for (int tx = x0; tx <= x1; ++tx) {
-            const int tile = ty * a.ntx + tx;
-            const unsigned slot = a.offsets[tile] + atomicAdd(&a.cursor[tile], 1u);
-            if (slot < a.capacity)
-                a.list[slot] = (unsigned)i;
-            else
-                atomicOr(a.overflow, 1u);
-        }
+// pass 3: per chunk, point indices into the tile lists (slot = tile offset +
+// chunk prefix + rank within the chunk from an LDS counter)
+__global__ __launch_bounds__(BIN_THREADS) void wrp_bin_fill(WrpArgs a) {
+    __shared__ unsigned cur[MAX_TILES];
+    for (int t = threadIdx.x; t < a.ntiles; t += BIN_THREADS) cur[t] = 0u;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * CHUNK;
+
+    for (int k = threadIdx.x; k < CHUNK; k += BIN_THREADS) {
+        const int64_t i = base + k;
+        if (i >= a.n) break;
+        const WrpPoint q = a.pts[i];
+        if (q.w <= 0) continue;
+        int x0, x1, y0, y1;
+        tile_range(q, a, x0, x1, y0, y1);
+        for (int ty = y0; ty <= y1; ++ty)
+            for (int tx = x0; tx <= x1; ++tx) {
+                const int tile = ty * a.ntx + tx;
+                const unsigned slot = a.offsets[tile] + a.chunk_counts[(size_t)tile * a.nchunks + blockIdx.x] +
+                                      atomicAdd(&cur[tile], 1u);
+                if (slot < a.capacity)
+                    a.list[slot] = (unsigned)i;
+                else
+                    *a.overflow = 1u;
+            }
+    }
 }
 
 struct GatherArgs {
     const WrpPoint* pts;
     const unsigned* offsets;
     const unsigned* list;
-    const unsigned* maxbits;
+    const float* gmax;
     unsigned capacity;
     int nx, ny, ntx;
     double pitch, inv_lambda;
@@ -187,74 +258,145 @@ struct GatherArgs {
     int accumulate;
 };
 
+// |w| range [lo, hi] covered by window offsets w in [w0, w1)
+__device__ __forceinline__ void abs_range(int w0, int w1, int& lo, int& hi) {
+    if (w0 >= 0) {
+        lo = w0;
+        hi = w1 - 1;
+    } else if (w1 <= 1) {
+        lo = 1 - w1;
+        hi = -w0;
+    } else {
+        lo = 0;
+        hi = max(-w0, w1 - 1);
+    }
+}
+
+// Point record as the gather stages it in LDS (32 bytes).
+struct GRec {
+    short tx, ty, w, exact;
+    int idx;
+    float f0, ks, p2dz2, ampd, cd;
+};
+constexpr int REC_BATCH = 128;
+
+// The window value depends on (|wx|, |wy|) only: each (ax, ay) is evaluated
+// once and added to the up-to-four mirrored pixels (+-ax, +-ay) that lie in
+// this tile's part of the window.  Sums are 32-bit fixed point with a scale
+// chosen per tile so that (list length) x (largest contribution) fits in
+// 2^30: exact integer adds, independent of the list order.
 __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
-    __shared__ unsigned long long acc[2 * TILE * TILE];
+    __shared__ int acc_re[TILE * TILE];
+    __shared__ int acc_im[TILE * TILE];
+    __shared__ GRec recs[REC_BATCH];
     const int tile = blockIdx.x;
     const int tyi = tile / g.ntx, txi = tile % g.ntx;
     const int X0 = txi * TILE, Y0 = tyi * TILE;
-    for (int i = threadIdx.x; i < 2 * TILE * TILE; i += GATHER_THREADS) acc[i] = 0ull;
-    // fixed-point scale: |contribution| * 2^S <= 2^30 (int32 per contribution)
-    const float mx = __uint_as_float(*g.maxbits);
-    const int S = mx > 0.f ? max(-60, min(60, 30 - (int)ceilf(log2f(mx)))) : 30;
-    const float scale = ldexpf(1.f, S);
-    __syncthreads();
+    for (int i = threadIdx.x; i < TILE * TILE; i += GATHER_THREADS) acc_re[i] = acc_im[i] = 0;
     unsigned b = g.offsets[tile], e = g.offsets[tile + 1];
     if (e > g.capacity) e = g.capacity;
     if (b > e) b = e;
+    const float mx = *g.gmax;
+    const float bound = mx * (float)max(e - b, 1u);
+    const int S = bound > 0.f ? max(-60, min(60, 30 - (int)ceilf(log2f(bound)))) : 30;
+    const float scale = ldexpf(1.f, S);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     constexpr int NW = GATHER_THREADS / 64;
-    for (unsigned k = b + wave; k < e; k += NW) {
-        const WrpPoint q = g.pts[g.list[k]];
-        const int cx0 = max(q.tx - q.w, max(X0, 0)), cx1 = min(q.tx + q.w, min(X0 + TILE, g.nx));
-        const int cy0 = max(q.ty - q.w, max(Y0, 0)), cy1 = min(q.ty + q.w, min(Y0 + TILE, g.ny));
-        const int wd = cx1 - cx0, ht = cy1 - cy0;
-        if (wd <= 0 || ht <= 0) continue;
-        const int cnt = wd * ht;
-        const float inv_wd = 1.f / (float)wd;
-        for (int qq = lane; qq < cnt; qq += 64) {
-            const int row = (int)(((float)qq + 0.5f) * inv_wd);
-            const int col = qq - row * wd;
-            const int wx = cx0 + col - q.tx, wy = cy0 + row - q.ty;
-            float re, im;
-            if (!q.exact) {
-                const float u = (float)(wx * wx + wy * wy) * q.p2dz2;
-                const float p = u * fmaf(u, fmaf(u, fmaf(u, -5.f / 128.f, 1.f / 16.f), -0.125f), 0.5f);
-                const float ph = fmaf(q.ks, p, q.f0);
-                const float am = q.ampd * fmaf(q.cd, p, 1.f);
-                re = am * cos_turns(ph);
-                im = am * sin_turns(ph);
-            } else {
-                const double dx = wx * g.pitch, dy = wy * g.pitch;
-                const double sgn = q.dz > 0.0 ? 1.0 : -1.0;
-                const double r = sgn * sqrt(dx * dx + dy * dy + q.dz * q.dz);
-                double t = r * g.inv_lambda;
-                t -= rint(t);
-                // amplitude a/(r + 0.05) = (a/D) * D/(r + 0.05), D = sign|dz| + 0.05
-                const double D = sgn * fabs(q.dz) + 0.05;
-                const float am = (float)((double)q.ampd * D / (r + 0.05));
-                re = am * cos_turns((float)t);
-                im = am * sin_turns((float)t);
+    for (unsigned bs = b; bs < e; bs += REC_BATCH) {
+        const int cnt_b = (int)min((unsigned)REC_BATCH, e - bs);
+        __syncthreads();
+        if ((int)threadIdx.x < cnt_b) {
+            const unsigned idx = g.list[bs + threadIdx.x];
+            const WrpPoint q = g.pts[idx];
+            GRec r;
+            r.tx = (short)q.tx;
+            r.ty = (short)q.ty;
+            r.w = (short)q.w;
+            r.exact = (short)q.exact;
+            r.idx = (int)idx;
+            r.f0 = q.f0;
+            r.ks = q.ks;
+            r.p2dz2 = q.p2dz2;
+            r.ampd = q.ampd;
+            r.cd = q.cd;
+            recs[threadIdx.x] = r;
+        }
+        __syncthreads();
+        for (int kk = wave; kk < cnt_b; kk += NW) {
+            const GRec q = recs[kk];
+            const int tx = q.tx, ty = q.ty, w = q.w;
+            // window offsets of this tile's part: wx in [wx0, wx1), wy in [wy0, wy1)
+            const int wx0 = max(tx - w, max(X0, 0)) - tx, wx1 = min(tx + w, min(X0 + TILE, g.nx)) - tx;
+            const int wy0 = max(ty - w, max(Y0, 0)) - ty, wy1 = min(ty + w, min(Y0 + TILE, g.ny)) - ty;
+            if (wx1 <= wx0 || wy1 <= wy0) continue;
+            int ax0, ax1, ay0, ay1;
+            abs_range(wx0, wx1, ax0, ax1);
+            abs_range(wy0, wy1, ay0, ay1);
+            const int na = ax1 - ax0 + 1;
+            const int cnt = na * (ay1 - ay0 + 1);
+            const float inv_na = 1.f / (float)na;
+            const int pbase = (ty - Y0) * TILE + (tx - X0);
+            for (int qq = lane; qq < cnt; qq += 64) {
+                const int iy = (int)(((float)qq + 0.5f) * inv_na);
+                const int ax = ax0 + (qq - iy * na), ay = ay0 + iy;
+                float re, im;
+                if (!q.exact) {
+                    const float u = (float)(ax * ax + ay * ay) * q.p2dz2;
+                    const float p = u * fmaf(u, fmaf(u, fmaf(u, -5.f / 128.f, 1.f / 16.f), -0.125f), 0.5f);
+                    const float ph = fmaf(q.ks, p, q.f0);
+                    const float am = q.ampd * fmaf(q.cd, p, 1.f);
+                    re = am * cos_turns(ph);
+                    im = am * sin_turns(ph);
+                } else {
+                    const double dz = g.pts[q.idx].dz;
+                    const double dx = ax * g.pitch, dy = ay * g.pitch;
+                    const double sgn = dz > 0.0 ? 1.0 : -1.0;
+                    const double r = sgn * sqrt(dx * dx + dy * dy + dz * dz);
+                    double t = r * g.inv_lambda;
+                    t -= rint(t);
+                    // amplitude a/(r + 0.05) = (a/D) D/(r + 0.05), D = sign|dz| + 0.05
+                    const double D = sgn * fabs(dz) + 0.05;
+                    const float am = (float)((double)q.ampd * D / (r + 0.05));
+                    re = am * cos_turns((float)t);
+                    im = am * sin_turns((float)t);
+                }
+                const int ire = (int)rintf(re * scale);
+                const int iim = (int)rintf(im * scale);
+                const bool xp = ax < wx1, xm = ax > 0 && -ax >= wx0;
+                const bool yp = ay < wy1, ym = ay > 0 && -ay >= wy0;
+                const bool xpv = xp && ax >= wx0, ypv = yp && ay >= wy0;
+                if (xpv && ypv) {
+                    atomicAdd(&acc_re[pbase + ay * TILE + ax], ire);
+                    atomicAdd(&acc_im[pbase + ay * TILE + ax], iim);
+                }
+                if (xm && ypv) {
+                    atomicAdd(&acc_re[pbase + ay * TILE - ax], ire);
+                    atomicAdd(&acc_im[pbase + ay * TILE - ax], iim);
+                }
+                if (xpv && ym) {
+                    atomicAdd(&acc_re[pbase - ay * TILE + ax], ire);
+                    atomicAdd(&acc_im[pbase - ay * TILE + ax], iim);
+                }
+                if (xm && ym) {
+                    atomicAdd(&acc_re[pbase - ay * TILE - ax], ire);
+                    atomicAdd(&acc_im[pbase - ay * TILE - ax], iim);
+                }
             }
-            const int pix = (cy0 + row - Y0) * TILE + (cx0 + col - X0);
-            const long long ire = (long long)(int)rintf(re * scale);
-            const long long iim = (long long)(int)rintf(im * scale);
-            atomicAdd(&acc[2 * pix], (unsigned long long)ire);
-            atomicAdd(&acc[2 * pix + 1], (unsigned long long)iim);
         }
     }
     __syncthreads();
-    const double inv = ldexp(1.0, -S);
+    const float inv = ldexpf(1.f, -S);
     for (int p = threadIdx.x; p < TILE * TILE; p += GATHER_THREADS) {
         const int row = Y0 + p / TILE, col = X0 + p % TILE;
         if (row >= g.ny || col >= g.nx) continue;
-        const double re = (double)(long long)acc[2 * p] * inv;
-        const double im = (double)(long long)acc[2 * p + 1] * inv;
+        const float re = (float)acc_re[p] * inv;
+        const float im = (float)acc_im[p] * inv;
         float2* o = g.wrp + (size_t)row * g.nx + col;
         if (g.accumulate) {
             const float2 v = *o;
-            *o = make_float2((float)(re + v.x), (float)(im + v.y));
+            *o = make_float2(re + v.x, im + v.y);
         } else {
-            *o = make_float2((float)re, (float)im);
+            *o = make_float2(re, im);
         }
     }
 }
@@ -274,24 +416,31 @@ int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, do
     a.ny = ny;
     a.ntx = (nx + TILE - 1) / TILE;
     a.nty = (ny + TILE - 1) / TILE;
+    a.ntiles = a.ntx * a.nty;
+    if (a.ntiles > MAX_TILES)
+        return oph_fail(ctx, OPH_ENOTSUP, "WRP plane of %dx%d exceeds %d tiles of 64x64", nx, ny, MAX_TILES);
     a.pitch = pitch;
     a.lambda = lambda;
     a.inv_lambda = 1.0 / lambda;
     a.z_wrp = z_wrp;
     a.grating = tan(asin(lambda / (2.0 * pitch)));
     a.mode = mode;
-    const int ntiles = a.ntx * a.nty;
+    const int ntiles = a.ntiles;
     const int64_t np = n > 0 ? n : 1;
+    const int nchunks = (int)((np + CHUNK - 1) / CHUNK);
+    a.nchunks = nchunks;
     a.pts = static_cast<WrpPoint*>(oph_scratch(ctx, SLOT_AUX0, sizeof(WrpPoint) * np));
-    // bookkeeping: counts, cursor, offsets (ntiles + 1), overflow, maxbits
-    const size_t book = (size_t)3 * ntiles + 3;
+    // bookkeeping: chunk counts, totals, offsets (ntiles + 1), overflow, chunk
+    // maxima, global max
+    const size_t book = (size_t)nchunks * ntiles + 2 * (size_t)ntiles + 2 + nchunks + 1;
     unsigned* bk = static_cast<unsigned*>(oph_scratch(ctx, SLOT_AUX1, sizeof(unsigned) * book));
     if (!a.pts || !bk) return OPH_ENOMEM;
-    a.counts = bk;
-    a.cursor = bk + ntiles;
-    a.offsets = bk + 2 * ntiles;
-    a.overflow = bk + 3 * ntiles + 1;
-    a.maxbits = bk + 3 * ntiles + 2;
+    a.chunk_counts = bk;
+    a.totals = bk + (size_t)nchunks * ntiles;
+    a.offsets = a.totals + ntiles;
+    a.overflow = a.offsets + ntiles + 1;
+    a.chunk_max = reinterpret_cast<float*>(a.overflow + 1);
+    a.gmax = a.chunk_max + nchunks;
     size_t cap = ctx->slot_size[SLOT_TABLE2] / sizeof(unsigned);
     const size_t want = (size_t)24 * np + 65536;
     if (cap < want) cap = want;
@@ -299,16 +448,15 @@ int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, do
         a.list = static_cast<unsigned*>(oph_scratch(ctx, SLOT_TABLE2, cap * sizeof(unsigned)));
         if (!a.list) return OPH_ENOMEM;
         a.capacity = (unsigned)(cap < 0xFFFFFFFFull ? cap : 0xFFFFFFFFull);
-        OPH_LAUNCH(ctx, "wrp_zero", wrp_zero, dim3((unsigned)((book + 255) / 256)), dim3(256), 0, bk, (int)book);
-        const unsigned pg = (unsigned)((np + 255) / 256);
-        OPH_LAUNCH(ctx, "wrp_prep", wrp_prep, dim3(pg), dim3(256), 0, a);
-        OPH_LAUNCH(ctx, "wrp_scan", wrp_scan, dim3(1), dim3(1024), 0, a.counts, a.offsets, ntiles);
-        OPH_LAUNCH(ctx, "wrp_fill", wrp_fill, dim3(pg), dim3(256), 0, a);
+        OPH_LAUNCH(ctx, "wrp_bin_count", wrp_bin_count, dim3(nchunks), dim3(BIN_THREADS), 0, a);
+        OPH_LAUNCH(ctx, "wrp_bin_colscan", wrp_bin_colscan, dim3((ntiles + 3) / 4), dim3(256), 0, a);
+        OPH_LAUNCH(ctx, "wrp_scan", wrp_scan, dim3(1), dim3(1024), 0, a.totals, a.offsets, ntiles);
+        OPH_LAUNCH(ctx, "wrp_bin_fill", wrp_bin_fill, dim3(nchunks), dim3(BIN_THREADS), 0, a);
         GatherArgs g{};
         g.pts = a.pts;
         g.offsets = a.offsets;
         g.list = a.list;
-        g.maxbits = a.maxbits;
+        g.gmax = a.gmax;
         g.capacity = a.capacity;
         g.nx = nx;
         g.ny = ny;

@@ -64,6 +64,7 @@ def load():
         "oph_copy_to_device": (I, [P, P, P, SZ]),
         "oph_copy_to_host": (I, [P, P, P, SZ]),
         "oph_device_zero": (I, [P, P, SZ]),
+        "oph_device_copy": (I, [P, P, P, SZ]),
         "oph_timer_begin": (I, [P]),
         "oph_timer_end": (I, [P, C.POINTER(C.c_float)]),
         "oph_profile_enable": (I, [P, I]),
@@ -143,6 +144,10 @@ class Context:
         d = DeviceArray(self, a.shape, a.dtype)
         self.check(self.lib.oph_copy_to_device(self.h, d.ptr, ptr(a), a.nbytes))
         return d
+
+    def copy(self, dst, src, nbytes=None):
+        """Device-to-device copy between DeviceArrays."""
+        self.check(self.lib.oph_device_copy(self.h, ptr(dst), ptr(src), nbytes or src.nbytes))
 
     def sync(self):
         self.check(self.lib.oph_sync(self.h))
