@@ -182,6 +182,50 @@ int oph_wrp_hologram(oph_ctx* ctx, int n_side, double pitch, double lambda,
                      double z_wrp, double distance, int pad, int band_limit,
                      int mode, const oph_points* pts, oph_c64* out, int flags);
 
+/* ---- depth-map layer hologram ----
+ * Replaces HologramGenerator::GenerateHologram's compute (DM/Hologram/src/
+ * HologramGenerator.cpp:261-294 with the CPU path HologramGenerator_CPU.cpp:
+ * 44-270 and the GPU path HologramGenerator_GPU.cpp:194-368): depth
+ * quantisation (FLAG_CHANGE_DEPTH_QUANTIZATION 0 or 1), the viewing-window
+ * transform z_l = -f d_l/(d_l - f) (or z_l = d_l without a field lens), per
+ * layer l = layer_lo..layer_hi:
+ *   u_l = img/255 [img > 0] [depth index == l] rho exp(i k z_l)
+ *   U  += H_l . fftshift(FFT2(fftshift(u_l))),
+ *   H_l = exp(i k (-z_l) sqrt(1 - (lambda fx)^2 - (lambda fy)^2)) on the
+ *         reference's frequency grids (HologramGenerator_CPU.cpp:196-216),
+ * then Encoding_Symmetrization(0, 1) (HologramGenerator.cpp:481-522,
+ * HologramGenerator_CPU.cpp:231-270) to the real fringe.  The scene (w x h,
+ * u8 intensity per colour and one u8 depth map) sits centred in an Nx x Ny
+ * transform grid (powers of two >= w, h; Nx = w, Ny = h is the reference's
+ * unpadded case).  Depth levels follow the reference's repeated addition
+ * (HologramGenerator.cpp:393-401), continued to exactly num_depth entries.
+ * Evanescent frequencies (absent at pitches above lambda/2) give 0 where the
+ * reference would produce NaN. */
+typedef struct oph_dm_spec {
+    int w, h;              /* scene size (pixels) */
+    int Nx, Ny;            /* transform grid */
+    double pitch_x, pitch_y;
+    double near_m, far_m;  /* depth range */
+    int num_depth;         /* number of depth levels L (<= 4096) */
+    int quant_mode;        /* 0: index = default_q - d;  1: requantise to L levels */
+    int default_q;         /* DEFAULT_DEPTH_QUANTIZATION (256) */
+    double field_lens;     /* focal length of the field lens; <= 0: none */
+    double rho_re, rho_im; /* the random-phase factor (1, 0 when RANDOM_PHASE = 0) */
+    int layer_lo, layer_hi;/* rendered layers, 1-based inclusive; 0 = all */
+} oph_dm_spec;
+
+/* output of oph_depthmap (one of; ORed into flags) */
+#define OPH_DM_OUT_FRINGE   0x0000 /* double fringe (u255_fringe_), ncolor x h x w */
+#define OPH_DM_OUT_FRINGE32 0x0100 /* float fringe */
+#define OPH_DM_OUT_FIELD    0x0200 /* complex64 hologram-plane field before encoding */
+#define OPH_DM_OUT_MASK     0x0300
+
+/* intensity: ncolor x h x w u8 (contiguous), depth: h x w u8, lambdas: ncolor
+ * wavelengths; out per OPH_DM_OUT_*, ncolor x h x w. */
+int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* spec, int ncolor, const double* lambdas,
+                 const unsigned char* intensity, const unsigned char* depth, void* out,
+                 int flags);
+
 #ifdef __cplusplus
 }
 #endif

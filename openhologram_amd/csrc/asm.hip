@@ -94,13 +94,15 @@ __global__ __launch_bounds__(512) void asm_rows_fwd(const float2* __restrict__ i
     float2* buf = lds + sub * PL;
     const bool ok = r < ny;
     const float2* src = in + (size_t)(ok ? r : 0) * nx;
+    fft::Twiddles<L> T;
+    T.load(tw, t);
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
         const int c = t + m * TPL - ox;
         buf[fft::lane_pad<L>(t, m)] = (ok && c >= 0 && c < nx) ? src[c] : make_float2(0.f, 0.f);
     }
     __syncthreads();
-    fft::line<L, -1>(buf, tw, t);
+    fft::line<L, -1>(buf, T, t);
     if (ok) {
         float2* dst = work + (size_t)(oy + r) * L;
 #pragma unroll
@@ -119,6 +121,8 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
     const int nthr = TPL * B;
     const int g = xcd_remap(blockIdx.x, gridDim.x);
     const int cbase = g * B;
+    fft::Twiddles<L> T;
+    T.load(tw, threadIdx.x % TPL);
     for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
         const int b = idx % B, i = idx / B;
         const int c = cbase + b;
@@ -129,7 +133,7 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
     __syncthreads();
     const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
     float2* line = lds + b * PL;
-    fft::line<L, -1>(line, tw, t);
+    fft::line<L, -1>(line, T, t);
     const int c = cbase + b;
     const float4 ax = tabx[c < npx ? c : 0];
 #pragma unroll
@@ -140,7 +144,7 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
         *e = fft::cmul(*e, h);
     }
     __syncthreads();
-    fft::line<L, +1>(line, tw, t);
+    fft::line<L, +1>(line, T, t);
     for (int idx = threadIdx.x; idx < ny * B; idx += nthr) {
         const int bb = idx % B, i = oy + idx / B;
         const int cc = cbase + bb;
@@ -161,10 +165,12 @@ __global__ __launch_bounds__(512) void asm_rows_inv(const float2* __restrict__ w
     float2* buf = lds + sub * PL;
     const bool ok = r < ny;
     const float2* src = work + (size_t)(oy + (ok ? r : 0)) * L;
+    fft::Twiddles<L> T;
+    T.load(tw, t);
 #pragma unroll
     for (int m = 0; m < 16; ++m) buf[fft::lane_pad<L>(t, m)] = ok ? src[t + m * TPL] : make_float2(0.f, 0.f);
     __syncthreads();
-    fft::line<L, +1>(buf, tw, t);
+    fft::line<L, +1>(buf, T, t);
     if (ok) {
         float2* dst = out + (size_t)r * nx;
         for (int c = t; c < nx; c += TPL) {

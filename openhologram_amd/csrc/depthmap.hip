@@ -1,0 +1,564 @@
+// depthmap.hip -- depth-map layer hologram (angular-spectrum method) on gfx950.
+//
+// Replaces HologramGenerator's CPU/GPU paths (DM/Hologram/src/
+// HologramGenerator_CPU.cpp:44-270, HologramGenerator_GPU.cpp:194-368,
+// HologramKernel.cu:10-187, HologramGenerator.cpp:391-522).  Per colour:
+//   u_l   = img/255 [img > 0] [idx == l]                  (CPU.cpp:57-66,139-141)
+//   U     = rho sum_l exp(i k z_l) H_l . fftshift(FFT2(fftshift(u_l)))
+//                                                          (CPU.cpp:152-160)
+//   H_l   = exp(i k (-z_l) sqrt(1 - (lambda fx)^2 - (lambda fy)^2)) on the
+//           reference's grids fx = -1/(2px) + x/Sx, fy = 1/(2py) - 1/Sy - y/Sy
+//                                                          (CPU.cpp:196-216)
+//   encode: keep centred spectrum rows [cropy1, cropy2], normalised inverse
+//           fftwShift, times exp(i 2 pi yy/(4 py)), real part (CPU.cpp:231-270,
+//           HologramGenerator.cpp:481-522)
+// with the scene (w x h) embedded in an Nx x Ny transform grid (Nx = w,
+// Ny = h reproduces the reference, which does not pad).
+//
+// The two fftshifts around each transform cancel against the shifts of the
+// encode: with k in FFT order, U_ref(y(k)) = (-1)^(kx+ky) A(k) where
+// A = sum_l m_l H_l(y(k)) FFT2(u_l), and the encoded field is IFFT2(crop(A)).
+// So the pipeline works on plain FFTs and evaluates H at the reference's
+// (off-by-one in fy) frequency of each FFT-order index:
+//   fx = s(kx)/Sx,  |fy| = |s(ky) + 1|/Sy,  s(k) = k < N/2 ? k : k - N.
+// The carrier exp(i k z_l) cancels the constant part of H, leaving the phase
+//   (z_l lambda / 2)(fx^2 + fy^2) + (z_l/lambda) p(u),
+//   p(u) = u^2/8 + u^3/16 + 5u^4/128 + 7u^5/256,  u = lambda^2 (fx^2 + fy^2),
+// whose first term is reduced per axis in FP64.
+//
+// Passes (per colour; the depth index is shared):
+//   dm_index  depth index of every pixel in one pass (the reference makes one
+//             pass per level);
+//   dm_cols   a workgroup owns 4 scene columns and a range of layers; two
+//             real columns are packed into one complex column, transformed
+//             along y (length Ny) and separated; only rows ky <= Ny/2 are
+//             stored (Hermitian symmetry of real input), with a presence flag
+//             per (layer, column group) -- absent groups are never written;
+//   dm_rows   a workgroup owns the row pair (ky, Ny-ky): per layer it
+//             transforms row ky along x once and obtains row Ny-ky from the
+//             same transform by symmetry, multiplies both by H and
+//             accumulates over all layers in registers; then the encode crop,
+//             the inverse row transforms, and the scene columns are stored;
+//   dm_final  inverse column transform, 1/(Nx Ny), the shift phase and the
+//             real part (or the complex field) in the scene window.
+#include <cmath>
+#include <vector>
+
+#include "fft_engine.h"
+#include "fft_kernels.h"
+
+namespace {
+
+constexpr int GROUP = 4;        // scene columns per dm_cols workgroup (2 complex lines)
+constexpr int MAX_LAYERS = 4096;
+
+struct DmArgs {
+    int w, h, Nx, Ny, ox, oy;
+    int L, lo, hi;                 // layers 1..L, rendered range [lo, hi]
+    int ngroups;
+    const unsigned char* img;      // h x w, this colour
+    const unsigned short* idx;     // h x w
+    float2* I;                     // [L][Ny/2 + 1][w]
+    unsigned char* flags;          // [L][ngroups]
+    unsigned char* any;            // [L]
+    const double* lc;              // per layer: cx, cy, K (3 doubles)
+    float ux, uy;                  // lambda^2 / Sx^2, lambda^2 / Sy^2
+    int exact;                     // FP64 transfer path
+    double lam2_sx2, lam2_sy2, inv_lambda;
+    float2 rho;
+    int encode, cy1, cy2;          // crop range (centred rows) when encoding
+    float2* T;                     // [Ny][w] after the inverse row transform
+    float2* spectrum;              // optional centred spectrum [Ny][Nx]
+    double scale;                  // 1/(Nx Ny)
+    double* fringe;                // h x w (double) or
+    float* fringe32;               // h x w (float) or
+    float2* field;                 // h x w complex
+};
+
+__device__ __forceinline__ int sgnidx(int k, int n) { return k < n / 2 ? k : k - n; }
+
+// ---- depth index -------------------------------------------------------------
+
+__global__ void dm_index(const unsigned char* dep, int n, int mode, int Q, const double* lev, int L,
+                         double dstep, double near_m, double far_m, unsigned short* idx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (mode == 0) {
+        const int v = Q - (int)dep[i];
+        idx[i] = (unsigned short)(v < 0 ? 0 : v);
+        return;
+    }
+    // change_depth_quan (CPU.cpp:74-103): every level adds (l+1) where the
+    // depth falls in [d1, d2) ([d1, d2] for the last)
+    const double dsrc = (double)dep[i] / 255.0;
+    const double dmap = (1 - dsrc) * (far_m - near_m) + near_m;
+    int acc = 0;
+    for (int l = 0; l < L; ++l) {
+        const double d1 = lev[l] - dstep / 2.0, d2 = lev[l] + dstep / 2.0;
+        const int t = (l < L - 1) ? ((dmap >= d1) && (dmap < d2)) : ((dmap >= d1) && (dmap <= d2));
+        acc += t * (l + 1);
+    }
+    idx[i] = (unsigned short)(acc > 65535 ? 65535 : acc);
+}
+
+__global__ void dm_zero(unsigned char* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// ---- column pass ---------------------------------------------------------------
+
+template <int LY>
+__global__ __launch_bounds__(2 * (LY / 16)) void dm_cols(DmArgs a, int layers_per_wg,
+                                                         const float2* __restrict__ twy) {
+    constexpr int TPL = LY / 16;
+    constexpr int NT = 2 * TPL;
+    constexpr int PL = fft::padded_len(LY);
+    extern __shared__ unsigned char smem[];
+    float2* lines = reinterpret_cast<float2*>(smem);                 // 2 x PL
+    unsigned* pix = reinterpret_cast<unsigned*>(lines + 2 * PL);     // h x GROUP: idx << 8 | img
+    unsigned* mask = pix + (size_t)a.h * GROUP;                      // layer bitmask
+    const int g = blockIdx.x;
+    const int x0 = g * GROUP;
+    const int l_begin = a.lo + blockIdx.y * layers_per_wg;
+    const int l_end = min(a.hi, l_begin + layers_per_wg - 1);
+    const int nwords = (a.L + 32) / 32;
+    const int p = threadIdx.x / TPL, t = threadIdx.x % TPL;
+    float2* line = lines + p * PL;
+    fft::Twiddles<LY> Tw;
+    Tw.load(twy, t);
+    for (int i = threadIdx.x; i < nwords; i += NT) mask[i] = 0u;
+    __syncthreads();
+    for (int e = threadIdx.x; e < a.h * GROUP; e += NT) {
+        const int c = e % GROUP, r = e / GROUP;
+        const int x = x0 + c;
+        unsigned v = 0u;
+        if (x < a.w) {
+            const size_t o = (size_t)r * a.w + x;
+            const unsigned im = a.img[o];
+            const unsigned id = a.idx[o];
+            // alpha = [img > 0] (CPU.cpp:61): a zero pixel contributes nothing
+            if (im > 0u && id >= (unsigned)l_begin && id <= (unsigned)l_end) {
+                v = (id << 8) | im;
+                atomicOr(&mask[id >> 5], 1u << (id & 31));
+            }
+        }
+        pix[e] = v;
+    }
+    __syncthreads();
+    const size_t lstride = (size_t)(a.Ny / 2 + 1) * a.w;
+    for (int l = l_begin; l <= l_end; ++l) {
+        if (!((mask[l >> 5] >> (l & 31)) & 1u)) continue;
+        // two real columns (x0 + 2p, x0 + 2p + 1) as one complex column
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int i = t + m * TPL;
+            const int r = i - a.oy;
+            float2 v = make_float2(0.f, 0.f);
+            if (r >= 0 && r < a.h) {
+                const unsigned wa = pix[r * GROUP + 2 * p], wb = pix[r * GROUP + 2 * p + 1];
+                if ((int)(wa >> 8) == l) v.x = (float)(wa & 255u) * (1.f / 255.f);
+                if ((int)(wb >> 8) == l) v.y = (float)(wb & 255u) * (1.f / 255.f);
+            }
+            line[fft::lane_pad<LY>(t, m)] = v;
+        }
+        __syncthreads();
+        fft::line<LY, -1>(line, Tw, t);
+        // separate: A(k) = (Z(k) + conj Z(-k))/2, B(k) = (Z(k) - conj Z(-k))/(2i)
+        float2* dst = a.I + (size_t)(l - 1) * lstride;
+        for (int e = threadIdx.x; e < (LY / 2 + 1) * GROUP; e += NT) {
+            const int c = e % GROUP, ky = e / GROUP;
+            const int x = x0 + c;
+            if (x >= a.w) continue;
+            const float2* ln = lines + (c >> 1) * PL;
+            const float2 Z = ln[fft::pad16(ky)];
+            const float2 Zm = ln[fft::pad16((LY - ky) & (LY - 1))];
+            float2 v;
+            if ((c & 1) == 0)
+                v = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+            else
+                v = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
+            dst[(size_t)ky * a.w + x] = v;
+        }
+        if (threadIdx.x == 0) {
+            a.flags[(size_t)(l - 1) * a.ngroups + g] = 1;
+            a.any[l - 1] = 1;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- row pass ------------------------------------------------------------------
+
+template <bool EXACT>
+__device__ __forceinline__ float2 dm_transfer(const DmArgs& a, float tx, double ty, float usx, int sy,
+                                              double K, float Kf, double sx2d) {
+    const float u = fmaf(a.uy, (float)(sy * sy), usx);
+    if (u >= 1.f) return make_float2(0.f, 0.f);
+    float ph;
+    if (!EXACT) {
+        const float pu = u * u * fmaf(u, fmaf(u, fmaf(u, 7.f / 256.f, 5.f / 128.f), 1.f / 16.f), 0.125f);
+        ph = tx + (float)ty + Kf * pu;
+    } else {
+        const double ud = a.lam2_sx2 * sx2d + a.lam2_sy2 * (double)sy * sy;
+        if (ud >= 1.0) return make_float2(0.f, 0.f);
+        double t = K * ud / (1.0 + sqrt(1.0 - ud));
+        t -= rint(t);
+        ph = (float)t;
+    }
+    return make_float2(cos_turns(ph), sin_turns(ph));
+}
+
+// Optional centred spectrum of one accumulated row, the encode crop, the
+// inverse row transform and the store of the scene columns into T.
+template <int LX>
+__device__ __forceinline__ void finish_row(const DmArgs& a, const float2 (&acc)[16], int ky, float2* line,
+                                           const fft::Twiddles<LX>& Tw, int t) {
+    constexpr int TPL = LX / 16;
+    const int yc = (ky + a.Ny / 2) & (a.Ny - 1);
+    if (a.spectrum) {
+        // U_ref(y, x) = (-1)^(kx + ky) A(ky, kx), y, x centred indices
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int kx = t + m * TPL;
+            const int xc = (kx + LX / 2) & (LX - 1);
+            const float sg = ((kx + ky) & 1) ? -1.f : 1.f;
+            a.spectrum[(size_t)yc * LX + xc] = make_float2(sg * acc[m].x, sg * acc[m].y);
+        }
+    }
+    const bool kept = !a.encode || (yc >= a.cy1 && yc <= a.cy2);
+    if (!kept) return;  // uniform over the workgroup
+#pragma unroll
+    for (int m = 0; m < 16; ++m) line[fft::lane_pad<LX>(t, m)] = acc[m];
+    __syncthreads();
+    fft::line<LX, +1>(line, Tw, t);
+    float2* dst = a.T + (size_t)ky * a.w;
+    for (int x = t; x < a.w; x += TPL) dst[x] = line[fft::pad16(x + a.ox)];
+    __syncthreads();
+}
+
+template <int LX, bool EXACT>
+__global__ __launch_bounds__(LX / 16) void dm_rows(DmArgs a, const float2* __restrict__ twx) {
+    constexpr int TPL = LX / 16;
+    constexpr int PL = fft::padded_len(LX);
+    __shared__ float2 line[PL];
+    const int t = threadIdx.x;
+    const int kya = blockIdx.x;               // 0 .. Ny/2
+    const int kyb = a.Ny - kya;
+    const bool has_b = kya != 0 && kya != a.Ny / 2;
+    const int sya = sgnidx(kya, a.Ny) + 1;
+    const int syb = sgnidx(kyb & (a.Ny - 1), a.Ny) + 1;
+    fft::Twiddles<LX> Tw;
+    Tw.load(twx, t);
+    float2 acca[16], accb[16];
+    float usx[16];
+    double sx2[16];
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        acca[m] = accb[m] = make_float2(0.f, 0.f);
+        const int sx = sgnidx(t + m * TPL, LX);
+        sx2[m] = (double)sx * sx;
+        usx[m] = a.ux * (float)(sx * sx);
+    }
+    const size_t lstride = (size_t)(a.Ny / 2 + 1) * a.w;
+    for (int l = a.lo; l <= a.hi; ++l) {
+        if (!a.any[l - 1]) continue;
+        const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * a.w;
+        const unsigned char* fl = a.flags + (size_t)(l - 1) * a.ngroups;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int x = t + m * TPL - a.ox;
+            float2 v = make_float2(0.f, 0.f);
+            if (x >= 0 && x < a.w && fl[x / GROUP]) v = src[x];
+            line[fft::lane_pad<LX>(t, m)] = v;
+        }
+        __syncthreads();
+        fft::line<LX, -1>(line, Tw, t);
+        const double cx = a.lc[3 * (l - 1)], cy = a.lc[3 * (l - 1) + 1], K = a.lc[3 * (l - 1) + 2];
+        const float Kf = (float)K;
+        double tya = cy * (double)sya * sya;
+        tya -= rint(tya);
+        double tyb = cy * (double)syb * syb;
+        tyb -= rint(tyb);
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int kx = t + m * TPL;
+            double txd = cx * sx2[m];
+            txd -= rint(txd);
+            const float tx = (float)txd;
+            const float2 gv = line[fft::lane_pad<LX>(t, m)];
+            const float2 ha = dm_transfer<EXACT>(a, tx, tya, usx[m], sya, K, Kf, sx2[m]);
+            acca[m] = fft::cadd(acca[m], fft::cmul(ha, gv));
+            if (has_b) {
+                const float2 gm = line[fft::pad16((LX - kx) & (LX - 1))];
+                const float2 gmc = make_float2(gm.x, -gm.y);
+                const float2 hb = dm_transfer<EXACT>(a, tx, tyb, usx[m], syb, K, Kf, sx2[m]);
+                accb[m] = fft::cadd(accb[m], fft::cmul(hb, gmc));
+            }
+        }
+        __syncthreads();
+    }
+    // rho (the same for every layer), optional centred spectrum, encode crop,
+    // inverse row transform
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        acca[m] = fft::cmul(acca[m], a.rho);
+        accb[m] = fft::cmul(accb[m], a.rho);
+    }
+    finish_row<LX>(a, acca, kya, line, Tw, t);
+    if (has_b) finish_row<LX>(a, accb, kyb, line, Tw, t);
+}
+
+// ---- final column pass ---------------------------------------------------------
+
+template <int LY, int B>
+__global__ __launch_bounds__(B * (LY / 16)) void dm_final(DmArgs a, const float2* __restrict__ twy) {
+    constexpr int TPL = LY / 16;
+    constexpr int PL = fft::padded_len(LY);
+    extern __shared__ float2 lds[];
+    const int nthr = B * TPL;
+    const int g = xcd_remap(blockIdx.x, gridDim.x);
+    const int c0 = g * B;
+    const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
+    fft::Twiddles<LY> Tw;
+    Tw.load(twy, t);
+    for (int e = threadIdx.x; e < LY * B; e += nthr) {
+        const int bb = e % B, ky = e / B;
+        const int x = c0 + bb;
+        const int yc = (ky + a.Ny / 2) & (a.Ny - 1);
+        const bool kept = !a.encode || (yc >= a.cy1 && yc <= a.cy2);
+        lds[bb * PL + fft::pad16(ky)] = (x < a.w && kept) ? a.T[(size_t)ky * a.w + x] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fft::line<LY, +1>(lds + b * PL, Tw, t);
+    const float sc = (float)a.scale;
+    for (int e = threadIdx.x; e < a.h * B; e += nthr) {
+        const int bb = e % B, r = e / B;
+        const int x = c0 + bb;
+        if (x >= a.w) continue;
+        const float2 v0 = lds[bb * PL + fft::pad16(a.oy + r)];
+        const float2 v = make_float2(v0.x * sc, v0.y * sc);
+        const size_t o = (size_t)r * a.w + x;
+        if (a.encode) {
+            // exp(i 2 pi yy/(4 py)), yy = Sy/2 - py r - py, Sy = h py (CPU.cpp:376-390)
+            double tt = a.h / 8.0 - (r + 1) / 4.0;
+            tt -= rint(tt);
+            const float ph = (float)tt;
+            const float re = v.x * cos_turns(ph) - v.y * sin_turns(ph);
+            if (a.fringe) a.fringe[o] = re;
+            if (a.fringe32) a.fringe32[o] = re;
+        } else {
+            a.field[o] = v;
+        }
+    }
+}
+
+// ---- launchers -------------------------------------------------------------------
+
+template <int LY>
+int launch_dm_cols(oph_ctx* ctx, const DmArgs& a, int chunks, int lpw) {
+    const float2* tw = oph_twiddles(ctx, LY);
+    if (!tw) return OPH_ENOMEM;
+    const size_t shm = 2 * fft::padded_len(LY) * sizeof(float2) + (size_t)a.h * GROUP * 4 +
+                       (size_t)((a.L + 32) / 32) * 4;
+    if (shm > 160 * 1024) return oph_fail(ctx, OPH_ENOTSUP, "scene height %d too large for the column pass", a.h);
+    static bool attr = false;
+    if (!attr) {
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&dm_cols<LY>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        hipGetLastError();
+        attr = true;
+    }
+    OPH_LAUNCH(ctx, "dm_cols", dm_cols<LY>, dim3(a.ngroups, chunks), dim3(2 * (LY / 16)), shm, a, lpw, tw);
+    return OPH_OK;
+}
+
+template <int LX>
+int launch_dm_rows(oph_ctx* ctx, const DmArgs& a) {
+    const float2* tw = oph_twiddles(ctx, LX);
+    if (!tw) return OPH_ENOMEM;
+    if (a.exact)
+        OPH_LAUNCH(ctx, "dm_rows", (dm_rows<LX, true>), dim3(a.Ny / 2 + 1), dim3(LX / 16), 0, a, tw);
+    else
+        OPH_LAUNCH(ctx, "dm_rows", (dm_rows<LX, false>), dim3(a.Ny / 2 + 1), dim3(LX / 16), 0, a, tw);
+    return OPH_OK;
+}
+
+template <int LY>
+int launch_dm_final(oph_ctx* ctx, const DmArgs& a) {
+    const float2* tw = oph_twiddles(ctx, LY);
+    if (!tw) return OPH_ENOMEM;
+    constexpr int B = fftk::col_lines(LY) >= 4 ? 4 : fftk::col_lines(LY);
+    const size_t shm = (size_t)B * fft::padded_len(LY) * sizeof(float2);
+    static bool attr = false;
+    if (!attr) {
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&dm_final<LY, B>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipGetLastError();
+        attr = true;
+    }
+    OPH_LAUNCH(ctx, "dm_final", (dm_final<LY, B>), dim3((a.w + B - 1) / B), dim3(B * (LY / 16)), shm, a, tw);
+    return OPH_OK;
+}
+
+}  // namespace
+
+extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, const double* lambdas,
+                            const unsigned char* intensity, const unsigned char* depth, void* out, int flags) {
+    int rc = oph_enter(ctx);
+    if (rc) return rc;
+    if (!s || !lambdas || !intensity || !depth || !out || ncolor < 1)
+        return oph_fail(ctx, OPH_EINVAL, "null argument");
+    const int w = s->w, h = s->h, Nx = s->Nx, Ny = s->Ny, L = s->num_depth;
+    if (w <= 0 || h <= 0 || w > Nx || h > Ny) return oph_fail(ctx, OPH_EINVAL, "scene %dx%d must fit the grid %dx%d", w, h, Nx, Ny);
+    if (!fftk::supported(Nx) || !fftk::supported(Ny))
+        return oph_fail(ctx, OPH_ENOTSUP, "grid %dx%d: sides must be powers of two in 16..8192", Nx, Ny);
+    if (L < 1 || L > MAX_LAYERS) return oph_fail(ctx, OPH_EINVAL, "num_depth must be in 1..%d", MAX_LAYERS);
+    if (!(s->pitch_x > 0) || !(s->pitch_y > 0) || !(s->far_m >= s->near_m))
+        return oph_fail(ctx, OPH_EINVAL, "bad pitch or depth range");
+    if (s->quant_mode != 0 && s->quant_mode != 1) return oph_fail(ctx, OPH_EINVAL, "quant_mode must be 0 or 1");
+    if ((flags & OPH_ASYNC) && !(flags & OPH_DEVICE_PTR))
+        return oph_fail(ctx, OPH_EINVAL, "OPH_ASYNC needs device pointers");
+    for (int c = 0; c < ncolor; ++c)
+        if (!(lambdas[c] > 0)) return oph_fail(ctx, OPH_EINVAL, "wavelength %d must be positive", c);
+    const int fmt = flags & OPH_DM_OUT_MASK;
+    const int lo = s->layer_lo >= 1 ? s->layer_lo : 1;
+    const int hi = (s->layer_hi >= 1 && s->layer_hi <= L) ? s->layer_hi : L;
+
+    // depth levels (GetDepthValues, HologramGenerator.cpp:393-401: repeated
+    // addition; continued to exactly L entries) and the viewing-window
+    // transform z_l = -f d/(d - f) (:432), or z_l = d without a field lens
+    std::vector<double> lev(L);
+    if (L > 1) {
+        const double dstep = (s->far_m - s->near_m) / (L - 1);
+        double v = s->near_m;
+        for (int i = 0; i < L; ++i) { lev[i] = v; v += dstep; }
+    } else {
+        lev[0] = s->far_m - s->near_m;
+    }
+    const double dstep = L > 1 ? (s->far_m - s->near_m) / (L - 1) : (s->far_m + s->near_m) / 2;
+    const double Sx = s->pitch_x * Nx, Sy = s->pitch_y * Ny;
+    std::vector<double>& hc = ctx->host_scratch;
+    hc.assign((size_t)3 * L * ncolor + L, 0.0);
+    double zmax = 0.0;
+    for (int c = 0; c < ncolor; ++c) {
+        const double lam = lambdas[c];
+        for (int l = 0; l < L; ++l) {
+            const double d = lev[l];
+            const double z = s->field_lens > 0.0 ? -s->field_lens * d / (d - s->field_lens) : d;
+            double* o = &hc[(size_t)3 * (c * L + l)];
+            o[0] = z * lam / (2.0 * Sx * Sx);
+            o[1] = z * lam / (2.0 * Sy * Sy);
+            o[2] = z / lam;
+            zmax = fmax(zmax, fabs(z / lam));
+        }
+    }
+    for (int l = 0; l < L; ++l) hc[(size_t)3 * L * ncolor + l] = lev[l];
+
+    const size_t npx = (size_t)w * h;
+    const int ngroups = (w + GROUP - 1) / GROUP;
+    const size_t lstride = (size_t)(Ny / 2 + 1) * w;
+    // device buffers
+    unsigned char* dimg = const_cast<unsigned char*>(intensity);
+    unsigned char* ddep = const_cast<unsigned char*>(depth);
+    if (!(flags & OPH_DEVICE_PTR)) {
+        dimg = static_cast<unsigned char*>(oph_scratch(ctx, SLOT_IN, npx * (ncolor + 1)));
+        if (!dimg) return OPH_ENOMEM;
+        ddep = dimg + npx * ncolor;
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync(dimg, intensity, npx * ncolor, hipMemcpyHostToDevice, ctx->stream));
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync(ddep, depth, npx, hipMemcpyHostToDevice, ctx->stream));
+    }
+    const size_t aux_bytes = npx * 2 + (size_t)L * ngroups + L + 64 + hc.size() * sizeof(double);
+    unsigned char* aux = static_cast<unsigned char*>(oph_scratch(ctx, SLOT_AUX0, aux_bytes + 256));
+    float2* I = static_cast<float2*>(oph_scratch(ctx, SLOT_WORK2, (size_t)L * lstride * sizeof(float2)));
+    float2* T = static_cast<float2*>(oph_scratch(ctx, SLOT_WORK1, (size_t)Ny * w * sizeof(float2)));
+    if (!aux || !I || !T) return OPH_ENOMEM;
+    double* dlc = reinterpret_cast<double*>(aux);
+    unsigned short* didx = reinterpret_cast<unsigned short*>(aux + hc.size() * sizeof(double));
+    unsigned char* dflags = reinterpret_cast<unsigned char*>(didx + npx);
+    unsigned char* dany = dflags + (size_t)L * ngroups;
+    OPH_HIP_CHECK(ctx, hipMemcpyAsync(dlc, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+
+    // outputs
+    const size_t osz = fmt == OPH_DM_OUT_FIELD ? sizeof(float2) : fmt == OPH_DM_OUT_FRINGE32 ? 4 : 8;
+    void* dout = out;
+    if (!(flags & OPH_DEVICE_PTR)) {
+        dout = oph_scratch(ctx, SLOT_OUT, npx * ncolor * osz);
+        if (!dout) return OPH_ENOMEM;
+    }
+
+    OPH_LAUNCH(ctx, "dm_index", dm_index, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, ddep, (int)npx,
+               s->quant_mode, s->default_q, dlc + (size_t)3 * L * ncolor, L, dstep, s->near_m, s->far_m, didx);
+
+    // truncated-series accuracy at the grid corner (see dm_transfer): the
+    // first dropped term is 21/1024 u^6 z/lambda turns
+    double lmax = 0;
+    for (int c = 0; c < ncolor; ++c) lmax = fmax(lmax, lambdas[c]);
+    const double fxm = 0.5 / s->pitch_x, fym = (Ny / 2 + 1.0) / Sy;
+    const double umax = lmax * lmax * (fxm * fxm + fym * fym);
+    const bool exact = umax >= 0.5 || zmax * pow(umax, 6) * (21.0 / 1024.0) > 1e-7;
+
+    for (int c = 0; c < ncolor; ++c) {
+        const double lam = lambdas[c];
+        DmArgs a{};
+        a.w = w; a.h = h; a.Nx = Nx; a.Ny = Ny;
+        a.ox = (Nx - w) / 2;
+        a.oy = (Ny - h) / 2;
+        a.L = L; a.lo = lo; a.hi = hi;
+        a.ngroups = ngroups;
+        a.img = dimg + npx * c;
+        a.idx = didx;
+        a.I = I;
+        a.flags = dflags;
+        a.any = dany;
+        a.lc = dlc + (size_t)3 * L * c;
+        a.ux = (float)(lam * lam / (Sx * Sx));
+        a.uy = (float)(lam * lam / (Sy * Sy));
+        a.lam2_sx2 = lam * lam / (Sx * Sx);
+        a.lam2_sy2 = lam * lam / (Sy * Sy);
+        a.inv_lambda = 1.0 / lam;
+        a.exact = exact ? 1 : 0;
+        a.rho = make_float2((float)s->rho_re, (float)s->rho_im);
+        a.encode = fmt != OPH_DM_OUT_FIELD;
+        const int cropy = Ny / 2;
+        a.cy1 = cropy - cropy / 2 - 1;
+        a.cy2 = a.cy1 + cropy - 1;
+        a.T = T;
+        a.spectrum = nullptr;
+        a.scale = 1.0 / ((double)Nx * Ny);
+        char* ob = static_cast<char*>(dout) + npx * c * osz;
+        a.fringe = fmt == OPH_DM_OUT_FRINGE ? reinterpret_cast<double*>(ob) : nullptr;
+        a.fringe32 = fmt == OPH_DM_OUT_FRINGE32 ? reinterpret_cast<float*>(ob) : nullptr;
+        a.field = fmt == OPH_DM_OUT_FIELD ? reinterpret_cast<float2*>(ob) : nullptr;
+
+        const int nfl = L * ngroups + L;
+        OPH_LAUNCH(ctx, "dm_zero", dm_zero, dim3((nfl + 255) / 256), dim3(256), 0, dflags, nfl);
+        const int nl = hi - lo + 1;
+        int chunks = (int)((4LL * ctx->cu_count + ngroups - 1) / ngroups);
+        if (chunks > nl) chunks = nl;
+        if (chunks < 1) chunks = 1;
+        const int lpw = (nl + chunks - 1) / chunks;
+        chunks = (nl + lpw - 1) / lpw;
+        switch (Ny) {
+#define OPH_DMC(n) case n: rc = launch_dm_cols<n>(ctx, a, chunks, lpw); break;
+            OPH_FFT_LENGTHS(OPH_DMC)
+#undef OPH_DMC
+        }
+        if (rc) return rc;
+        switch (Nx) {
+#define OPH_DMR(n) case n: rc = launch_dm_rows<n>(ctx, a); break;
+            OPH_FFT_LENGTHS(OPH_DMR)
+#undef OPH_DMR
+        }
+        if (rc) return rc;
+        switch (Ny) {
+#define OPH_DMF(n) case n: rc = launch_dm_final<n>(ctx, a); break;
+            OPH_FFT_LENGTHS(OPH_DMF)
+#undef OPH_DMF
+        }
+        if (rc) return rc;
+    }
+    if (!(flags & OPH_DEVICE_PTR))
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync(out, dout, npx * ncolor * osz, hipMemcpyDeviceToHost, ctx->stream));
+    return oph_finish(ctx, flags);
+}

@@ -84,10 +84,12 @@ __global__ __launch_bounds__(512) void fft_rows_kernel(float2* data, int rows, i
     float2* buf = lds + sub * PL;
     const bool ok = row < rows;
     float2* src = data + (size_t)(ok ? row : 0) * stride;
+    fft::Twiddles<L> T;
+    T.load(tw, t);
 #pragma unroll
     for (int m = 0; m < 16; ++m) buf[fft::lane_pad<L>(t, m)] = ok ? src[t + m * TPL] : make_float2(0.f, 0.f);
     __syncthreads();
-    fft::line<L, SIGN>(buf, tw, t);
+    fft::line<L, SIGN>(buf, T, t);
     if (ok) {
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
@@ -106,6 +108,8 @@ __global__ __launch_bounds__(1024) void fft_cols_kernel(float2* data, int ncols,
     const int nthr = TPL * B;
     const int g = xcd_remap(blockIdx.x, gridDim.x);
     const int c0 = g * B;
+    fft::Twiddles<L> T;
+    T.load(tw, threadIdx.x % TPL);
     for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
         const int b = idx % B, i = idx / B;
         const int c = c0 + b;
@@ -113,7 +117,7 @@ __global__ __launch_bounds__(1024) void fft_cols_kernel(float2* data, int ncols,
     }
     __syncthreads();
     const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
-    fft::line<L, SIGN>(lds + b * PL, tw, t);
+    fft::line<L, SIGN>(lds + b * PL, T, t);
     for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
         const int bb = idx % B, i = idx / B;
         const int c = c0 + bb;

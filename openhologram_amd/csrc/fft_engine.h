@@ -143,22 +143,12 @@ __host__ __device__ constexpr int padded_len(int L) { return L + L / 16; }
 // consecutive entries.  The tables of all stages of one length are
 // concatenated; stage 0 (Ns = 1) has none.
 
-// One Stockham stage of radix R over a line in LDS (padded indexing).  The
-// stage's twiddles are loaded first so that their latency overlaps the LDS
-// reads and the barrier.
+// One Stockham stage of radix R over a line in LDS (padded indexing), with the
+// stage's twiddles already in registers (w[p][r], r >= 1; unused for Ns = 1).
 template <int L, int R, int SIGN, int TPL>
-__device__ __forceinline__ void stage(float2* buf, const float2* __restrict__ tws, int Ns, int t) {
+__device__ __forceinline__ void stage(float2* buf, const float2 (&w)[L / R / TPL][R], int Ns, int t) {
     constexpr int NB = L / R;
     constexpr int PER = NB / TPL;  // butterflies per thread (TPL divides NB)
-    float2 w[PER][R];
-    if (Ns > 1) {
-#pragma unroll
-        for (int p = 0; p < PER; ++p) {
-            const int k = (t + p * TPL) & (Ns - 1);
-#pragma unroll
-            for (int r = 1; r < R; ++r) w[p][r] = tws[(r - 1) * Ns + k];
-        }
-    }
     float2 v[PER][R];
     // padded index of j + r*NB is pad16(j) + r*(NB + NB/16) when 16 | NB:
     // the per-r part folds into the LDS instruction's immediate offset
@@ -212,22 +202,66 @@ __device__ __forceinline__ int lane_pad(int t, int m) {
     else return pad16(t + m * TPL);
 }
 
+// Twiddles of every stage of one thread's butterflies, held in registers for
+// the whole kernel: loaded once, before the data, and shared by a forward and
+// an inverse transform of the same length (the inverse conjugates).
+// Stage tables are laid out as tws[(r-1) * Ns + k] = exp(-2 pi i k r/(Ns R))
+// and concatenated over the stages (oph_twiddles).
+template <int L>
+struct Twiddles {
+    static constexpr int TPL = L / 16;
+    static constexpr int A = n16(L);
+    static constexpr int RL = last_radix(L);
+    static constexpr int PL = RL > 1 ? (L / RL) / TPL : 1;   // butterflies per thread, last stage
+    float2 w16[A > 1 ? A - 1 : 1][1][16];                   // radix-16 stages s = 1..A-1
+    float2 wl[PL][RL > 1 ? RL : 1];                           // last radix-RL stage
+    float2 w0[1][16];                                          // stage 0: unused
+
+    __device__ __forceinline__ void load(const float2* __restrict__ tw, int t) {
+        int Ns = 16, off = 0;
+#pragma unroll
+        for (int s = 1; s < A; ++s) {
+            const int k = t & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < 16; ++r) w16[s - 1][0][r] = tw[off + (r - 1) * Ns + k];
+            off += 15 * Ns;
+            Ns *= 16;
+        }
+        if constexpr (RL > 1) {
+            // Ns = 16^A here
+#pragma unroll
+            for (int p = 0; p < PL; ++p) {
+                const int k = (t + p * TPL) & (Ns - 1);
+#pragma unroll
+                for (int r = 1; r < RL; ++r) wl[p][r] = tw[off + (r - 1) * Ns + k];
+            }
+        }
+    }
+};
+
 // Full transform of one LDS line by TPL = L/16 threads (t in [0, TPL)).
 // Caller syncs before (line loaded) and the last stage ends with a barrier.
-// tw: the concatenated per-stage tables of length L (oph_twiddles).
 template <int L, int SIGN>
-__device__ __forceinline__ void line(float2* buf, const float2* __restrict__ tw, int t) {
+__device__ __forceinline__ void line(float2* buf, const Twiddles<L>& T, int t) {
     constexpr int TPL = L / 16;
     constexpr int A = n16(L);
     constexpr int RL = last_radix(L);
-    int Ns = 1, off = 0;
+    stage<L, 16, SIGN, TPL>(buf, T.w0, 1, t);
+    int Ns = 16;
 #pragma unroll
-    for (int s = 0; s < A; ++s) {
-        stage<L, 16, SIGN, TPL>(buf, tw + off, Ns, t);
-        if (Ns > 1) off += Ns * 15;
+    for (int s = 1; s < A; ++s) {
+        stage<L, 16, SIGN, TPL>(buf, T.w16[s - 1], Ns, t);
         Ns *= 16;
     }
-    if (RL > 1) stage<L, RL, SIGN, TPL>(buf, tw + off, Ns, t);
+    if constexpr (RL > 1) stage<L, RL, SIGN, TPL>(buf, T.wl, Ns, t);
+}
+
+// Convenience: load the twiddles and transform.
+template <int L, int SIGN>
+__device__ __forceinline__ void line(float2* buf, const float2* __restrict__ tw, int t) {
+    Twiddles<L> T;
+    T.load(tw, t);
+    line<L, SIGN>(buf, T, t);
 }
 
 }  // namespace fft

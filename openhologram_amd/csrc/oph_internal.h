@@ -51,6 +51,7 @@ struct oph_ctx {
     int cu_count = 256;
     void* tw[24] = {};  // FFT twiddle tables by log2(length)
     unsigned* pending_overflow = nullptr;  // device flag left by an async WRP call
+    std::vector<double> host_scratch;      // per-call host tables uploaded to the device
 };
 
 // Records an error message on the context and returns code.

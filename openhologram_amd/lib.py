@@ -34,6 +34,18 @@ class Spec(C.Structure):
                 ("tilt_deg", C.c_double * 2)]
 
 
+class DmSpec(C.Structure):
+    """oph_dm_spec"""
+    _fields_ = [("w", C.c_int), ("h", C.c_int), ("Nx", C.c_int), ("Ny", C.c_int),
+                ("pitch_x", C.c_double), ("pitch_y", C.c_double), ("near_m", C.c_double),
+                ("far_m", C.c_double), ("num_depth", C.c_int), ("quant_mode", C.c_int),
+                ("default_q", C.c_int), ("field_lens", C.c_double), ("rho_re", C.c_double),
+                ("rho_im", C.c_double), ("layer_lo", C.c_int), ("layer_hi", C.c_int)]
+
+
+DM_OUT = {"fringe": (0x0000, np.float64), "fringe32": (0x0100, np.float32), "field": (0x0200, np.complex64)}
+
+
 class Points(C.Structure):
     """oph_points"""
     _fields_ = [("x", C.c_void_p), ("y", C.c_void_p), ("z", C.c_void_p), ("amp", C.c_void_p),
@@ -76,6 +88,7 @@ def load():
         "oph_asm_propagate": (I, [P, P, I, I, D, D, D, D, I, I, I]),
         "oph_wrp": (I, [P, I, I, D, D, D, I, C.POINTER(Points), P, I]),
         "oph_wrp_hologram": (I, [P, I, D, D, D, D, I, I, I, C.POINTER(Points), P, I]),
+        "oph_depthmap": (I, [P, C.POINTER(DmSpec), I, P, P, P, P, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -231,6 +244,28 @@ class Context:
                                              int(band_limit), mode, C.byref(pts), ptr(out), flags))
         del keep
         return out
+
+
+    def depthmap(self, w, h, Nx, Ny, pitch, near, far, L, lambdas, intensity, depth, out="fringe",
+                 quant_mode=0, Q=256, field_lens=0.0, rho=(1.0, 0.0), layers=(0, 0), device=False,
+                 asynchronous=False, dst=None):
+        """Depth-map hologram (oph_depthmap).  intensity: (ncolor, h, w) u8,
+        depth: (h, w) u8; returns (ncolor, h, w) per `out`."""
+        px, py = pitch if hasattr(pitch, "__len__") else (pitch, pitch)
+        spec = DmSpec(w, h, Nx, Ny, px, py, near, far, L, quant_mode, Q, field_lens, rho[0], rho[1],
+                      layers[0], layers[1])
+        lam = np.ascontiguousarray(np.atleast_1d(lambdas), np.float64)
+        flag, dt = DM_OUT[out]
+        nc = len(lam)
+        if not device:
+            intensity = np.ascontiguousarray(intensity, np.uint8).reshape(nc, h, w)
+            depth = np.ascontiguousarray(depth, np.uint8).reshape(h, w)
+        if dst is None:
+            dst = self.empty((nc, h, w), dt) if device else np.zeros((nc, h, w), dt)
+        flags = flag | (DEVICE_PTR if device else 0) | (ASYNC if asynchronous else 0)
+        self.check(self.lib.oph_depthmap(self.h, C.byref(spec), nc, ptr(lam), ptr(intensity), ptr(depth),
+                                         ptr(dst), flags))
+        return dst
 
 
 class DeviceArray:

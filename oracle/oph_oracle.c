@@ -519,15 +519,17 @@ void orc_depth_index(int n, const unsigned char* dep8, int quant_mode, int Q,
  * (all columns), inverse fftwShift normalised by Nx Ny, crop the scene
  * window, multiply exp(i 2 pi yy/(4 py)), yy = Sy/2 - py r - py with r the
  * scene row and Sy = h py (CPU.cpp:376-390), keep Re.
- * Outputs: spectrum (Ny x Nx, centred order as U_complex_; may be NULL) and
- * fringe (h x w, may be NULL).  The CPU path's skipping of empty layers
+ * Outputs: spectrum (Ny x Nx, centred order as U_complex_; may be NULL),
+ * fringe (h x w, may be NULL) and field (h x w complex: the normalised
+ * inverse transform of the whole spectrum in the scene window, i.e. the
+ * hologram-plane field before encoding; may be NULL).  The CPU path's skipping of empty layers
  * (CPU.cpp:145) changes nothing numerically and is kept.
  */
 void orc_depthmap(int w, int h, const unsigned char* img8, const unsigned char* dep8,
                   int Nx, int Ny, double px, double py, double lambda,
                   double near_m, double far_m, int L, int quant_mode, int Q,
                   double field_lens, double rho_re, double rho_im, int layer_lo,
-                  int layer_hi, c128* spectrum, double* fringe, int nthreads) {
+                  int layer_hi, c128* spectrum, double* fringe, c128* field, int nthreads) {
     const size_t N = (size_t)Nx * Ny;
     int* didx = (int*)malloc(sizeof(int) * (size_t)w * h);
     orc_depth_index(w * h, dep8, quant_mode, Q, near_m, far_m, L, didx);
@@ -578,6 +580,16 @@ void orc_depthmap(int w, int h, const unsigned char* img8, const unsigned char* 
         }
     }
     if (spectrum) memcpy(spectrum, U, sizeof(c128) * N);
+    if (field) {
+        /* complex hologram field without the encoding: the normalised inverse
+         * fftwShift of the whole spectrum, cropped to the scene window */
+        c128* V = (c128*)malloc(sizeof(c128) * N);
+        memcpy(V, U, sizeof(c128) * N);
+        ref_fftwshift(V, Nx, Ny, -1, 1, nthreads);
+        for (int r = 0; r < h; ++r)
+            for (int c = 0; c < w; ++c) field[(size_t)r * w + c] = V[(size_t)(r + oy) * Nx + (c + ox)];
+        free(V);
+    }
     if (fringe) {
         /* Encoding_Symmetrization(ivec2(0,1)), HologramGenerator.cpp:487-514 */
         const int cropy = Ny / 2;

@@ -62,7 +62,7 @@ def lib():
         _lib.orc_depthmap.argtypes = [C.c_int, C.c_int, P, P, C.c_int, C.c_int, C.c_double,
                                       C.c_double, C.c_double, C.c_double, C.c_double, C.c_int,
                                       C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
-                                      C.c_int, C.c_int, P, P, C.c_int]
+                                      C.c_int, C.c_int, P, P, P, C.c_int]
     return _lib
 
 
@@ -78,6 +78,12 @@ def ref_lib():
         _ref.ref_keti_fringe.argtypes = [P, C.c_int, P, P]
         _ref.ref_keti_normalize.argtypes = [P, P, C.c_int, C.c_int]
     return _ref
+
+
+def _pow2(*ns):
+    for n in ns:
+        if n < 1 or (n & (n - 1)):
+            raise ValueError(f"the oracle's FFT needs power-of-two sizes, got {n}")
 
 
 def _p(a):
@@ -147,12 +153,14 @@ def wrp(nx, ny, pitch, lam, z_wrp, x, y, z, amp=None, mode=0, nthreads=0):
 
 def fft2d(a, sign=-1, normalize=False, nthreads=0):
     a = np.array(a, np.complex128, order="C", copy=True)
+    _pow2(*a.shape)
     lib().orc_fft2d(_p(a), a.shape[1], a.shape[0], sign, int(normalize), nthreads)
     return a
 
 
 def asm(field, px, py, lam, z, pad=2, band_limit=False, nthreads=0):
     f = np.array(field, np.complex128, order="C", copy=True)
+    _pow2(f.shape[0] * pad, f.shape[1] * pad)
     lib().orc_asm(_p(f), f.shape[1], f.shape[0], px, py, lam, z, pad, int(band_limit), nthreads)
     return f
 
@@ -172,13 +180,21 @@ def depth_index(dep8, quant_mode, Q, near, far, L):
 
 def depthmap(img8, dep8, Nx, Ny, px, py, lam, near, far, L, quant_mode=0, Q=256,
              field_lens=0.0, rho=(1.0, 0.0), layers=(1, 0), want_spectrum=False,
-             want_fringe=True, nthreads=0):
+             want_fringe=True, want_field=False, nthreads=0):
+    """FP64 depth-map hologram of one colour.  Returns (spectrum, fringe) or,
+    with want_field, (spectrum, fringe, field); entries not asked for are None."""
     img8 = np.ascontiguousarray(img8, np.uint8)
     dep8 = np.ascontiguousarray(dep8, np.uint8)
     h, w = img8.shape
+    _pow2(Nx, Ny)
+    if w > Nx or h > Ny:
+        raise ValueError("scene larger than the transform grid")
     spec = np.zeros((Ny, Nx), np.complex128) if want_spectrum else None
     fr = np.zeros((h, w), np.float64) if want_fringe else None
+    fld = np.zeros((h, w), np.complex128) if want_field else None
     lib().orc_depthmap(w, h, _p(img8), _p(dep8), Nx, Ny, px, py, lam, near, far, L,
                        quant_mode, Q, field_lens, rho[0], rho[1], layers[0], layers[1],
-                       _p(spec), _p(fr), nthreads)
+                       _p(spec), _p(fr), _p(fld), nthreads)
+    if want_field:
+        return spec, fr, fld
     return spec, fr
