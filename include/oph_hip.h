@@ -182,6 +182,24 @@ int oph_wrp_hologram(oph_ctx* ctx, int n_side, double pitch, double lambda,
                      double z_wrp, double distance, int pad, int band_limit,
                      int mode, const oph_points* pts, oph_c64* out, int flags);
 
+/* ---- Fresnel point-cloud hologram ----
+ * The direct Fresnel sum of cwoPLS::Fresnel (CWO/cwoPLS.cpp:370-422) with the
+ * point amplitude, initial phase and plane carrier:
+ *   U[i][j] = sum_n a_n exp(i (k z_n + phi_n)) exp(i k ((X_j - x_n)^2 + (Y_i - y_n)^2)/(2 z_n)),
+ *   X_j = (j - nx/2) pitch, Y_i = (i - ny/2) pitch (row index i is y).
+ * method OPH_FRESNEL_VALU sums it directly (FP64 phase reduction, FP32
+ * accumulation); the MFMA methods evaluate the same field as the separable
+ * complex contraction (pixels x points)(points x pixels) with chirp factors
+ * rounded to BF16 or FP16 (optionally split hi + lo) and FP32 accumulation.
+ * out: ny x nx complex64.  z_n > 0 (distance of the point's plane). */
+#define OPH_FRESNEL_VALU             0
+#define OPH_FRESNEL_MFMA_BF16        1
+#define OPH_FRESNEL_MFMA_BF16_SPLIT  2
+#define OPH_FRESNEL_MFMA_FP16        3
+#define OPH_FRESNEL_MFMA_FP16_SPLIT  4
+int oph_pc_fresnel(oph_ctx* ctx, int nx, int ny, double pitch, double lambda,
+                   const oph_points* pts, int method, oph_c64* out, int flags);
+
 /* ---- depth-map layer hologram ----
  * Replaces HologramGenerator::GenerateHologram's compute (DM/Hologram/src/
  * HologramGenerator.cpp:261-294 with the CPU path HologramGenerator_CPU.cpp:

@@ -1,0 +1,427 @@
+// fresnel.hip -- Fresnel point-cloud hologram on gfx950: a plain FP32 VALU
+// kernel and the separable form as an MFMA complex contraction.
+//
+// Field (the direct Fresnel sum of CWO/cwoPLS.cpp:370-422, with the point
+// amplitude, its initial phase and the plane carrier exp(i k z_n)):
+//   U[i][j] = sum_n a_n exp(i (k z_n + phi_n)) exp(i k ((X_j - x_n)^2 + (Y_i - y_n)^2) / (2 z_n))
+//   X_j = (j - nx/2) p,  Y_i = (i - ny/2) p                 (cwoPLS.cpp:400-401)
+// The quadratic phase separates, exp(i k (dx^2 + dy^2)/(2z)) =
+// exp(i k dx^2/(2z)) exp(i k dy^2/(2z)), so U = A B^T with
+//   A[i][n] = w_n exp(i pi (Y_i - y_n)^2/(lambda z_n)),  B[j][n] = exp(i pi (X_j - x_n)^2/(lambda z_n)),
+//   w_n = a_n exp(i (k z_n + phi_n)):
+// a complex (pixels x points)(points x pixels) contraction.  The chirp phases
+// reach ~1e4 turns and are reduced in FP64 before the FP32 sin/cos; the
+// factors are rounded to BF16 or FP16, optionally split as hi + lo (the lo
+// part carries the rounding error of hi), and contracted on MFMA with FP32
+// accumulation (A B = Ah Bh + Ah Bl + Al Bh).
+#include <cmath>
+
+#include "oph_internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+struct FrPoint {
+    double x, y, c;   // position, c = 1/(2 lambda z) (turns per m^2)
+    float wre, wim;   // a exp(i 2 pi frac(z/lambda + phi/2pi))
+    float f0, a;      // frac(z/lambda + phi/2pi), amplitude
+};
+
+__global__ void fresnel_prep(const double* x, const double* y, const double* z, const double* amp,
+                             const double* phase, int64_t n, int64_t npad, double lambda, FrPoint* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    FrPoint q;
+    if (i < n) {
+        q.x = x[i];
+        q.y = y[i];
+        q.c = 1.0 / (2.0 * lambda * z[i]);
+        double t = z[i] / lambda + (phase ? phase[i] * 0.15915494309189535 : 0.0);
+        t -= rint(t);
+        const float a = amp ? (float)amp[i] : 1.f;
+        q.f0 = (float)t;
+        q.a = a;
+        q.wre = a * cos_turns((float)t);
+        q.wim = a * sin_turns((float)t);
+    } else {  // padding up to the contraction chunk: zero weight
+        q.x = q.y = 0.0;
+        q.c = 0.0;
+        q.f0 = 0.f;
+        q.a = 0.f;
+        q.wre = q.wim = 0.f;
+    }
+    out[i] = q;
+}
+
+// ---------------------------------------------------------------------------
+// plain kernel: a 64x64 pixel tile per workgroup, 4x4 pixels per thread,
+// points staged in LDS; per (point, thread) the phase at the block centre in
+// FP64, per pixel the exact quadratic offset c (2 ex dx0 + ex^2 + 2 ey dy0 + ey^2)
+// in FP32 (a few turns at most).
+
+constexpr int PT = 256;
+
+struct ValuArgs {
+    const FrPoint* pts;
+    int64_t n, chunk;
+    int nx, ny;
+    double p;
+    float2* partial;
+};
+
+__global__ __launch_bounds__(256) void fresnel_valu(ValuArgs a) {
+    __shared__ double s_x[PT], s_y[PT], s_c[PT];
+    __shared__ float s_f0[PT], s_a[PT];
+    const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * 16 + tx;
+    const int c0 = blockIdx.x * 64 + tx * 4, r0 = blockIdx.y * 64 + ty * 4;
+    const double X0 = (c0 + 1.5 - a.nx / 2) * a.p, Y0 = (r0 + 1.5 - a.ny / 2) * a.p;
+    float ex[4], ey[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        ex[k] = (float)((k - 1.5) * a.p);
+        ey[k] = (float)((k - 1.5) * a.p);
+    }
+    float re[4][4], im[4][4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) re[r][c] = im[r][c] = 0.f;
+    const int64_t pb = (int64_t)blockIdx.z * a.chunk;
+    const int64_t pe = min(a.n, pb + a.chunk);
+    for (int64_t base = pb; base < pe; base += PT) {
+        __syncthreads();
+        const int64_t j = base + tid;
+        if (j < pe) {
+            const FrPoint q = a.pts[j];
+            s_x[tid] = q.x;
+            s_y[tid] = q.y;
+            s_c[tid] = q.c;
+            s_f0[tid] = q.f0;
+            s_a[tid] = q.a;
+        }
+        __syncthreads();
+        const int cnt = (int)min((int64_t)PT, pe - base);
+        for (int k = 0; k < cnt; ++k) {
+            const double dx0 = X0 - s_x[k], dy0 = Y0 - s_y[k];
+            const double cc = s_c[k];
+            double t = fma(cc, fma(dx0, dx0, dy0 * dy0), (double)s_f0[k]);
+            const float f0 = (float)(t - rint(t));
+            const float cf = (float)cc, am = s_a[k];
+            const float dxf = (float)dx0, dyf = (float)dy0;
+            float A[4], B[4];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) A[c] = cf * fmaf(2.f * ex[c], dxf, ex[c] * ex[c]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) B[r] = cf * fmaf(2.f * ey[r], dyf, ey[r] * ey[r]);
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const float ph = f0 + (A[c] + B[r]);
+                    re[r][c] = fmaf(am, cos_turns(ph), re[r][c]);
+                    im[r][c] = fmaf(am, sin_turns(ph), im[r][c]);
+                }
+        }
+    }
+    float2* dst = a.partial + (size_t)blockIdx.z * a.nx * a.ny;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int row = r0 + r;
+        if (row >= a.ny) continue;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const int col = c0 + c;
+            if (col < a.nx) dst[(size_t)row * a.nx + col] = make_float2(re[r][c], im[r][c]);
+        }
+    }
+}
+
+__global__ void fresnel_sum_chunks(const float2* partial, int chunks, int64_t npix, float2* out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    float re = 0.f, im = 0.f;
+    for (int c = 0; c < chunks; ++c) {
+        const float2 v = partial[(size_t)c * npix + i];
+        re += v.x;
+        im += v.y;
+    }
+    out[i] = make_float2(re, im);
+}
+
+// ---------------------------------------------------------------------------
+// MFMA contraction.  A workgroup of 4 waves owns a 128 x 128 output tile; for
+// each chunk of 32 points it writes the chirp factors A (128 rows of y) and
+// B (128 columns of x) to LDS as 16-bit planes [re_hi, im_hi, re_lo, im_lo]
+// laid out [row][point], then each wave contracts a 64 x 64 sub-tile with
+// v_mfma_f32_32x32x16 (lane l supplies row l%32, points 8(l/32)..+7 of A and
+// column l%32, the same points of B; accumulator register v of lane l is
+// output row 8(v/4) + 4(l/32) + v%4, column l%32).
+
+constexpr int MT = 128;         // output tile (rows, columns)
+constexpr int KC = 32;          // points per chunk
+constexpr int KS = KC + 8;      // LDS row stride (elements), 16-byte aligned rows
+
+template <typename T>
+struct Mfma;
+
+template <>
+struct Mfma<__bf16> {
+    typedef bf16x8 V;
+    static __device__ __forceinline__ __bf16 cvt(float f) { return (__bf16)f; }
+    static __device__ __forceinline__ float back(__bf16 h) { return (float)h; }
+    static __device__ __forceinline__ f32x16 mac(V a, V b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Mfma<_Float16> {
+    typedef f16x8 V;
+    static __device__ __forceinline__ _Float16 cvt(float f) { return (_Float16)f; }
+    static __device__ __forceinline__ float back(_Float16 h) { return (float)h; }
+    static __device__ __forceinline__ f32x16 mac(V a, V b, f32x16 c) {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <typename V>
+__device__ __forceinline__ V negate(V v) {
+    u32x4 u = __builtin_bit_cast(u32x4, v);
+    u ^= 0x80008000u;
+    return __builtin_bit_cast(V, u);
+}
+
+struct MfmaArgs {
+    const FrPoint* pts;
+    int64_t npad;     // multiple of KC
+    int nx, ny;
+    double p;
+    float2* out;
+};
+
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
+    typedef typename Mfma<T>::V V;
+    constexpr int NP = SPLIT ? 4 : 2;
+    __shared__ __attribute__((aligned(16))) T As[NP][MT][KS];
+    __shared__ __attribute__((aligned(16))) T Bs[NP][MT][KS];
+    __shared__ double p_x[KC], p_y[KC], p_c[KC];
+    __shared__ float p_wr[KC], p_wi[KC];
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wr = wave >> 1, wc = wave & 1;
+    // XCD-aware order of the tiles: consecutive logical tiles on one XCD
+    const int ntx = (a.nx + MT - 1) / MT;
+    const int g = xcd_remap(blockIdx.x, gridDim.x);
+    const int i0 = (g / ntx) * MT, j0 = (g % ntx) * MT;
+
+    f32x16 cr[2][2], ci[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            cr[x][y] = (f32x16)0.f;
+            ci[x][y] = (f32x16)0.f;
+        }
+    // generation role of this thread: one row (A) / column (B), 16 points
+    const int gr = tid & (MT - 1), gh = tid >> 7;
+    const double Yr = (i0 + gr - a.ny / 2) * a.p;
+    const double Xc = (j0 + gr - a.nx / 2) * a.p;
+    const bool row_ok = i0 + gr < a.ny, col_ok = j0 + gr < a.nx;
+
+    for (int64_t k0 = 0; k0 < a.npad; k0 += KC) {
+        __syncthreads();
+        if (tid < KC) {
+            const FrPoint q = a.pts[k0 + tid];
+            p_x[tid] = q.x;
+            p_y[tid] = q.y;
+            p_c[tid] = q.c;
+            p_wr[tid] = q.wre;
+            p_wi[tid] = q.wim;
+        }
+        __syncthreads();
+        // 16 points of this thread's row/column, written as 8-element vectors
+#pragma unroll
+        for (int hb = 0; hb < 2; ++hb) {
+            const int kb0 = gh * 16 + hb * 8;
+            V va[NP], vb[NP];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const int kk = kb0 + e;
+                const double c = p_c[kk];
+                // A: w exp(i 2 pi c (Y - y)^2)
+                const double dy = Yr - p_y[kk];
+                double ty = c * dy * dy;
+                ty -= rint(ty);
+                const float cy = cos_turns((float)ty), sy = sin_turns((float)ty);
+                const float wr_ = p_wr[kk], wi_ = p_wi[kk];
+                const float are = row_ok ? wr_ * cy - wi_ * sy : 0.f;
+                const float aim = row_ok ? wr_ * sy + wi_ * cy : 0.f;
+                // B: exp(i 2 pi c (X - x)^2)
+                const double dx = Xc - p_x[kk];
+                double tx = c * dx * dx;
+                tx -= rint(tx);
+                const float bre = col_ok ? cos_turns((float)tx) : 0.f;
+                const float bim = col_ok ? sin_turns((float)tx) : 0.f;
+                const T arh = Mfma<T>::cvt(are), aih = Mfma<T>::cvt(aim);
+                const T brh = Mfma<T>::cvt(bre), bih = Mfma<T>::cvt(bim);
+                va[0][e] = arh;
+                va[1][e] = aih;
+                vb[0][e] = brh;
+                vb[1][e] = bih;
+                if constexpr (SPLIT) {
+                    va[2][e] = Mfma<T>::cvt(are - Mfma<T>::back(arh));
+                    va[3][e] = Mfma<T>::cvt(aim - Mfma<T>::back(aih));
+                    vb[2][e] = Mfma<T>::cvt(bre - Mfma<T>::back(brh));
+                    vb[3][e] = Mfma<T>::cvt(bim - Mfma<T>::back(bih));
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                *reinterpret_cast<V*>(&As[q][gr][kb0]) = va[q];
+                *reinterpret_cast<V*>(&Bs[q][gr][kb0]) = vb[q];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < KC / 16; ++s) {
+            const int kb = 16 * s + 8 * (lane >> 5);
+            V A[2][NP], B[2][NP];
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                const int row = wr * 64 + 32 * x + (lane & 31);
+                const int col = wc * 64 + 32 * x + (lane & 31);
+#pragma unroll
+                for (int q = 0; q < NP; ++q) {
+                    A[x][q] = *reinterpret_cast<const V*>(&As[q][row][kb]);
+                    B[x][q] = *reinterpret_cast<const V*>(&Bs[q][col][kb]);
+                }
+            }
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                const V nbi = negate(B[y][1]);
+                const V nbil = SPLIT ? negate(B[y][NP - 1]) : nbi;
+#pragma unroll
+                for (int x = 0; x < 2; ++x) {
+                    // re: Ar Br - Ai Bi ; im: Ar Bi + Ai Br  (hi x hi)
+                    cr[x][y] = Mfma<T>::mac(A[x][0], B[y][0], cr[x][y]);
+                    cr[x][y] = Mfma<T>::mac(A[x][1], nbi, cr[x][y]);
+                    ci[x][y] = Mfma<T>::mac(A[x][0], B[y][1], ci[x][y]);
+                    ci[x][y] = Mfma<T>::mac(A[x][1], B[y][0], ci[x][y]);
+                    if constexpr (SPLIT) {
+                        // hi x lo and lo x hi
+                        cr[x][y] = Mfma<T>::mac(A[x][0], B[y][2], cr[x][y]);
+                        cr[x][y] = Mfma<T>::mac(A[x][1], nbil, cr[x][y]);
+                        cr[x][y] = Mfma<T>::mac(A[x][2], B[y][0], cr[x][y]);
+                        cr[x][y] = Mfma<T>::mac(A[x][3], nbi, cr[x][y]);
+                        ci[x][y] = Mfma<T>::mac(A[x][0], B[y][3], ci[x][y]);
+                        ci[x][y] = Mfma<T>::mac(A[x][1], B[y][2], ci[x][y]);
+                        ci[x][y] = Mfma<T>::mac(A[x][2], B[y][1], ci[x][y]);
+                        ci[x][y] = Mfma<T>::mac(A[x][3], B[y][0], ci[x][y]);
+                    }
+                }
+            }
+        }
+    }
+    // epilogue
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+                const int row = i0 + wr * 64 + 32 * x + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+                const int col = j0 + wc * 64 + 32 * y + (lane & 31);
+                if (row < a.ny && col < a.nx) a.out[(size_t)row * a.nx + col] = make_float2(cr[x][y][v], ci[x][y][v]);
+            }
+}
+
+}  // namespace
+
+extern "C" int oph_pc_fresnel(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, const oph_points* pts,
+                              int method, oph_c64* out, int flags) {
+    int rc = oph_enter(ctx);
+    if (rc) return rc;
+    if (!pts || !out) return oph_fail(ctx, OPH_EINVAL, "null argument");
+    if (nx <= 0 || ny <= 0 || nx > 65536 || ny > 65536) return oph_fail(ctx, OPH_EINVAL, "bad hologram size");
+    if (!(pitch > 0) || !(lambda > 0)) return oph_fail(ctx, OPH_EINVAL, "bad pitch or wavelength");
+    if (method < OPH_FRESNEL_VALU || method > OPH_FRESNEL_MFMA_FP16_SPLIT)
+        return oph_fail(ctx, OPH_EINVAL, "unknown method %d", method);
+    if (pts->n < 0 || (pts->n > 0 && (!pts->x || !pts->y || !pts->z)))
+        return oph_fail(ctx, OPH_EINVAL, "bad point arrays");
+    if ((flags & OPH_ASYNC) && !(flags & OPH_DEVICE_PTR))
+        return oph_fail(ctx, OPH_EINVAL, "OPH_ASYNC needs device pointers");
+    const int64_t n = pts->n;
+    const int64_t npad = ((n + KC - 1) / KC) * KC;
+    const size_t pb = (size_t)(n > 0 ? n : 1) * sizeof(double);
+    const int sf = n > 0 ? flags : (flags | OPH_DEVICE_PTR);
+    if (!(flags & OPH_DEVICE_PTR) && !oph_scratch(ctx, SLOT_PTS, 5 * pb)) return OPH_ENOMEM;
+    const double* dx = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->x, pb, sf, 0));
+    const double* dy = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->y, pb, sf, pb));
+    const double* dz = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->z, pb, sf, 2 * pb));
+    const double* da = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->amp, pb, sf, 3 * pb));
+    const double* dp = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->phase, pb, sf, 4 * pb));
+    if (n > 0 && (!dx || !dy || !dz)) return OPH_ENOMEM;
+    FrPoint* fp = static_cast<FrPoint*>(oph_scratch(ctx, SLOT_AUX0, sizeof(FrPoint) * (npad > 0 ? npad : 1)));
+    if (!fp) return OPH_ENOMEM;
+    if (npad > 0)
+        OPH_LAUNCH(ctx, "fresnel_prep", fresnel_prep, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0, dx, dy,
+                   dz, da, dp, n, npad, lambda, fp);
+    const size_t npix = (size_t)nx * ny;
+    float2* o = reinterpret_cast<float2*>(out);
+    if (!(flags & OPH_DEVICE_PTR)) {
+        o = static_cast<float2*>(oph_scratch(ctx, SLOT_OUT, npix * sizeof(float2)));
+        if (!o) return OPH_ENOMEM;
+    }
+    if (method == OPH_FRESNEL_VALU) {
+        const int gx = (nx + 63) / 64, gy = (ny + 63) / 64;
+        int64_t chunks = (4LL * ctx->cu_count + (int64_t)gx * gy - 1) / ((int64_t)gx * gy);
+        const int64_t maxc = (n + 1023) / 1024;
+        if (chunks > maxc) chunks = maxc;
+        if (chunks > 16) chunks = 16;
+        if (chunks < 1) chunks = 1;
+        ValuArgs v{};
+        v.pts = fp;
+        v.n = n;
+        v.chunk = n > 0 ? (n + chunks - 1) / chunks : 1;
+        v.nx = nx;
+        v.ny = ny;
+        v.p = pitch;
+        v.partial = static_cast<float2*>(oph_scratch(ctx, SLOT_WORK0, chunks * npix * sizeof(float2)));
+        if (!v.partial) return OPH_ENOMEM;
+        OPH_LAUNCH(ctx, "fresnel_valu", fresnel_valu, dim3(gx, gy, (unsigned)chunks), dim3(16, 16), 0, v);
+        OPH_LAUNCH(ctx, "fresnel_sum_chunks", fresnel_sum_chunks, dim3((unsigned)((npix + 255) / 256)), dim3(256),
+                   0, v.partial, (int)chunks, (int64_t)npix, o);
+    } else {
+        MfmaArgs m{};
+        m.pts = fp;
+        m.npad = npad;
+        m.nx = nx;
+        m.ny = ny;
+        m.p = pitch;
+        m.out = o;
+        const unsigned tiles = (unsigned)(((nx + MT - 1) / MT) * ((ny + MT - 1) / MT));
+        switch (method) {
+            case OPH_FRESNEL_MFMA_BF16:
+                OPH_LAUNCH(ctx, "fresnel_mfma", (fresnel_mfma<__bf16, false>), dim3(tiles), dim3(256), 0, m);
+                break;
+            case OPH_FRESNEL_MFMA_BF16_SPLIT:
+                OPH_LAUNCH(ctx, "fresnel_mfma", (fresnel_mfma<__bf16, true>), dim3(tiles), dim3(256), 0, m);
+                break;
+            case OPH_FRESNEL_MFMA_FP16:
+                OPH_LAUNCH(ctx, "fresnel_mfma", (fresnel_mfma<_Float16, false>), dim3(tiles), dim3(256), 0, m);
+                break;
+            default:
+                OPH_LAUNCH(ctx, "fresnel_mfma", (fresnel_mfma<_Float16, true>), dim3(tiles), dim3(256), 0, m);
+                break;
+        }
+    }
+    if (!(flags & OPH_DEVICE_PTR))
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync(out, o, npix * sizeof(float2), hipMemcpyDeviceToHost, ctx->stream));
+    return oph_finish(ctx, flags);
+}

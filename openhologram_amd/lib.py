@@ -43,6 +43,7 @@ class DmSpec(C.Structure):
                 ("rho_im", C.c_double), ("layer_lo", C.c_int), ("layer_hi", C.c_int)]
 
 
+FRESNEL_METHODS = {"valu": 0, "bf16": 1, "bf16_split": 2, "fp16": 3, "fp16_split": 4}
 DM_OUT = {"fringe": (0x0000, np.float64), "fringe32": (0x0100, np.float32), "field": (0x0200, np.complex64)}
 
 
@@ -89,6 +90,7 @@ def load():
         "oph_wrp": (I, [P, I, I, D, D, D, I, C.POINTER(Points), P, I]),
         "oph_wrp_hologram": (I, [P, I, D, D, D, D, I, I, I, C.POINTER(Points), P, I]),
         "oph_depthmap": (I, [P, C.POINTER(DmSpec), I, P, P, P, P, I]),
+        "oph_pc_fresnel": (I, [P, I, I, D, D, C.POINTER(Points), I, P, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -245,6 +247,19 @@ class Context:
         del keep
         return out
 
+
+    def fresnel(self, nx, ny, pitch, lambda_, x, y, z, amp=None, phase=None, method="valu", out=None,
+                device=False, asynchronous=False):
+        """Fresnel point-cloud field (oph_pc_fresnel), (ny, nx) complex64.
+        method: valu | bf16 | bf16_split | fp16 | fp16_split."""
+        m = FRESNEL_METHODS[method]
+        pts, keep = _points(x, y, z, amp, phase, device)
+        if out is None:
+            out = self.empty((ny, nx), np.complex64) if device else np.zeros((ny, nx), np.complex64)
+        flags = (DEVICE_PTR if device else 0) | (ASYNC if asynchronous else 0)
+        self.check(self.lib.oph_pc_fresnel(self.h, nx, ny, pitch, lambda_, C.byref(pts), m, ptr(out), flags))
+        del keep
+        return out
 
     def depthmap(self, w, h, Nx, Ny, pitch, near, far, L, lambdas, intensity, depth, out="fringe",
                  quant_mode=0, Q=256, field_lens=0.0, rho=(1.0, 0.0), layers=(0, 0), device=False,
