@@ -202,7 +202,192 @@ def cpu_wrp(x, y, z, a, zw, n_side, pitch, lam, dist, pad):
                       f"by point count) + one full {n_side * pad}^2 padded ASM ({t_asm:.2f} s)"}
 
 
-WORKLOADS = {"rs": bench_rs, "wrp": bench_wrp}
+DM_KERNELS = ["dm_index", "dm_zero", "dm_cols", "dm_rows", "dm_final"]
+
+
+def bench_depthmap(ctx, args):
+    """Depth-map RGB hologram: 1920x1080 scene, 4096x2048 grid, 256 layers x
+    3 colours, double fringe out."""
+    from openhologram_amd import scenes
+    w, h, Nx, Ny, L = 1920, 1080, 4096, 2048, 256
+    img, dep = scenes.depth_scene(w, h, seed=3)
+    lams = (638e-9, 532e-9, 473e-9)
+    dimg, ddep = ctx.to_device(img), ctx.to_device(dep)
+    out = ctx.empty((3, h, w), np.float64)
+    steps = max(2, args.steps // 4)
+
+    def step():
+        ctx.depthmap(w, h, Nx, Ny, 8e-6, 0.09, 0.11, L, lams, dimg, ddep, device=True, asynchronous=True,
+                     dst=out)
+    a2 = argparse.Namespace(**vars(args))
+    a2.steps = steps
+    wall, per = _time_steps(ctx, step, a2, DM_KERNELS)
+    line = {"value": steps / wall, "unit": "holograms/s (RGB)", "ms_per_step": wall / steps * 1e3,
+            "config": {"workload": "depthmap_rgb_1920x1080_grid4096x2048_256layers_8um"}, "kernels": per}
+    # algorithmic HBM bytes of the layer passes (complex64 intermediates of
+    # the Hermitian half, written once by dm_cols and read once by dm_rows)
+    idx = 256 - dep.astype(np.int64)
+    layers = int((np.bincount(idx.ravel(), minlength=257)[1:] > 0).sum())
+    inter = layers * (Ny // 2 + 1) * w * 8
+    line["layers_populated"] = layers
+    if "dm_rows" in per:
+        t = (per["dm_rows"]["avg_us"] + per["dm_cols"]["avg_us"]) * 1e-6
+        line["hbm_gbs_layer_passes"] = 2 * inter / t / 1e9
+    if not args.no_cpu:
+        from oracle import oracle as O
+        l0 = int(np.argmax(np.bincount(idx.ravel(), minlength=257)))
+        t0 = time.perf_counter()
+        O.depthmap(img[1], dep, Nx, Ny, 8e-6, 8e-6, lams[1], 0.09, 0.11, L, layers=(l0, l0 + 3),
+                   nthreads=host_cores())
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": 1.0 / (dt * 3 * layers / 4), "unit": "holograms/s (RGB)",
+                                "cores": host_cores(), "kind": "port",
+                                "sample": f"FP64 oracle, 4 populated layers of one colour at full size "
+                                          f"({dt:.2f} s incl. one encode), scaled to {layers} layers x 3 colours"}
+    return line
+
+
+def bench_fresnel(ctx, args):
+    """Fresnel point CGH: 1e6 points on 64 planes, 4096^2 at 8 um: MFMA
+    contraction per precision and the plain FP32 VALU kernel."""
+    from openhologram_amd import scenes
+    n = 4096
+    x, y, pl, z, a, ph = scenes.fresnel_points(1_000_000, planes=64, seed=4)
+    d = [ctx.to_device(v) for v in (x, y, z, a, ph)]
+    out = ctx.empty((n, n), np.complex64)
+    res = {}
+    flops = 8.0 * n * n * len(x)
+    for m in ("fp16", "bf16_split", "fp16_split", "bf16"):
+        def step(m=m):
+            ctx.fresnel(n, n, 8e-6, 532e-9, *d, method=m, out=out, device=True, asynchronous=True)
+        a2 = argparse.Namespace(**vars(args))
+        a2.steps, a2.warmup = 3, 1
+        wall, per = _time_steps(ctx, step, a2, ["fresnel_mfma", "fresnel_prep"])
+        mult = 3 if m.endswith("split") else 1
+        kt = per["fresnel_mfma"]["avg_us"] * 1e-6
+        res[m] = {"holograms_per_s": 3 / wall, "ms": wall / 3 * 1e3,
+                  "mfma_tflops": flops * mult / kt / 1e12, "mfma_frac": flops * mult / kt / 2.5e15}
+    def vstep():
+        ctx.fresnel(n, n, 8e-6, 532e-9, *d, method="valu", out=out, device=True, asynchronous=True)
+    a2 = argparse.Namespace(**vars(args))
+    a2.steps, a2.warmup = 1, 0
+    wall, per = _time_steps(ctx, vstep, a2, ["fresnel_valu"])
+    res["valu"] = {"holograms_per_s": 1 / wall, "ms": wall * 1e3,
+                   "point_pixel_per_s": n * n * len(x) / wall}
+    for m in ("fp16", "bf16_split", "fp16_split", "bf16"):
+        res[m]["speedup_vs_valu"] = res[m]["holograms_per_s"] / res["valu"]["holograms_per_s"]
+    line = {"value": res["fp16_split"]["holograms_per_s"], "unit": "holograms/s",
+            "config": {"workload": "fresnel_1e6_points_64_planes_4096sq_8um_532nm"}, "methods": res}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        rng = np.random.default_rng(0)
+        rows, cols = rng.integers(0, n, (2, 256))
+        t0 = time.perf_counter()
+        O.fresnel(n, n, 8e-6, 532e-9, x, y, z, a, ph, rows=rows, cols=cols, nthreads=host_cores())
+        dt = time.perf_counter() - t0
+        pairs = 256 * len(x)
+        line["cpu_baseline"] = {"value": pairs / dt / (n * n * len(x)), "unit": "holograms/s",
+                                "cores": host_cores(), "kind": "port",
+                                "sample": f"FP64 oracle direct sum on 256 sampled pixels x all points "
+                                          f"({dt:.2f} s), scaled by pair count"}
+    return line
+
+
+CNN_KERNELS = ["conv3x3_fwd", "conv3x3_dw", "dw_reduce", "conv_flip", "relu_mask", "mse_grad", "sum_partials",
+               "adam_step"]
+
+
+def bench_learned(ctx, args):
+    """Learned correction at 1024^2 (2048^2 transform grid, 532 nm): a
+    16-layer depth-map field corrected by the 8-block residual CNN, against
+    the 256-layer field.  The network is trained here on method-generated
+    pairs (bounded by --learn-train scenes and --learn-steps Adam steps),
+    evaluated on held-out scenes by reconstruction PSNR at 3 depths; the
+    per-frame gate falls back to the 256-layer field below 35 dB."""
+    from openhologram_amd import learned
+    from openhologram_amd.lib import Cnn
+    n, N = 1024, 2048
+    t0 = time.perf_counter()
+    xs, ts = learned.make_pairs(ctx, range(100000, 100000 + args.learn_train), n)
+    held = range(200000, 200000 + args.learn_eval)
+    t_pairs = time.perf_counter() - t0
+    scale = float(np.sqrt(np.mean(np.abs(xs) ** 2)))
+    net = Cnn(ctx, blocks=8, channels=32, seed=1)
+    ctx.profile(True)
+    t0 = time.perf_counter()
+    hist = learned.train(ctx, net, xs, ts, scale, steps=args.learn_steps, batch=8, patch=128, lr=1e-3)
+    ctx.sync()
+    t_train = time.perf_counter() - t0
+    train_k = {}
+    for k in CNN_KERNELS:
+        c, ms = ctx.profile_get(k)
+        if c:
+            train_k[k] = {"launches_per_step": c / args.learn_steps, "avg_us": ms / c * 1e3}
+    ctx.profile(False)
+    del xs, ts
+    # held-out evaluation
+    depths = learned.sampled_depths()
+    ev = []
+    scenes_ = []
+    for s in held:
+        img, dep = learned.scenes.depth_scene(n, n, seed=s, colors=1)
+        scenes_.append((img, dep))
+        c = learned.field(ctx, img, dep, n, 16)
+        f = learned.field(ctx, img, dep, n, 256)
+        u = learned.correct(ctx, net, c, scale)
+        ev.append((rel_l2(c, f), rel_l2(u, f), learned.recon_psnr(ctx, c, f, depths),
+                   learned.recon_psnr(ctx, u, f, depths)))
+    ev = np.array(ev)
+    # timed, device resident: 256-layer field; 16-layer field + CNN
+    img, dep = scenes_[0]
+    dimg, ddep = ctx.to_device(img), ctx.to_device(dep)
+    f_out = ctx.empty((1, n, n), np.complex64)
+    c_out = ctx.empty((1, n, n), np.complex64)
+    y_out = ctx.empty((1, n, n, 2), np.float32)
+    lam = [learned.LAMBDA]
+
+    def full_step():
+        ctx.depthmap(n, n, N, N, learned.PITCH, learned.NEAR, learned.FAR, 256, lam, dimg, ddep, out="field",
+                     device=True, asynchronous=True, dst=f_out)
+
+    def learned_step():
+        ctx.depthmap(n, n, N, N, learned.PITCH, learned.NEAR, learned.FAR, 16, lam, dimg, ddep, out="field",
+                     quant_mode=1, rho=(1.0 / scale, 0.0), device=True, asynchronous=True, dst=c_out)
+        net.forward(c_out, out=y_out, device=True, shape=(1, n, n, 2))
+    a2 = argparse.Namespace(**vars(args))
+    a2.steps = max(5, args.steps // 2)
+    w_full, per_full = _time_steps(ctx, full_step, a2, DM_KERNELS)
+    w_l, per_l = _time_steps(ctx, learned_step, a2, DM_KERNELS + ["conv3x3_fwd"])
+    hz_full, hz_l = a2.steps / w_full, a2.steps / w_l
+    passed = ev[:, 3] >= 35.0
+    # with every frame audited, each frame pays the learned path plus the
+    # 256-layer check (and its 6 reconstructions); a failing frame returns
+    # the check's own 256-layer field
+    conv_flops = 2.0 * n * n * 9 * (2 * 32 + 16 * 32 * 32 + 32 * 2)
+    kt = per_l.get("conv3x3_fwd", {})
+    line = {"value": hz_l, "unit": "holograms/s (16-layer + CNN, no per-frame check)",
+            "ms_per_step": 1e3 / hz_l,
+            "config": {"workload": "learned_1024sq_grid2048_16_to_256_layers_8x32ch_532nm",
+                       "train_scenes": args.learn_train, "train_steps": args.learn_steps,
+                       "heldout_scenes": args.learn_eval},
+            "full_256_layer_hz": hz_full, "speedup_vs_256_layer": hz_l / hz_full,
+            "heldout": {"field_rel_l2_coarse": float(ev[:, 0].mean()),
+                        "field_rel_l2_learned": float(ev[:, 1].mean()),
+                        "psnr_db_coarse_mean": float(ev[:, 2].mean()),
+                        "psnr_db_learned_mean": float(ev[:, 3].mean()),
+                        "psnr_db_learned_min": float(ev[:, 3].min()),
+                        "gate_pass_rate": float(passed.mean())},
+            "gated_effective_speedup": (hz_l / hz_full) if passed.all() else None,
+            "train": {"seconds": t_train, "pair_gen_seconds": t_pairs, "loss_first": hist[0],
+                      "loss_last": float(np.mean(hist[-20:])), "kernels": train_k},
+            "kernels": {"full": per_full, "learned": per_l}}
+    if kt:
+        line["conv_fwd_tflops"] = conv_flops / (kt["launches_per_step"] * kt["avg_us"] * 1e-6) / 1e12
+    return line
+
+
+WORKLOADS = {"rs": bench_rs, "wrp": bench_wrp, "depthmap": bench_depthmap, "fresnel": bench_fresnel,
+             "learned": bench_learned}
 DEFAULT = "wrp"
 
 
@@ -214,6 +399,10 @@ def main():
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--check", action="store_true", help="also report rel-L2 of the output vs the FP64 oracle")
+    ap.add_argument("--only", action="store_true", help="run the selected workload only (no others)")
+    ap.add_argument("--learn-train", type=int, default=64, help="learned: training scenes (1024^2 pairs)")
+    ap.add_argument("--learn-steps", type=int, default=300, help="learned: Adam steps (batch 8, 128^2 patches)")
+    ap.add_argument("--learn-eval", type=int, default=8, help="learned: held-out scenes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the hologram "
@@ -223,6 +412,18 @@ def main():
     import openhologram_amd as oph
     with oph.Context(0) as ctx:
         line = WORKLOADS[args.workload](ctx, args)
+        if not args.only:
+            # the other workloads, reported beside the headline
+            extra = {}
+            for name, fn in WORKLOADS.items():
+                if name == args.workload:
+                    continue
+                try:
+                    extra[name] = fn(ctx, args)
+                except Exception as e:  # report, do not hide the headline
+                    extra[name] = {"error": repr(e)}
+                print(f"# {name} done", file=sys.stderr, flush=True)
+            line["workloads"] = extra
     base = {"metric": METRIC, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "data": "synthetic (seeded SplitMix64)"}

@@ -244,6 +244,34 @@ int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* spec, int ncolor, const double
                  const unsigned char* intensity, const unsigned char* depth, void* out,
                  int flags);
 
+/* ---- learned correction (no counterpart in the reference) ----
+ * A residual CNN that maps a coarse (few-layer) depth-map hologram field to
+ * the full-layer one: h0 = conv(x) (2 -> 32 channels), h_b = h_{b-1} +
+ * conv(relu(conv(h_{b-1}))) for b = 1..blocks, y = x + conv(h_blocks)
+ * (32 -> 2); 3x3 convolutions with zero padding and bias.  Tensors are NHWC
+ * float32 (B x H x W x 2: real and imaginary parts); the convolutions run on
+ * MFMA with BF16-rounded operands and FP32 accumulation.  Parameters are one
+ * float32 vector: head W[32][9][2], b[32]; per block W1[32][9][32], b1[32],
+ * W2[32][9][32], b2[32]; tail W[2][9][32], b[2]. */
+typedef struct oph_cnn oph_cnn;
+int oph_cnn_create(oph_ctx* ctx, int blocks, int channels, uint64_t seed, oph_cnn** out);
+void oph_cnn_destroy(oph_ctx* ctx, oph_cnn* net);
+int64_t oph_cnn_num_params(const oph_cnn* net);
+/* which = 0: parameters, 1: gradients of the last oph_cnn_gradients/train_step */
+int oph_cnn_get_params(oph_ctx* ctx, oph_cnn* net, float* host, int which);
+int oph_cnn_set_params(oph_ctx* ctx, oph_cnn* net, const float* host);
+int oph_cnn_forward(oph_ctx* ctx, oph_cnn* net, const float* x, float* y, int batch, int h, int w,
+                    int flags);
+/* loss = mean over all elements of (y - target)^2; gradients only, or
+ * gradients then one Adam step (beta 0.9 / 0.999, eps 1e-8, rate lr) */
+int oph_cnn_gradients(oph_ctx* ctx, oph_cnn* net, const float* x, const float* target, int batch,
+                      int h, int w, double* loss, int flags);
+int oph_cnn_train_step(oph_ctx* ctx, oph_cnn* net, const float* x, const float* target, int batch,
+                       int h, int w, float lr, double* loss, int flags);
+/* binary file: "OPHCNN1\0", int32 blocks, int32 channels, int64 n, n float32 */
+int oph_cnn_save(oph_ctx* ctx, oph_cnn* net, const char* path);
+int oph_cnn_load(oph_ctx* ctx, oph_cnn* net, const char* path);
+
 #ifdef __cplusplus
 }
 #endif

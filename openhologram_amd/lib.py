@@ -91,6 +91,16 @@ def load():
         "oph_wrp_hologram": (I, [P, I, D, D, D, D, I, I, I, C.POINTER(Points), P, I]),
         "oph_depthmap": (I, [P, C.POINTER(DmSpec), I, P, P, P, P, I]),
         "oph_pc_fresnel": (I, [P, I, I, D, D, C.POINTER(Points), I, P, I]),
+        "oph_cnn_create": (I, [P, I, I, C.c_uint64, C.POINTER(P)]),
+        "oph_cnn_destroy": (None, [P, P]),
+        "oph_cnn_num_params": (I64, [P]),
+        "oph_cnn_get_params": (I, [P, P, P, I]),
+        "oph_cnn_set_params": (I, [P, P, P]),
+        "oph_cnn_forward": (I, [P, P, P, P, I, I, I, I]),
+        "oph_cnn_gradients": (I, [P, P, P, P, I, I, I, C.POINTER(D), I]),
+        "oph_cnn_train_step": (I, [P, P, P, P, I, I, I, C.c_float, C.POINTER(D), I]),
+        "oph_cnn_save": (I, [P, P, C.c_char_p]),
+        "oph_cnn_load": (I, [P, P, C.c_char_p]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -281,6 +291,77 @@ class Context:
         self.check(self.lib.oph_depthmap(self.h, C.byref(spec), nc, ptr(lam), ptr(intensity), ptr(depth),
                                          ptr(dst), flags))
         return dst
+
+
+class Cnn:
+    """oph_cnn: the residual correction network (NHWC float32 tensors)."""
+
+    def __init__(self, ctx, blocks=8, channels=32, seed=1):
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(ctx.lib.oph_cnn_create(ctx.h, blocks, channels, seed, C.byref(h)))
+        self.h = h
+        self.blocks, self.channels = blocks, channels
+        self.n = ctx.lib.oph_cnn_num_params(h)
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            self.ctx.lib.oph_cnn_destroy(self.ctx.h, self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self, which=0):
+        out = np.zeros(self.n, np.float32)
+        self.ctx.check(self.ctx.lib.oph_cnn_get_params(self.ctx.h, self.h, ptr(out), which))
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.n
+        self.ctx.check(self.ctx.lib.oph_cnn_set_params(self.ctx.h, self.h, ptr(p)))
+
+    def forward(self, x, out=None, device=False, shape=None):
+        """y = net(x); x (B, H, W, 2) float32 (a complex64 (B, H, W) field
+        is the same memory).  `shape` overrides x.shape for device arrays."""
+        B, H, W, _ = shape or x.shape
+        if not device:
+            x = np.ascontiguousarray(x, np.float32)
+        if out is None:
+            out = self.ctx.empty((B, H, W, 2), np.float32) if device else np.zeros(x.shape, np.float32)
+        self.ctx.check(self.ctx.lib.oph_cnn_forward(self.ctx.h, self.h, ptr(x), ptr(out), B, H, W,
+                                                    DEVICE_PTR if device else 0))
+        return out
+
+    def gradients(self, x, t, device=False):
+        B, H, W, _ = x.shape
+        if not device:
+            x = np.ascontiguousarray(x, np.float32)
+            t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_double()
+        self.ctx.check(self.ctx.lib.oph_cnn_gradients(self.ctx.h, self.h, ptr(x), ptr(t), B, H, W,
+                                                      C.byref(loss), DEVICE_PTR if device else 0))
+        return loss.value, self.params(1)
+
+    def train_step(self, x, t, lr, device=False):
+        B, H, W, _ = x.shape
+        if not device:
+            x = np.ascontiguousarray(x, np.float32)
+            t = np.ascontiguousarray(t, np.float32)
+        loss = C.c_double()
+        self.ctx.check(self.ctx.lib.oph_cnn_train_step(self.ctx.h, self.h, ptr(x), ptr(t), B, H, W, lr,
+                                                       C.byref(loss), DEVICE_PTR if device else 0))
+        return loss.value
+
+    def save(self, path):
+        self.ctx.check(self.ctx.lib.oph_cnn_save(self.ctx.h, self.h, path.encode()))
+
+    def load(self, path):
+        self.ctx.check(self.ctx.lib.oph_cnn_load(self.ctx.h, self.h, path.encode()))
 
 
 class DeviceArray:

@@ -80,32 +80,44 @@ def fresnel_points(n=1_000_000, planes=64, seed=4, z0=100e-3, z1=120e-3, half=15
     return x, y, p.astype(np.int32), zp[p], a, ph
 
 
+def _box_blur(u, k):
+    """k x k box blur with edge padding (cumulative sums, separable)."""
+    r = k // 2
+    for axis in (0, 1):
+        pad = [(0, 0), (0, 0)]
+        pad[axis] = (r + 1, r)
+        c = np.cumsum(np.pad(u, pad, mode="edge"), axis=axis)
+        n = u.shape[axis]
+        hi = np.take(c, np.arange(k, k + n), axis=axis)
+        lo = np.take(c, np.arange(0, n), axis=axis)
+        u = (hi - lo) / k
+    return u
+
+
 def depth_scene(w=1920, h=1080, seed=3, colors=3, blur=9, gaussians=8):
     """Depth-map workload: depth = sum of `gaussians` seeded 2-D Gaussians
     (centre uniform over the image, sigma U[50,300] px scaled by the image
-    size / 1920, weight U[0.5,1]) normalised to [0,1] and quantised to u8;
+    width / 1920, weight U[0.5,1]) normalised to [0,1] and quantised to u8;
     intensity per colour = seeded U[0,1) noise box-blurred (blur x blur),
-    quantised to u8.  Returns (intensity[colors, h, w] u8, depth[h, w] u8)."""
+    normalised and quantised to u8.  Returns (intensity[colors, h, w] u8,
+    depth[h, w] u8)."""
     g = SplitMix64(seed)
-    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    xs = np.arange(w, dtype=np.float64)
+    ys = np.arange(h, dtype=np.float64)
     d = np.zeros((h, w))
     s = w / 1920.0
     for _ in range(gaussians):
         cx, cy = g.uniform(1, 0, w)[0], g.uniform(1, 0, h)[0]
         sig = g.uniform(1, 50, 300)[0] * s
         wt = g.uniform(1, 0.5, 1.0)[0]
-        d += wt * np.exp(-((xx - cx) ** 2 + (yy - cy) ** 2) / (2 * sig * sig))
+        gx = np.exp(-(xs - cx) ** 2 / (2 * sig * sig))
+        gy = np.exp(-(ys - cy) ** 2 / (2 * sig * sig))
+        d += wt * np.outer(gy, gx)
     d = (d - d.min()) / max(d.max() - d.min(), 1e-300)
     dep = np.floor(d * 255 + 0.5).astype(np.uint8)
     imgs = []
-    k = np.ones(blur) / blur
     for _ in range(colors):
-        u = g.uniform(w * h).reshape(h, w)
-        # separable box blur with edge padding
-        up = np.pad(u, ((blur // 2, blur // 2), (0, 0)), mode="edge")
-        u = np.apply_along_axis(lambda c: np.convolve(c, k, mode="valid"), 0, up)
-        up = np.pad(u, ((0, 0), (blur // 2, blur // 2)), mode="edge")
-        u = np.apply_along_axis(lambda r: np.convolve(r, k, mode="valid"), 1, up)
+        u = _box_blur(g.uniform(w * h).reshape(h, w), blur)
         u = (u - u.min()) / max(u.max() - u.min(), 1e-300)
         imgs.append(np.floor(u * 255 + 0.5).astype(np.uint8))
     return np.stack(imgs), dep
