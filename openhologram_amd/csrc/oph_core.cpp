@@ -1,5 +1,6 @@
 // oph_core.cpp -- context lifetime, errors, device memory helpers, timing,
 // and the KETI file readers of the C ABI.
+#include <algorithm>
 #include <cerrno>
 #include <cmath>
 #include <cstdlib>
@@ -199,10 +200,27 @@ int oph_copy_to_host(oph_ctx* ctx, void* dst, const void* src, size_t bytes) {
     return OPH_OK;
 }
 
+// Plain streaming copy, 16 B per lane per iteration (grid-stride).  Besides
+// serving oph_device_copy it is the calibration kernel for hardware byte
+// counters: it reads and writes exactly `n16` * 16 bytes.
+__global__ void __launch_bounds__(256) copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                   size_t n16) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += stride) dst[i] = src[i];
+}
+
 int oph_device_copy(oph_ctx* ctx, void* dst, const void* src, size_t bytes) {
     int rc = oph_enter(ctx);
     if (rc) return rc;
-    OPH_HIP_CHECK(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    const bool aligned = (((uintptr_t)dst | (uintptr_t)src) & 15) == 0;
+    const size_t n16 = aligned ? bytes / 16 : 0;
+    if (n16) {
+        const unsigned grid = (unsigned)std::min<size_t>((n16 + 255) / 256, (size_t)std::max(ctx->cu_count, 1) * 16);
+        OPH_LAUNCH(ctx, "copy_kernel", copy_kernel, dim3(grid), dim3(256), 0, (const uint4*)src, (uint4*)dst, n16);
+    }
+    if (bytes > n16 * 16)
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync((char*)dst + n16 * 16, (const char*)src + n16 * 16, bytes - n16 * 16,
+                                          hipMemcpyDeviceToDevice, ctx->stream));
     OPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
     return OPH_OK;
 }
