@@ -272,6 +272,97 @@ int oph_cnn_train_step(oph_ctx* ctx, oph_cnn* net, const float* x, const float* 
 int oph_cnn_save(oph_ctx* ctx, oph_cnn* net, const char* path);
 int oph_cnn_load(oph_ctx* ctx, oph_cnn* net, const char* path);
 
+/* ---- encoding and image files ----
+ * 8-bit normalisation of a float fringe (nx x ny, row-major): min and max
+ * over the array, vertical flip, dst[(ny-1-y) nx + x] =
+ * (unsigned char)(((v - min)/(max - min)) * 255 + 0.5) with the reference's
+ * float/double operation order -- replaces ophNormalize,
+ * K/oph_common/oph_common.cpp:92-116 (bit-exact).  A constant input (max ==
+ * min, undefined in the reference) gives all zeros.  flags: OPH_DEVICE_PTR. */
+int oph_normalize_u8(oph_ctx* ctx, const float* src, unsigned char* dst, int nx, int ny, int flags);
+
+/* 8-bit scaling of a double array without flip: dst[i] = (unsigned char)
+ * (unsigned)((v - min)/(max - min) * 255) -- the scaling of
+ * HologramGenerator::Write_Result_image and Write_Simulation_image
+ * (DM/Hologram/src/HologramGenerator.cpp:528-565,
+ * HologramGenerator_CPU.cpp:619-658).  Constant input gives zeros.
+ * flags: OPH_DEVICE_PTR. */
+int oph_scale_u8(oph_ctx* ctx, const double* src, unsigned char* dst, int64_t n, int flags);
+
+/* 8-bit grayscale BMP with a 256-entry gray palette, byte-for-byte the file
+ * ophSaveFileBmp writes (K/oph_common/oph_common.cpp:49-90): 1078-byte
+ * header, rows stored as given with no row padding.  `path` is used as is
+ * (the reference appends ".bmp"). */
+int oph_save_bmp8(const unsigned char* pixels, int width, int height, const char* path);
+
+/* 24-bit BMP of packed RGB888 pixels (3*width bytes per row, top row
+ * first): the file Write_Result_image saves through QImage
+ * (HologramGenerator.cpp:553-559): bottom-up rows, BGR order, rows padded
+ * to 4 bytes. */
+int oph_save_bmp24(const unsigned char* rgb, int width, int height, const char* path);
+
+/* Reads an uncompressed 8-bit (palette), 24-bit or 32-bit BMP as 8-bit gray,
+ * top row first: the Grayscale8 input of ReadImageDepth
+ * (HologramGenerator.cpp:304-383; colour pixels use the gray formula
+ * (11 r + 16 g + 5 b)/32).  With dst == NULL only the size is returned;
+ * otherwise width*height bytes are written (OPH_EINVAL if cap is smaller). */
+int oph_load_bmp_gray8(const char* path, int* width, int* height, unsigned char* dst, int64_t cap);
+
+/* ---- point-cloud files of the WRP path ----
+ * Whitespace-separated x y z triples (any line layout) -- OPHWRP::readxyz,
+ * W/openwrp/WRP.cpp:7-55.  With xyz == NULL only *n is set; otherwise up to
+ * cap points are written as 3 doubles each. */
+int oph_load_xyz(const char* path, double* xyz, int64_t cap, int64_t* n);
+
+/* CWO binary point files -- cwoPLS::Load, CWO/cwoPLS.cpp:48-95: int32 count,
+ * then count (x, y, z) triples of int32 (".3d") or float32 (".3df"), chosen
+ * by the file extension. */
+int oph_load_cwo_points(const char* path, double* xyz, int64_t cap, int64_t* n);
+
+/* ---- depth-map configuration file ----
+ * The positional config_openholo.txt read by HologramGenerator::readConfig
+ * (DM/Hologram/src/HologramGenerator.cpp:65-229): the same line skips and
+ * fields; the pixel number and pitch fields are arithmetic expressions
+ * (numbers, + - * /, parentheses, unary signs) evaluated in double;
+ * RENDER_DEPTH is "a:b" or a list of level numbers.  OPH_EIO on a missing
+ * file or an unparsable field. */
+#define OPH_DM_MAX_RENDER 4096
+typedef struct {
+    char source_folder[256], image_prefix[128], depth_prefix[128];
+    char result_folder[256], result_prefix[128], sim_prefix[128];
+    int flag_static_image, start_of_frame_numbering, number_of_frame, number_of_digit;
+    int transform_method, propagation_method, encoding_method;
+    double field_lens, lambda;
+    int pn[2];
+    double pp[2];
+    double near_depthmap, far_depthmap;   /* ordered: near <= far */
+    int flag_change_depth_quantization, default_depth_quantization, number_of_depth_quantization;
+    int num_of_depth;                     /* as the reference selects it */
+    int n_render_depth;
+    int render_depth[OPH_DM_MAX_RENDER];
+    int random_phase;
+    double test_pixel_number_scale, eye_length, eye_pupil_diameter, eye_center_xy[2], focus_distance;
+    int sim_type, sim_step_num;
+    double sim_from, sim_to;
+} oph_dm_config;
+int oph_dm_config_load(const char* path, oph_dm_config* cfg);
+
+/* ---- numerical reconstruction (eye model) ----
+ * Propagation of a real fringe (nx x ny, double) to the eye pupil through
+ * the field lens and imaging onto the retina with the eye focused at
+ * focus_distance -- HologramGenerator::ReconstructImage /
+ * Test_Propagation_to_Eye_Pupil / Reconstruction
+ * (DM/Hologram/src/HologramGenerator_CPU.cpp:415-614): shifted FFT, field-lens
+ * chirp, eye-centre circular shift, eye-lens chirp with the anti-aliasing and
+ * pupil masks, shifted FFT, magnitude.  nx, ny: powers of two (16..8192).
+ * out: nx x ny doubles (the reference's sim_final_).  flags: OPH_DEVICE_PTR. */
+typedef struct {
+    double pitch_x, pitch_y, lambda, field_lens;
+    double eye_length, eye_pupil_diameter, eye_center_x, eye_center_y, focus_distance;
+} oph_eye_spec;
+int oph_dm_reconstruct(oph_ctx* ctx, const oph_eye_spec* spec, const double* fringe, int nx, int ny,
+                       double* out, int flags);
+
 #ifdef __cplusplus
 }
 #endif

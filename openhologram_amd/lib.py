@@ -43,6 +43,32 @@ class DmSpec(C.Structure):
                 ("rho_im", C.c_double), ("layer_lo", C.c_int), ("layer_hi", C.c_int)]
 
 
+class DmConfig(C.Structure):
+    """oph_dm_config: the positional depth-map configuration file."""
+    _fields_ = [("source_folder", C.c_char * 256), ("image_prefix", C.c_char * 128),
+                ("depth_prefix", C.c_char * 128), ("result_folder", C.c_char * 256),
+                ("result_prefix", C.c_char * 128), ("sim_prefix", C.c_char * 128),
+                ("flag_static_image", C.c_int), ("start_of_frame_numbering", C.c_int),
+                ("number_of_frame", C.c_int), ("number_of_digit", C.c_int),
+                ("transform_method", C.c_int), ("propagation_method", C.c_int), ("encoding_method", C.c_int),
+                ("field_lens", C.c_double), ("lambda_", C.c_double), ("pn", C.c_int * 2), ("pp", C.c_double * 2),
+                ("near_depthmap", C.c_double), ("far_depthmap", C.c_double),
+                ("flag_change_depth_quantization", C.c_int), ("default_depth_quantization", C.c_int),
+                ("number_of_depth_quantization", C.c_int), ("num_of_depth", C.c_int),
+                ("n_render_depth", C.c_int), ("render_depth", C.c_int * 4096), ("random_phase", C.c_int),
+                ("test_pixel_number_scale", C.c_double), ("eye_length", C.c_double),
+                ("eye_pupil_diameter", C.c_double), ("eye_center_xy", C.c_double * 2),
+                ("focus_distance", C.c_double), ("sim_type", C.c_int), ("sim_step_num", C.c_int),
+                ("sim_from", C.c_double), ("sim_to", C.c_double)]
+
+
+class EyeSpec(C.Structure):
+    """oph_eye_spec: eye model of the reconstruction."""
+    _fields_ = [(n, C.c_double) for n in ("pitch_x", "pitch_y", "lambda_", "field_lens", "eye_length",
+                                          "eye_pupil_diameter", "eye_center_x", "eye_center_y",
+                                          "focus_distance")]
+
+
 FRESNEL_METHODS = {"valu": 0, "bf16": 1, "bf16_split": 2, "fp16": 3, "fp16_split": 4}
 DM_OUT = {"fringe": (0x0000, np.float64), "fringe32": (0x0100, np.float32), "field": (0x0200, np.complex64)}
 
@@ -101,6 +127,15 @@ def load():
         "oph_cnn_train_step": (I, [P, P, P, P, I, I, I, C.c_float, C.POINTER(D), I]),
         "oph_cnn_save": (I, [P, P, C.c_char_p]),
         "oph_cnn_load": (I, [P, P, C.c_char_p]),
+        "oph_normalize_u8": (I, [P, P, P, I, I, I]),
+        "oph_scale_u8": (I, [P, P, P, I64, I]),
+        "oph_save_bmp8": (I, [P, I, I, C.c_char_p]),
+        "oph_save_bmp24": (I, [P, I, I, C.c_char_p]),
+        "oph_load_bmp_gray8": (I, [C.c_char_p, C.POINTER(I), C.POINTER(I), P, I64]),
+        "oph_load_xyz": (I, [C.c_char_p, P, I64, C.POINTER(I64)]),
+        "oph_load_cwo_points": (I, [C.c_char_p, P, I64, C.POINTER(I64)]),
+        "oph_dm_config_load": (I, [C.c_char_p, C.POINTER(DmConfig)]),
+        "oph_dm_reconstruct": (I, [P, C.POINTER(EyeSpec), P, I, I, P, I]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -193,6 +228,38 @@ class Context:
         n, t = C.c_int(), C.c_double()
         self.check(self.lib.oph_profile_get(self.h, kernel.encode(), C.byref(n), C.byref(t)))
         return n.value, t.value
+
+    # --- encoding and reconstruction ---
+    def normalize_u8(self, src, device=False, dst=None):
+        """ophNormalize: float fringe (ny, nx) -> flipped u8 (ny, nx)."""
+        ny, nx = src.shape
+        if not device:
+            src = np.ascontiguousarray(src, np.float32)
+            dst = np.zeros((ny, nx), np.uint8)
+        self.check(self.lib.oph_normalize_u8(self.h, ptr(src), ptr(dst), nx, ny, DEVICE_PTR if device else 0))
+        return dst
+
+    def scale_u8(self, src, device=False, dst=None):
+        """Write_Result_image scaling of a double array -> u8 (same shape)."""
+        if not device:
+            src = np.ascontiguousarray(src, np.float64)
+            dst = np.zeros(src.shape, np.uint8)
+        n = int(np.prod(src.shape))
+        self.check(self.lib.oph_scale_u8(self.h, ptr(src), ptr(dst), n, DEVICE_PTR if device else 0))
+        return dst
+
+    def dm_reconstruct(self, fringe, pitch, lambda_, field_lens, eye_length, pupil, eye_center=(0.0, 0.0),
+                       focus=0.0, device=False, out=None):
+        """Eye-model reconstruction magnitude of a real fringe (ny, nx)."""
+        px, py = pitch if hasattr(pitch, "__len__") else (pitch, pitch)
+        ny, nx = fringe.shape
+        spec = EyeSpec(px, py, lambda_, field_lens, eye_length, pupil, eye_center[0], eye_center[1], focus)
+        if not device:
+            fringe = np.ascontiguousarray(fringe, np.float64)
+            out = np.zeros((ny, nx), np.float64)
+        self.check(self.lib.oph_dm_reconstruct(self.h, C.byref(spec), ptr(fringe), nx, ny, ptr(out),
+                                               DEVICE_PTR if device else 0))
+        return out
 
     # --- generators ---
     def pc_rs(self, spec, x, y, z, amp=None, phase=None, out=None, fmt=OUT_C128, accumulate=False,
@@ -421,6 +488,57 @@ def spec_from_keti_config(path):
     if rc != OPH_OK:
         raise OphError(rc, f"cannot parse {path}")
     return s
+
+
+def _check(rc, what):
+    if rc != OPH_OK:
+        raise OphError(rc, what)
+
+
+def save_bmp8(path, pixels):
+    """8-bit palette BMP, byte-identical to the reference's ophSaveFileBmp."""
+    a = np.ascontiguousarray(pixels, np.uint8)
+    _check(load().oph_save_bmp8(ptr(a), a.shape[1], a.shape[0], path.encode()), f"cannot write {path}")
+
+
+def save_bmp24(path, rgb):
+    """24-bit BMP of packed RGB888 rows (h x 3w bytes or h x w x 3)."""
+    a = np.ascontiguousarray(rgb, np.uint8)
+    w = a.shape[1] if a.ndim == 3 else a.shape[1] // 3
+    _check(load().oph_save_bmp24(ptr(a), w, a.shape[0], path.encode()), f"cannot write {path}")
+
+
+def load_bmp_gray8(path):
+    lib = load()
+    w, h = C.c_int(), C.c_int()
+    _check(lib.oph_load_bmp_gray8(path.encode(), C.byref(w), C.byref(h), None, 0), f"cannot read {path}")
+    out = np.zeros((h.value, w.value), np.uint8)
+    _check(lib.oph_load_bmp_gray8(path.encode(), C.byref(w), C.byref(h), ptr(out), out.size), f"cannot read {path}")
+    return out
+
+
+def _load_points3(fn, path):
+    n = C.c_int64()
+    _check(fn(path.encode(), None, 0, C.byref(n)), f"cannot load {path}")
+    out = np.zeros((n.value, 3), np.float64)
+    _check(fn(path.encode(), ptr(out), n.value, C.byref(n)), f"cannot load {path}")
+    return out
+
+
+def load_xyz(path):
+    """(n, 3) float64 x, y, z of a WRP xyz text file."""
+    return _load_points3(load().oph_load_xyz, path)
+
+
+def load_cwo_points(path):
+    """(n, 3) float64 of a CWO .3d (int32) or .3df (float32) point file."""
+    return _load_points3(load().oph_load_cwo_points, path)
+
+
+def dm_config_load(path):
+    c = DmConfig()
+    _check(load().oph_dm_config_load(path.encode(), C.byref(c)), f"cannot parse {path}")
+    return c
 
 
 def load_keti_points(path):

@@ -27,6 +27,13 @@
  *   orc_depthmap          one-pixel scene vs closed form, layer linearity,
  *                         hand-computed 8x8 encode, FFT-shift cancellation
  *   orc_normalize_u8      bit-exact vs the compiled reference ophNormalize
+ *   orc_scale_u8          hand-computed values (min, max, interior, constant)
+ *   orc_dm_reconstruct    closed form: with the eye focused at distance 0 the
+ *                         eye-lens chirp cancels the field-lens chirp and the
+ *                         result is N |fringe| reflected about the centre
+ *                         (numpy FFT check of two shifted transforms); the
+ *                         eye-centre shift moves the image by the expected
+ *                         number of pixels; pupil mask energy monotonicity
  */
 #include <math.h>
 #include <stdint.h>
@@ -610,4 +617,96 @@ void orc_depthmap(int w, int h, const unsigned char* img8, const unsigned char* 
     free(U);
     free(lev);
     free(didx);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Depth-map result encoding and eye-model reconstruction                    */
+/* ------------------------------------------------------------------------ */
+
+/*
+ * 8-bit scaling of Write_Result_image (DM/Hologram/src/HologramGenerator.cpp:
+ * 539-556): min/max with the reference's if / else-if update (equivalent to
+ * plain min/max, since min <= max throughout), then
+ * data[x] = (uint)((v - min)/(max - min) * 255) stored in an unsigned char.
+ * A constant input (division 0/0 in the reference) gives 0 here.
+ */
+void orc_scale_u8(const double* src, unsigned char* dst, int64_t n) {
+    double mn = src[0], mx = src[0];
+    for (int64_t i = 0; i < n; ++i) {
+        if (mn > src[i]) mn = src[i];
+        else if (mx < src[i]) mx = src[i];
+    }
+    for (int64_t i = 0; i < n; ++i)
+        dst[i] = mx > mn ? (unsigned char)(unsigned)((src[i] - mn) / (mx - mn) * 255) : 0;
+}
+
+/* HologramGenerator::circshift (HologramGenerator_CPU.cpp:663-680). */
+static void ref_circshift(const c128* in, c128* out, int shift_x, int shift_y, int nx, int ny) {
+    for (int i = 0; i < nx; i++)
+        for (int j = 0; j < ny; j++) {
+            int ti = (i + shift_x) % nx;
+            if (ti < 0) ti += nx;
+            int tj = (j + shift_y) % ny;
+            if (tj < 0) tj += ny;
+            out[ti + (size_t)tj * nx] = in[i + (size_t)j * nx];
+        }
+}
+
+/*
+ * Eye-model reconstruction, FP64, as ReconstructImage with
+ * test_pixel_number_scale 1 and one focus distance
+ * (HologramGenerator_CPU.cpp:415-614):
+ *   Test_Propagation_to_Eye_Pupil (:496-532): hh = fftwShift(fringe, +1),
+ *     pe = lambda f / (n p), xe = -E/2 + pe x, ye = (E_y/2 - pe_y) - pe_y y,
+ *     hh_c = hh exp(i pi/lambda/f (xe^2 + ye^2));
+ *   Reconstruction (:536-614): circshift(hh_c, -round(cx/pe_x), round(cy/pe_y)),
+ *     f_eye = L (f - d)/(L + f - d), f_eff = f_eye L/(f_eye - L),
+ *     hh_e = hh_s exp(i pi/lambda/f_eff r^2) [sqrt(r^2) < |lambda f_eff/(2 max pe)|]
+ *            [sqrt(r^2) < pupil/2], fftwShift(hh_e, +1),
+ *     sim_final = |hh_e exp(i pi/lambda/L (xr^2 + yr^2))|.
+ * out: nx x ny.  Sizes: powers of two (the oracle's FFT).
+ */
+void orc_dm_reconstruct(const double* fringe, int nx, int ny, double ppx, double ppy, double lambda,
+                        double f_field, double eye_length, double pupil, double cx, double cy,
+                        double focus, double* out, int nthreads) {
+    const size_t n = (size_t)nx * ny;
+    c128* hh = (c128*)malloc(sizeof(c128) * n);
+    c128* hs = (c128*)malloc(sizeof(c128) * n);
+    for (size_t k = 0; k < n; ++k) { hh[k].re = fringe[k]; hh[k].im = 0.0; }
+    ref_fftwshift(hh, nx, ny, 1, 0, nthreads);
+    const double pp_ex = lambda * f_field / (nx * ppx), pp_ey = lambda * f_field / (ny * ppy);
+    const double E_x = pp_ex * nx, E_y = pp_ey * ny;
+    for (size_t p = 0; p < n; ++p) {
+        const double x = (double)(p % nx), y = (double)(p / nx);
+        const double xe = (-E_x / 2.0) + (pp_ex * x);
+        const double ye = (E_y / 2.0 - pp_ey) - (pp_ey * y);
+        hh[p] = cmul(hh[p], cexpi(M_PI / lambda / f_field * (xe * xe + ye * ye)));
+    }
+    const int sx = (int)lround(cx / pp_ex), sy = (int)lround(cy / pp_ey);
+    ref_circshift(hh, hs, -sx, sy, nx, ny);
+    const double f_eye = eye_length * (f_field - focus) / (eye_length + (f_field - focus));
+    const double f_eff = f_eye * eye_length / (f_eye - eye_length);
+    const double amax = fabs(lambda * f_eff / (2.0 * (pp_ex > pp_ey ? pp_ex : pp_ey)));
+    for (size_t p = 0; p < n; ++p) {
+        const double x = (double)(p % nx), y = (double)(p / nx);
+        const double xe = (-E_x / 2.0) + (pp_ex * x);
+        const double ye = (E_y / 2.0 - pp_ey) - (pp_ey * y);
+        const double r = sqrt(xe * xe + ye * ye);
+        const int m = (r < amax) && (r < pupil / 2.0);
+        c128 v = cmul(hs[p], cexpi(M_PI / lambda / f_eff * (xe * xe + ye * ye)));
+        hs[p].re = m ? v.re : 0.0;
+        hs[p].im = m ? v.im : 0.0;
+    }
+    ref_fftwshift(hs, nx, ny, 1, 0, nthreads);
+    const double pr_x = lambda * eye_length / E_x, pr_y = lambda * eye_length / E_y;
+    const double R_x = pr_x * nx, R_y = pr_y * ny;
+    for (size_t p = 0; p < n; ++p) {
+        const double x = (double)(p % nx), y = (double)(p / nx);
+        const double xr = (-R_x / 2.0) + (pr_x * x);
+        const double yr = (R_y / 2.0 - pr_y) - (pr_y * y);
+        const c128 v = cmul(hs[p], cexpi(M_PI / lambda / eye_length * (xr * xr + yr * yr)));
+        out[p] = sqrt(v.re * v.re + v.im * v.im);
+    }
+    free(hh);
+    free(hs);
 }

@@ -49,6 +49,8 @@ def lib():
         _lib.orc_rs.argtypes = [C.POINTER(_Spec), C.c_int64, P, P, P, P, P, P, C.c_int]
         _lib.orc_keti_fringe_f32.argtypes = [C.POINTER(_KetiSpecF32), C.c_int, P, P, P, C.c_int]
         _lib.orc_normalize_u8.argtypes = [P, P, C.c_int, C.c_int]
+        _lib.orc_scale_u8.argtypes = [P, P, C.c_int64]
+        _lib.orc_dm_reconstruct.argtypes = [P, C.c_int, C.c_int] + [C.c_double] * 9 + [P, C.c_int]
         _lib.orc_fresnel.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int64,
                                      P, P, P, P, P, P, P, C.c_int64, P, C.c_int]
         _lib.orc_wrp.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
@@ -126,6 +128,24 @@ def normalize_u8(src):
     dst = np.zeros(src.shape, np.uint8)
     lib().orc_normalize_u8(_p(src), _p(dst), src.shape[1], src.shape[0])
     return dst
+
+
+def scale_u8(src):
+    """Write_Result_image 8-bit scaling of a double array (no flip)."""
+    src = np.ascontiguousarray(src, np.float64)
+    dst = np.zeros(src.shape, np.uint8)
+    lib().orc_scale_u8(_p(src), _p(dst), src.size)
+    return dst
+
+
+def dm_reconstruct(fringe, ppx, ppy, lam, field_lens, eye_length, pupil, cx, cy, focus, nthreads=0):
+    """Eye-model reconstruction magnitude (ny x nx doubles) of a real fringe."""
+    f = np.ascontiguousarray(fringe, np.float64)
+    _pow2(*f.shape)
+    out = np.zeros(f.shape, np.float64)
+    lib().orc_dm_reconstruct(_p(f), f.shape[1], f.shape[0], ppx, ppy, lam, field_lens, eye_length, pupil,
+                             cx, cy, focus, _p(out), nthreads)
+    return out
 
 
 def fresnel(nx, ny, pitch, lam, x, y, z, amp, phase=None, rows=None, cols=None, nthreads=0):

@@ -9,6 +9,7 @@
 //   ophLoadSpecConfig(char*, OphSpec*)          ophgen_pointcloud.cpp:112-158
 //   ophGenCghPointCloud(const OphPointCloud*,…) ophgen_pointcloud.cpp:195-249
 //   ophNormalize(float*, uchar*, nx, ny)        oph_common.cpp:92-116
+//   ophSaveFileBmp(uchar*, w, h, char*)         oph_common.cpp:49-90
 #include "ophgen_pointcloud.h"
 #include "oph_common.h"
 #include <cstdlib>
@@ -71,6 +72,11 @@ void ref_keti_fringe(const float* pts, int n, const float* spec13, float* dst) {
 
 void ref_keti_normalize(float* src, unsigned char* dst, int nx, int ny) {
     openholo::ophNormalize(src, dst, nx, ny);
+}
+
+// Writes `<name>.bmp` through the reference writer.
+void ref_keti_save_bmp(unsigned char* pixels, int w, int h, const char* name) {
+    openholo::ophSaveFileBmp(pixels, w, h, const_cast<char*>(name));
 }
 
 }  // extern "C"
