@@ -112,6 +112,11 @@ typedef struct oph_points {
  * lines. */
 int oph_spec_from_keti_config(const char* path, oph_spec* spec);
 
+/* The 17 value strings of the same file in file order (the reference keeps
+ * field 8, the filter shape, as text), for callers that map them onto the
+ * reference's float OphSpec themselves. */
+int oph_keti_config_fields(const char* path, char values[17][64]);
+
 /* Loads a KETI .dat point cloud (first line P, then `idx x y z phase amp`
  * per line, idx == line index) -- replaces ophLoadPointCloud,
  * K/ophgen_pointcloud/ophgen_pointcloud.cpp:13-109.  With xyzpa == NULL only

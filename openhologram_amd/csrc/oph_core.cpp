@@ -4,6 +4,7 @@
 #include <cerrno>
 #include <cmath>
 #include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "oph_internal.h"
@@ -280,6 +281,25 @@ int oph_spec_from_keti_config(const char* path, oph_spec* spec) {
     spec->tilt_deg[0] = d(15);
     spec->tilt_deg[1] = d(16);
     return OPH_OK;
+}
+
+int oph_keti_config_fields(const char* path, char values[17][64]) {
+    if (!path || !values) return OPH_EINVAL;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return OPH_EIO;
+    int i = 0;
+    char line[512];
+    while (std::fgets(line, sizeof line, f)) {
+        char title[256] = {0}, value[256] = {0};
+        if (std::sscanf(line, "%255s = %255s", title, value) < 2) value[0] = 0;
+        if (i < 17) {
+            std::strncpy(values[i], value, 63);
+            values[i][63] = 0;
+        }
+        ++i;
+    }
+    std::fclose(f);
+    return i == 17 ? OPH_OK : OPH_EIO;
 }
 
 // KETI .dat point cloud, ophgen_pointcloud.cpp:13-47: first token P, then per
