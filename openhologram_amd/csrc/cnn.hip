@@ -17,6 +17,7 @@
 // channels; the weight gradient is a GEMM over pixels (M = output channels,
 // N = 9 x input channels + 1 bias column) split into pixel chunks whose
 // partial sums are reduced in a fixed order (deterministic).
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <vector>
@@ -24,6 +25,7 @@
 #include "oph_internal.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 struct oph_cnn {
@@ -33,7 +35,8 @@ struct oph_cnn {
     float* grads = nullptr;
     float* m = nullptr;       // Adam moments
     float* v = nullptr;
-    float* wt = nullptr;      // flipped/transposed weights for input gradients
+    bf16x8* wf = nullptr;     // BF16 B fragments of every forward convolution
+    bf16x8* wb = nullptr;     // ... of every input-gradient convolution (flipped taps, transposed channels)
     int64_t step = 0;
     // activation workspace
     float* act = nullptr;
@@ -69,7 +72,7 @@ struct Layout {
 
 struct ConvArgs {
     const float* in;      // [B][H][W][CIN]
-    const float* w;       // [COUT][9][CIN]
+    const bf16x8* wf;     // B fragments [KSTEPS][64 lanes] (pack_weights)
     const float* bias;    // [COUT] or null
     const float* res;     // residual added after the bias (or null), [..][COUT]
     float* out;           // [B][H][W][COUT]
@@ -108,65 +111,156 @@ __device__ __forceinline__ bf16x8 load_patch(const ConvArgs& a, int b, int y, in
     return v;
 }
 
-// B operand for lane (output channel n, k0..k0+7)
+// Tiled forward convolution.  A workgroup of 8 waves owns an 8-row x 32-
+// column pixel tile of one image: it stages the tile and its one-pixel halo
+// in LDS once, as BF16 (pixel stride padded by 16 bytes so that the 16-byte
+// A-fragment reads of a half-wave fall on distinct banks), and each wave
+// keeps its B fragments (the weights, BF16) in registers for the whole tile.
+// Wave w computes tile rows 2w and 2w + 1: 32 pixels x 32 output channels
+// each, K = 9 CIN in steps of 16 on v_mfma_f32_32x32x16_bf16 (one weight
+// fragment feeds two MFMAs).  Lane l: pixel l%32, k half l/32;
+// accumulator register v of lane l is pixel 8(v/4) + 4(l/32) + v%4 of the
+// row, channel l%32.
+constexpr int CT_H = 8, CT_W = 32;
+constexpr int CT_WAVES = CT_H / 2;
+
+template <int CIN>
+__host__ __device__ constexpr int tile_stride() { return CIN % 8 == 0 ? CIN + 8 : CIN; }
+
 template <int CIN, int COUT>
-__device__ __forceinline__ bf16x8 load_weights(const float* w, int n, int k0) {
+__global__ __launch_bounds__(64 * CT_WAVES) void conv3x3_fwd(ConvArgs a) {
+    constexpr int K = 9 * CIN;
+    constexpr int KSTEPS = (K + 15) / 16;
+    constexpr int PS = tile_stride<CIN>();
+    constexpr int TWH = CT_W + 2;
+    __shared__ __attribute__((aligned(16))) __bf16 tile[(CT_H + 2) * TWH * PS];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int x0 = blockIdx.x * CT_W, y0 = blockIdx.y * CT_H, b = blockIdx.z;
+    const size_t img = (size_t)b * a.H * a.W;
+
+    // stage the tile (+ halo) as BF16; outside the image is zero padding
+    if constexpr (CIN % 4 == 0) {
+        constexpr int Q = CIN / 4;  // float4 per pixel
+        for (int i = threadIdx.x; i < (CT_H + 2) * TWH * Q; i += 64 * CT_WAVES) {
+            const int pix = i / Q, q = i % Q;
+            const int yy = y0 + pix / TWH - 1, xx = x0 + pix % TWH - 1;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                v = reinterpret_cast<const float4*>(a.in + (img + (size_t)yy * a.W + xx) * CIN)[q];
+            bf16x4 w4;
+            w4[0] = (__bf16)v.x;
+            w4[1] = (__bf16)v.y;
+            w4[2] = (__bf16)v.z;
+            w4[3] = (__bf16)v.w;
+            *reinterpret_cast<bf16x4*>(tile + pix * PS + 4 * q) = w4;
+        }
+    } else {
+        for (int i = threadIdx.x; i < (CT_H + 2) * TWH * CIN; i += 64 * CT_WAVES) {
+            const int pix = i / CIN, c = i % CIN;
+            const int yy = y0 + pix / TWH - 1, xx = x0 + pix % TWH - 1;
+            float v = 0.f;
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = a.in[(img + (size_t)yy * a.W + xx) * CIN + c];
+            tile[pix * PS + c] = (__bf16)v;
+        }
+    }
+    // weights -> registers (one 16-byte fragment per k step)
+    const int n = lane & 31, kh = lane >> 5;
+    bf16x8 Bf[KSTEPS];
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) Bf[s] = a.wf[s * 64 + lane];
+    __syncthreads();
+
+    const int col = lane & 31;
+    f32x16 acc[2] = {(f32x16)0.f, (f32x16)0.f};
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) {
+        const int k0 = 16 * s + 8 * kh;
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int r = 2 * wave + h;
+            bf16x8 A;
+            if constexpr (CIN % 8 == 0) {
+                const int tap = k0 / CIN, c = k0 % CIN;
+                if (tap < 9) {
+                    const __bf16* src = tile + ((r + tap / 3) * TWH + col + tap % 3) * PS + c;
+                    A = *reinterpret_cast<const bf16x8*>(src);
+                } else {
+                    A = (bf16x8)(__bf16)0.f;
+                }
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) {
+                    const int k = k0 + e, tap = k / CIN, c = k % CIN;
+                    A[e] = tap < 9 ? tile[((r + tap / 3) * TWH + col + tap % 3) * PS + c] : (__bf16)0.f;
+                }
+            }
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, Bf[s], acc[h], 0, 0, 0);
+        }
+    }
+    if (n >= COUT) return;
+    const float bias = a.bias ? a.bias[n] : 0.f;
+    // residuals first, all in flight together (out may not be assumed
+    // distinct from res by the compiler, which would otherwise serialise
+    // each load behind the previous store)
+    float rv[2][16];
+#pragma unroll
+    for (int h = 0; h < 2; ++h)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int y = y0 + 2 * wave + h, x = x0 + 8 * (v >> 2) + 4 * kh + (v & 3);
+            rv[h][v] = (a.res && y < a.H && x < a.W) ? a.res[(img + (size_t)y * a.W + x) * COUT + n] : 0.f;
+        }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int y = y0 + 2 * wave + h;
+        if (y >= a.H) continue;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int x = x0 + 8 * (v >> 2) + 4 * kh + (v & 3);
+            if (x >= a.W) continue;
+            float o = acc[h][v] + bias;
+            if (a.relu) o = fmaxf(o, 0.f);
+            a.out[(img + (size_t)y * a.W + x) * COUT + n] = o + rv[h][v];
+        }
+    }
+}
+
+// B fragments of convolution layers: fragment (s, lane) holds, for output
+// channel n = lane % 32 and k = 16 s + 8 (lane / 32) + e (e = 0..7; k = tap
+// CIN + c), W[n][tap][c] rounded to BF16 (0 beyond COUT or 9 CIN).  For an
+// input-gradient convolution (flip) the weights of the original layer
+// (CIN_o = cout, COUT_o = cin) are read with flipped taps and transposed
+// channels: W'[n][tap][c] = W_o[c][8 - tap][n].  One launch packs every layer
+// of a pass (blockIdx.y = layer).
+struct PackLayer {
+    int64_t woff, foff;   // parameter offset of W_o, fragment offset
+    int cin, cout, flip;  // channels of the convolution as applied
+};
+constexpr int MAX_BLOCKS = 16;
+constexpr int MAX_PACK = 2 * MAX_BLOCKS + 2;
+struct PackPlan {
+    PackLayer l[MAX_PACK];
+};
+
+__global__ void pack_weights(const float* __restrict__ P, bf16x8* __restrict__ out, PackPlan plan) {
+    const PackLayer L = plan.l[blockIdx.y];
+    const int K = 9 * L.cin, ks = (K + 15) / 16;
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= ks * 64) return;
+    const int s = f / 64, lane = f % 64, n = lane & 31, k0 = 16 * s + 8 * (lane >> 5);
     bf16x8 v;
 #pragma unroll
     for (int e = 0; e < 8; ++e) {
         const int k = k0 + e;
-        float f = 0.f;
-        if (n < COUT && k < 9 * CIN) f = w[(size_t)n * 9 * CIN + k];
-        v[e] = (__bf16)f;
+        float w = 0.f;
+        if (n < L.cout && k < K) {
+            const int tap = k / L.cin, c = k % L.cin;
+            w = L.flip ? P[L.woff + ((int64_t)c * 9 + (8 - tap)) * L.cout + n]
+                       : P[L.woff + ((int64_t)n * 9 + tap) * L.cin + c];
+        }
+        v[e] = (__bf16)w;
     }
-    return v;
-}
-
-// One wave: 32 consecutive pixels (row-major over B*H*W) x 32 output
-// channels.  Lane l: pixel l%32, k half l/32; accumulator register v of lane
-// l is pixel 8(v/4) + 4(l/32) + v%4, channel l%32.
-template <int CIN, int COUT>
-__global__ __launch_bounds__(256) void conv3x3_fwd(ConvArgs a) {
-    constexpr int K = 9 * CIN;
-    constexpr int KSTEPS = (K + 15) / 16;
-    const int lane = threadIdx.x & 63;
-    const int64_t tile = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int64_t npix = (int64_t)a.B * a.H * a.W;
-    const int64_t p0 = tile * 32;
-    if (p0 >= npix) return;
-    const int64_t p = p0 + (lane & 31);
-    const bool pok = p < npix;
-    const int b = (int)(p / ((int64_t)a.H * a.W));
-    const int rem = (int)(p % ((int64_t)a.H * a.W));
-    const int y = rem / a.W, x = rem % a.W;
-    f32x16 acc = (f32x16)0.f;
-    const int n = lane & 31;
-#pragma unroll 2
-    for (int s = 0; s < KSTEPS; ++s) {
-        const int k0 = 16 * s + 8 * (lane >> 5);
-        const bf16x8 A = pok ? load_patch<CIN>(a, b, y, x, k0) : (bf16x8)(__bf16)0.f;
-        const bf16x8 Bv = load_weights<CIN, COUT>(a.w, n, k0);
-        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, Bv, acc, 0, 0, 0);
-    }
-    if (n >= COUT) return;
-    const float bias = a.bias ? a.bias[n] : 0.f;
-#pragma unroll
-    for (int v = 0; v < 16; ++v) {
-        const int64_t q = p0 + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-        if (q >= npix) continue;
-        float o = acc[v] + bias;
-        if (a.relu) o = fmaxf(o, 0.f);
-        if (a.res) o += a.res[q * COUT + n];
-        a.out[q * COUT + n] = o;
-    }
-}
-
-// flipped taps, transposed channels: wt[c_in][8 - tap][c_out] = w[c_out][tap][c_in]
-__global__ void conv_flip(const float* w, float* wt, int cin, int cout) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= cin * cout * 9) return;
-    const int co = i / (9 * cin), r = i % (9 * cin), tap = r / cin, ci = r % cin;
-    wt[((size_t)ci * 9 + (8 - tap)) * cout + co] = w[i];
+    out[L.foff + f] = v;
 }
 
 // relu'(pre) from the post-activation: g *= [act > 0]
@@ -177,88 +271,118 @@ __global__ void relu_mask(float* g, const float* act, int64_t n) {
 
 // ---- weight gradient -------------------------------------------------------------
 // dW[n][k] = sum_p dY[p][n] X[p + off(tap k)][c k], db[n] = sum_p dY[p][n].
-// GEMM M = COUT (padded to 32), N = 9 CIN + 1 (bias column), K = pixels;
-// workgroup g sums pixel chunk g into part[g][32][NCOL].
+// GEMM M = COUT (padded to 32), N = 9 CIN + 1 (bias column), K = pixels.
+// Pixels are taken in 8-row x 32-column tiles of one image.  A workgroup
+// stages a tile in LDS as BF16: dY transposed ([channel][pixel], so the 8
+// pixels of an A fragment are 16 contiguous bytes) and X as three copies of
+// its channel planes shifted by dx = -1, 0, +1 (so the 8 pixels of a B
+// fragment at any tap are 16 aligned bytes).  Five waves take two of the ten
+// 32-column N tiles each.  Workgroup g accumulates tiles g, g + G, ... in
+// registers (a fixed order) and writes part[g][32][NCOL]; dw_reduce sums the
+// partials in workgroup order (deterministic).
+
+constexpr int DW_WAVES = 5;
+constexpr int DW_MAX_GROUPS = 256;
 
 struct DwArgs {
     const float* dy;   // [npix][COUT]
     const float* x;    // [npix][CIN]
-    float* part;       // [nchunks][32][ncol]
+    float* part;       // [groups][32][ncol]
     int B, H, W;
-    int64_t chunk;     // pixels per workgroup (multiple of 16)
+    int tiles_x, tiles_y;
 };
 
 template <int CIN, int COUT>
-__global__ __launch_bounds__(64) void conv3x3_dw(DwArgs a) {
+__global__ __launch_bounds__(64 * DW_WAVES) void conv3x3_dw(DwArgs a) {
     constexpr int NCOL = 9 * CIN + 1;
     constexpr int NT = (NCOL + 31) / 32;
-    const int lane = threadIdx.x;
-    const int64_t npix = (int64_t)a.B * a.H * a.W;
-    const int64_t pb = (int64_t)blockIdx.x * a.chunk;
-    const int64_t pe = min(npix, pb + a.chunk);
-    f32x16 acc[NT];
+    constexpr int TPW = (NT + DW_WAVES - 1) / DW_WAVES;  // N tiles per wave
+    constexpr int TP = CT_H * CT_W;                      // pixels per tile
+    constexpr int XR = CT_H + 2;                         // X rows incl. halo
+    __shared__ __attribute__((aligned(16))) __bf16 sdy[32 * TP];
+    __shared__ __attribute__((aligned(16))) __bf16 sx[3 * CIN * XR * CT_W];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int row = lane & 31, kh = 8 * (lane >> 5);
+    f32x16 acc[TPW];
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[t] = (f32x16)0.f;
-    const int row = lane & 31;          // output channel (A row) / column within a tile (B)
-    const int kh = 8 * (lane >> 5);     // pixel offset of this lane's 8 values
-    for (int64_t q0 = pb; q0 < pe; q0 += 16) {
-        bf16x8 A;
-        int qb[8], qy[8], qx[8];
-        bool qv[8];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) {
-            const int64_t q = q0 + kh + e;
-            qv[e] = q < pe;
-            const int64_t qq = qv[e] ? q : pb;
-            qb[e] = (int)(qq / ((int64_t)a.H * a.W));
-            const int rem = (int)(qq % ((int64_t)a.H * a.W));
-            qy[e] = rem / a.W;
-            qx[e] = rem % a.W;
-            float f = 0.f;
-            if (qv[e] && row < COUT) f = a.dy[q * COUT + row];
-            A[e] = (__bf16)f;
+    for (int t = 0; t < TPW; ++t) acc[t] = (f32x16)0.f;
+    const int ntiles = a.tiles_x * a.tiles_y * a.B;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int tx = tile % a.tiles_x, ty = (tile / a.tiles_x) % a.tiles_y, b = tile / (a.tiles_x * a.tiles_y);
+        const int x0 = tx * CT_W, y0 = ty * CT_H;
+        const size_t img = (size_t)b * a.H * a.W;
+        __syncthreads();  // the previous tile's reads are done
+        for (int i = threadIdx.x; i < TP * 32; i += 64 * DW_WAVES) {
+            const int px = i / 32, n = i % 32;
+            const int yy = y0 + px / CT_W, xx = x0 + px % CT_W;
+            float v = 0.f;
+            if (n < COUT && yy < a.H && xx < a.W) v = a.dy[(img + (size_t)yy * a.W + xx) * COUT + n];
+            sdy[n * TP + px] = (__bf16)v;
         }
+        for (int i = threadIdx.x; i < XR * (CT_W + 2) * CIN; i += 64 * DW_WAVES) {
+            const int px = i / CIN, c = i % CIN;
+            const int r = px / (CT_W + 2), j = px % (CT_W + 2);   // j = 0..33 <-> x0 - 1 + j
+            const int yy = y0 + r - 1, xx = x0 + j - 1;
+            float v = 0.f;
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = a.x[(img + (size_t)yy * a.W + xx) * CIN + c];
+            const __bf16 bv = (__bf16)v;
+            // shifted copy d (dx = d - 1) holds column jj = j - d at [d][c][r][jj]
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = 32 * t + row;   // (tap, c) or the bias column
-            const int tap = col / CIN, c = col % CIN;
-            const int dyo = tap / 3 - 1, dxo = tap % 3 - 1;
-            bf16x8 Bv;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                float f = 0.f;
-                if (qv[e] && col < NCOL) {
-                    if (col == NCOL - 1) {
-                        f = 1.f;
-                    } else {
-                        const int yy = qy[e] + dyo, xx = qx[e] + dxo;
-                        if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-                            f = a.x[(((size_t)qb[e] * a.H + yy) * a.W + xx) * CIN + c];
-                    }
-                }
-                Bv[e] = (__bf16)f;
+            for (int d = 0; d < 3; ++d) {
+                const int jj = j - d;
+                if (jj >= 0 && jj < CT_W) sx[((d * CIN + c) * XR + r) * CT_W + jj] = bv;
             }
-            acc[t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, Bv, acc[t], 0, 0, 0);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tt = 0; tt < TPW; ++tt) {
+            const int t = wave + DW_WAVES * tt;
+            if (t >= NT) break;
+            const int col = 32 * t + row;   // B column of this lane: (tap, c) or the bias column
+            const int tap = col / CIN, c = col % CIN;
+            const bool is_w = col < NCOL - 1, is_b = col == NCOL - 1;
+            const int dyo = tap / 3, dxo = tap % 3;  // row offset into the halo rows, shifted copy
+#pragma unroll 4
+            for (int s = 0; s < TP / 16; ++s) {
+                const int p0 = 16 * s + kh;               // 8 pixels of one tile row
+                const int pr = p0 / CT_W, pc = p0 % CT_W;
+                const bf16x8 A = *reinterpret_cast<const bf16x8*>(sdy + row * TP + p0);
+                bf16x8 Bv;
+                if (is_w)
+                    Bv = *reinterpret_cast<const bf16x8*>(sx + ((dxo * CIN + c) * XR + pr + dyo) * CT_W + pc);
+                else
+                    Bv = (bf16x8)(__bf16)(is_b ? 1.f : 0.f);
+                acc[tt] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, Bv, acc[tt], 0, 0, 0);
+            }
         }
     }
     float* dst = a.part + (size_t)blockIdx.x * 32 * NCOL;
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
+    for (int tt = 0; tt < TPW; ++tt) {
+        const int t = wave + DW_WAVES * tt;
+        if (t >= NT) break;
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int r = 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);  // output channel
             const int col = 32 * t + (lane & 31);
-            if (col < NCOL) dst[(size_t)r * NCOL + col] = acc[t][v];
+            if (col < NCOL) dst[(size_t)r * NCOL + col] = acc[tt][v];
         }
+    }
 }
 
-// sum the chunk partials in chunk order into the gradient (W and b)
-__global__ void dw_reduce(const float* part, int nchunks, int cout, int ncol, float* gw, float* gb) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Sum of the workgroup partials into the gradient (W and b): one wave per
+// element; lane j adds partials j, j + 64, ... in order, then the lanes are
+// combined by a fixed butterfly (the same order on every run).
+__global__ __launch_bounds__(256) void dw_reduce(const float* part, int groups, int cout, int ncol, float* gw,
+                                                 float* gb) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (i >= cout * ncol) return;
     const int r = i / ncol, col = i % ncol;
     double s = 0.0;
-    for (int c = 0; c < nchunks; ++c) s += part[((size_t)c * 32 + r) * ncol + col];
+    for (int g = lane; g < groups; g += 64) s += part[((size_t)g * 32 + r) * ncol + col];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (lane != 0) return;
     if (col == ncol - 1)
         gb[r] = (float)s;
     else
@@ -314,12 +438,12 @@ __global__ void add_inplace(float* a, const float* b, int64_t n) {
 // ---- host helpers ------------------------------------------------------------------
 
 template <int CIN, int COUT>
-int conv_fwd(oph_ctx* ctx, const float* in, const float* w, const float* bias, const float* res, float* out,
+int conv_fwd(oph_ctx* ctx, const float* in, const bf16x8* wf, const float* bias, const float* res, float* out,
              int B, int H, int W, int relu) {
-    ConvArgs a{in, w, bias, res, out, B, H, W, relu};
-    const int64_t npix = (int64_t)B * H * W;
-    const int64_t tiles = (npix + 31) / 32;
-    OPH_LAUNCH(ctx, "conv3x3_fwd", (conv3x3_fwd<CIN, COUT>), dim3((unsigned)((tiles + 3) / 4)), dim3(256), 0, a);
+    ConvArgs a{in, wf, bias, res, out, B, H, W, relu};
+    const dim3 grid((unsigned)((W + CT_W - 1) / CT_W), (unsigned)((H + CT_H - 1) / CT_H), (unsigned)B);
+    if (B > 65535 || grid.y > 65535) return oph_fail(ctx, OPH_EINVAL, "convolution batch or height too large");
+    OPH_LAUNCH(ctx, "conv3x3_fwd", (conv3x3_fwd<CIN, COUT>), grid, dim3(64 * CT_WAVES), 0, a);
     return OPH_OK;
 }
 
@@ -327,25 +451,63 @@ template <int CIN, int COUT>
 int conv_dw(oph_ctx* ctx, oph_cnn* net, const float* dy, const float* x, int B, int H, int W, float* gw,
             float* gb) {
     constexpr int NCOL = 9 * CIN + 1;
-    const int64_t npix = (int64_t)B * H * W;
-    int64_t chunk = 4096;
-    int64_t nch = (npix + chunk - 1) / chunk;
-    const size_t need = (size_t)nch * 32 * NCOL * sizeof(float);
+    DwArgs a{dy, x, net->part, B, H, W, (W + CT_W - 1) / CT_W, (H + CT_H - 1) / CT_H};
+    const int64_t ntiles = (int64_t)a.tiles_x * a.tiles_y * B;
+    const int groups = (int)std::min<int64_t>(ntiles, DW_MAX_GROUPS);
+    const size_t need = (size_t)groups * 32 * NCOL * sizeof(float);
     if (need > net->part_bytes) return oph_fail(ctx, OPH_ENOMEM, "weight-gradient workspace too small");
-    DwArgs a{dy, x, net->part, B, H, W, chunk};
-    OPH_LAUNCH(ctx, "conv3x3_dw", (conv3x3_dw<CIN, COUT>), dim3((unsigned)nch), dim3(64), 0, a);
-    OPH_LAUNCH(ctx, "dw_reduce", dw_reduce, dim3((COUT * NCOL + 255) / 256), dim3(256), 0, net->part, (int)nch, COUT,
+    if (ntiles > INT32_MAX) return oph_fail(ctx, OPH_EINVAL, "too many pixels");
+    OPH_LAUNCH(ctx, "conv3x3_dw", (conv3x3_dw<CIN, COUT>), dim3((unsigned)groups), dim3(64 * DW_WAVES), 0, a);
+    OPH_LAUNCH(ctx, "dw_reduce", dw_reduce, dim3((COUT * NCOL + 3) / 4), dim3(256), 0, net->part, groups, COUT,
                NCOL, gw, gb);
     return OPH_OK;
 }
 
+// dX = conv(dY, flipped/transposed W): input channels COUT, output CIN
 template <int CIN, int COUT>
-int conv_dx(oph_ctx* ctx, oph_cnn* net, const float* dy, const float* w, float* dx, const float* res, int B, int H,
-            int W) {
-    // dX = conv(dY, flipped/transposed W): input channels COUT, output CIN
-    OPH_LAUNCH(ctx, "conv_flip", conv_flip, dim3((CIN * COUT * 9 + 255) / 256), dim3(256), 0, w, net->wt, CIN,
-               COUT);
-    return conv_fwd<COUT, CIN>(ctx, dy, net->wt, nullptr, res, dx, B, H, W, 0);
+int conv_dx(oph_ctx* ctx, const float* dy, const bf16x8* wfrag, float* dx, const float* res, int B, int H, int W) {
+    return conv_fwd<COUT, CIN>(ctx, dy, wfrag, nullptr, res, dx, B, H, W, 0);
+}
+
+constexpr int ksteps(int cin) { return (9 * cin + 15) / 16; }
+
+// fragment offsets: forward (head, block b conv1/conv2, tail) and input-
+// gradient (tail, block b conv2/conv1) convolutions, 64 fragments per k step
+struct Frags {
+    int NB;
+    int64_t head() const { return 0; }
+    int64_t conv1(int b) const { return 64LL * (ksteps(2) + (int64_t)b * 2 * ksteps(32)); }
+    int64_t conv2(int b) const { return conv1(b) + 64LL * ksteps(32); }
+    int64_t tail() const { return conv1(NB); }
+    int64_t total() const { return tail() + 64LL * ksteps(32); }
+    // input-gradient convolutions
+    int64_t tail_t() const { return 0; }
+    int64_t conv2_t(int b) const { return 64LL * (ksteps(2) + (int64_t)b * 2 * ksteps(32)); }
+    int64_t conv1_t(int b) const { return conv2_t(b) + 64LL * ksteps(32); }
+};
+
+int pack(oph_ctx* ctx, oph_cnn* net, bool transposed) {
+    const Layout L{net->ch, net->blocks};
+    const Frags F{net->blocks};
+    PackPlan plan{};
+    int n = 0;
+    if (!transposed) {
+        plan.l[n++] = {L.head_w(), F.head(), 2, 32, 0};
+        for (int b = 0; b < net->blocks; ++b) {
+            plan.l[n++] = {L.w1(b), F.conv1(b), 32, 32, 0};
+            plan.l[n++] = {L.w2(b), F.conv2(b), 32, 32, 0};
+        }
+        plan.l[n++] = {L.tail_w(), F.tail(), 32, 2, 0};
+    } else {
+        plan.l[n++] = {L.tail_w(), F.tail_t(), 2, 32, 1};
+        for (int b = 0; b < net->blocks; ++b) {
+            plan.l[n++] = {L.w2(b), F.conv2_t(b), 32, 32, 1};
+            plan.l[n++] = {L.w1(b), F.conv1_t(b), 32, 32, 1};
+        }
+    }
+    OPH_LAUNCH(ctx, "pack_weights", pack_weights, dim3((unsigned)((64 * ksteps(32) + 255) / 256), (unsigned)n),
+               dim3(256), 0, net->params, transposed ? net->wb : net->wf, plan);
+    return OPH_OK;
 }
 
 uint64_t splitmix(uint64_t& s) {
@@ -386,8 +548,7 @@ int ensure_workspace(oph_ctx* ctx, oph_cnn* net, int64_t npix, Acts& A) {
     A.g1 = reinterpret_cast<float*>(p); p += per;
     A.g2 = reinterpret_cast<float*>(p); p += per;
     A.y = reinterpret_cast<float*>(p);
-    const int64_t nch = (npix + 4095) / 4096;
-    const size_t pneed = (size_t)nch * 32 * (9 * 32 + 1) * sizeof(float);
+    const size_t pneed = (size_t)DW_MAX_GROUPS * 32 * (9 * 32 + 1) * sizeof(float);
     if (pneed > net->part_bytes) {
         if (net->part) hipFree(net->part);
         net->part = nullptr;
@@ -404,14 +565,18 @@ int ensure_workspace(oph_ctx* ctx, oph_cnn* net, int64_t npix, Acts& A) {
 int forward(oph_ctx* ctx, oph_cnn* net, const float* x, int B, int H, int W, Acts& A) {
     const Layout L{net->ch, net->blocks};
     const float* P = net->params;
+    const Frags F{net->blocks};
+    const bf16x8* WF = net->wf;
     int rc;
-    if ((rc = conv_fwd<2, 32>(ctx, x, P + L.head_w(), P + L.head_b(), nullptr, A.h[0], B, H, W, 0))) return rc;
+    if ((rc = pack(ctx, net, false))) return rc;
+    if ((rc = conv_fwd<2, 32>(ctx, x, WF + F.head(), P + L.head_b(), nullptr, A.h[0], B, H, W, 0))) return rc;
     for (int b = 0; b < net->blocks; ++b) {
-        if ((rc = conv_fwd<32, 32>(ctx, A.h[b], P + L.w1(b), P + L.b1(b), nullptr, A.a[b + 1], B, H, W, 1))) return rc;
-        if ((rc = conv_fwd<32, 32>(ctx, A.a[b + 1], P + L.w2(b), P + L.b2(b), A.h[b], A.h[b + 1], B, H, W, 0)))
+        if ((rc = conv_fwd<32, 32>(ctx, A.h[b], WF + F.conv1(b), P + L.b1(b), nullptr, A.a[b + 1], B, H, W, 1)))
+            return rc;
+        if ((rc = conv_fwd<32, 32>(ctx, A.a[b + 1], WF + F.conv2(b), P + L.b2(b), A.h[b], A.h[b + 1], B, H, W, 0)))
             return rc;
     }
-    return conv_fwd<32, 2>(ctx, A.h[net->blocks], P + L.tail_w(), P + L.tail_b(), x, A.y, B, H, W, 0);
+    return conv_fwd<32, 2>(ctx, A.h[net->blocks], WF + F.tail(), P + L.tail_b(), x, A.y, B, H, W, 0);
 }
 
 // gradients of loss = mean (y - t)^2 into net->grads; loss into *loss
@@ -426,20 +591,23 @@ int backward(oph_ctx* ctx, oph_cnn* net, const float* x, const float* t, int B, 
     float* dy = A.g2;  // 2-channel output gradient (fits in a C-channel buffer)
     OPH_LAUNCH(ctx, "mse_grad", mse_grad, dim3(nb), dim3(256), 0, A.y, t, dy, n2, net->red);
     OPH_LAUNCH(ctx, "sum_partials", sum_partials, dim3(1), dim3(64), 0, net->red, (int)nb);
+    const Frags F{net->blocks};
+    const bf16x8* WB = net->wb;
     int rc;
+    if ((rc = pack(ctx, net, true))) return rc;
     // tail: y = x + conv(h_NB)
     if ((rc = conv_dw<32, 2>(ctx, net, dy, A.h[net->blocks], B, H, W, G + L.tail_w(), G + L.tail_b()))) return rc;
     float* dh = A.g0;
-    if ((rc = conv_dx<32, 2>(ctx, net, dy, P + L.tail_w(), dh, nullptr, B, H, W))) return rc;
+    if ((rc = conv_dx<32, 2>(ctx, dy, WB + F.tail_t(), dh, nullptr, B, H, W))) return rc;
     for (int b = net->blocks - 1; b >= 0; --b) {
         // h_{b+1} = h_b + conv2(a); a = relu(conv1(h_b))
         float* da = A.g1;
         if ((rc = conv_dw<32, 32>(ctx, net, dh, A.a[b + 1], B, H, W, G + L.w2(b), G + L.b2(b)))) return rc;
-        if ((rc = conv_dx<32, 32>(ctx, net, dh, P + L.w2(b), da, nullptr, B, H, W))) return rc;
+        if ((rc = conv_dx<32, 32>(ctx, dh, WB + F.conv2_t(b), da, nullptr, B, H, W))) return rc;
         OPH_LAUNCH(ctx, "relu_mask", relu_mask, dim3((unsigned)((nc + 255) / 256)), dim3(256), 0, da, A.a[b + 1], nc);
         if ((rc = conv_dw<32, 32>(ctx, net, da, A.h[b], B, H, W, G + L.w1(b), G + L.b1(b)))) return rc;
         // dh_b = dh_{b+1} + conv1^T(da)
-        if ((rc = conv_dx<32, 32>(ctx, net, da, P + L.w1(b), A.g2, dh, B, H, W))) return rc;
+        if ((rc = conv_dx<32, 32>(ctx, da, WB + F.conv1_t(b), A.g2, dh, B, H, W))) return rc;
         float* tmp = dh;
         dh = A.g2;
         A.g2 = tmp;
@@ -462,8 +630,8 @@ int oph_cnn_create(oph_ctx* ctx, int blocks, int channels, uint64_t seed, oph_cn
     int rc = oph_enter(ctx);
     if (rc) return rc;
     if (!out) return oph_fail(ctx, OPH_EINVAL, "null output");
-    if (channels != 32 || blocks < 0 || blocks > CMAX)
-        return oph_fail(ctx, OPH_ENOTSUP, "channels must be 32 and blocks 0..%d", CMAX);
+    if (channels != 32 || blocks < 0 || blocks > MAX_BLOCKS)
+        return oph_fail(ctx, OPH_ENOTSUP, "channels must be 32 and blocks 0..%d", MAX_BLOCKS);
     oph_cnn* n = new oph_cnn();
     n->blocks = blocks;
     n->ch = channels;
@@ -472,7 +640,8 @@ int oph_cnn_create(oph_ctx* ctx, int blocks, int channels, uint64_t seed, oph_cn
     const size_t pb = (size_t)n->nparams * sizeof(float);
     if (hipMalloc(&n->params, pb) != hipSuccess || hipMalloc(&n->grads, pb) != hipSuccess ||
         hipMalloc(&n->m, pb) != hipSuccess || hipMalloc(&n->v, pb) != hipSuccess ||
-        hipMalloc(&n->wt, (size_t)9 * 32 * 32 * sizeof(float)) != hipSuccess ||
+        hipMalloc(&n->wf, (size_t)Frags{blocks}.total() * sizeof(bf16x8)) != hipSuccess ||
+        hipMalloc(&n->wb, (size_t)Frags{blocks}.total() * sizeof(bf16x8)) != hipSuccess ||
         hipMalloc(&n->red, (size_t)(1 << 22) * sizeof(double)) != hipSuccess) {
         hipGetLastError();
         oph_cnn_destroy(ctx, n);
@@ -510,7 +679,7 @@ void oph_cnn_destroy(oph_ctx* ctx, oph_cnn* n) {
     if (!n) return;
     if (ctx) hipSetDevice(ctx->device);
     hipDeviceSynchronize();
-    for (void* p : {(void*)n->params, (void*)n->grads, (void*)n->m, (void*)n->v, (void*)n->wt, (void*)n->act,
+    for (void* p : {(void*)n->params, (void*)n->grads, (void*)n->m, (void*)n->v, (void*)n->wf, (void*)n->wb, (void*)n->act,
                     (void*)n->part, (void*)n->red})
         if (p) hipFree(p);
     delete n;
