@@ -407,6 +407,22 @@ def bench_learned(ctx, args):
             "kernels": {"full": per_full, "learned": per_l}}
     if kt:
         line["conv_fwd_tflops"] = conv_flops / (kt["launches_per_step"] * kt["avg_us"] * 1e-6) / 1e12
+    if not args.no_cpu:
+        # the reference CPU path for this workload is the 256-layer hologram
+        # (one colour at 1024^2): FP64 oracle on 4 populated layers, scaled
+        from oracle import oracle as O
+        idx = 256 - dep.astype(np.int64)
+        counts = np.bincount(idx.ravel(), minlength=257)[1:]
+        layers = int((counts > 0).sum())
+        l0 = int(np.argmax(counts)) + 1
+        t0 = time.perf_counter()
+        O.depthmap(img[0], dep, N, N, learned.PITCH, learned.PITCH, learned.LAMBDA, learned.NEAR, learned.FAR, 256,
+                   layers=(l0, l0 + 3), nthreads=host_cores())
+        dt = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": 1.0 / (dt * layers / 4), "unit": "holograms/s (256-layer path)",
+                                "cores": host_cores(), "kind": "port",
+                                "sample": f"FP64 oracle, 4 populated layers of the 1024^2 scene on the 2048^2 grid "
+                                          f"({dt:.2f} s incl. one encode), scaled to {layers} layers"}
     return line
 
 
