@@ -64,6 +64,8 @@ struct DmArgs {
     const double* lc;              // per layer: cx, cy, K (3 doubles)
     float ux, uy;                  // lambda^2 / Sx^2, lambda^2 / Sy^2
     int exact;                     // FP64 transfer path
+    int horner;                    // equally spaced layers: chunked recurrence in dm_rows_h
+    double dcx, dcy, dK;           // layer-step constants (z_{l+1} - z_l replacing z_l)
     double lam2_sx2, lam2_sy2, inv_lambda;
     float2 rho;
     int encode, cy1, cy2;          // crop range (centred rows) when encoding
@@ -213,7 +215,7 @@ __device__ __forceinline__ float2 dm_transfer(const DmArgs& a, float tx, double 
 // inverse row transform and the store of the scene columns into T.
 template <int LX>
 __device__ __forceinline__ void finish_row(const DmArgs& a, const float2 (&acc)[16], int ky, float2* line,
-                                           const fft::Twiddles<LX>& Tw, int t) {
+                                           const float2* __restrict__ tw, int t) {
     constexpr int TPL = LX / 16;
     const int yc = (ky + a.Ny / 2) & (a.Ny - 1);
     if (a.spectrum) {
@@ -231,7 +233,7 @@ __device__ __forceinline__ void finish_row(const DmArgs& a, const float2 (&acc)[
 #pragma unroll
     for (int m = 0; m < 16; ++m) line[fft::lane_pad<LX>(t, m)] = acc[m];
     __syncthreads();
-    fft::line<LX, +1>(line, Tw, t);
+    fft::line_tw<LX, +1>(line, tw, t);
     float2* dst = a.T + (size_t)ky * a.w;
     for (int x = t; x < a.w; x += TPL) dst[x] = line[fft::pad16(x + a.ox)];
     __syncthreads();
@@ -305,8 +307,155 @@ __global__ __launch_bounds__(LX / 16) void dm_rows(DmArgs a, const float2* __res
         acca[m] = fft::cmul(acca[m], a.rho);
         accb[m] = fft::cmul(accb[m], a.rho);
     }
-    finish_row<LX>(a, acca, kya, line, Tw, t);
-    if (has_b) finish_row<LX>(a, accb, kyb, line, Tw, t);
+    finish_row<LX>(a, acca, kya, line, twx, t);
+    if (has_b) finish_row<LX>(a, accb, kyb, line, twx, t);
+}
+
+// Row pass for equally spaced layers (no field lens).  The transfer of
+// layer l is H_l = exp(i 2 pi z_l Phi(kx, ky)) with one Phi for all layers,
+// so over a chunk of 16 consecutive layers starting at c,
+//   sum_j H_{c+j} G_{c+j} = H_c sum_j R^j G_{c+j},   R = exp(i 2 pi dz Phi),
+// evaluated by Horner from the top layer down (T = T R + G_l): one complex
+// multiply-add per element and layer instead of the phase reduction, series
+// and sin/cos, which are evaluated once per chunk for the anchor H_c (the
+// recurrence's rounding stays within ~16 ulp).  The row transforms are
+// radix-8 with LX/8 threads (8 elements per thread) so that R and T of both
+// rows fit in registers next to the transform; the chunk sums S accumulate
+// in LDS, each element touched only by its owner thread.
+template <int LX>
+__host__ __device__ constexpr size_t rows_h_lds() { return (fft::padded_len(LX) + 2 * (size_t)LX) * sizeof(float2); }
+
+// finish_row for the radix-8 layout (8 elements per thread)
+template <int LX>
+__device__ __forceinline__ void finish_row8(const DmArgs& a, const float2 (&acc)[8], int ky, float2* line,
+                                            const float2* __restrict__ tw8, int t) {
+    constexpr int TPL = LX / 8;
+    const int yc = (ky + a.Ny / 2) & (a.Ny - 1);
+    if (a.spectrum) {
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int kx = t + m * TPL;
+            const int xc = (kx + LX / 2) & (LX - 1);
+            const float sg = ((kx + ky) & 1) ? -1.f : 1.f;
+            a.spectrum[(size_t)yc * LX + xc] = make_float2(sg * acc[m].x, sg * acc[m].y);
+        }
+    }
+    const bool kept = !a.encode || (yc >= a.cy1 && yc <= a.cy2);
+    if (!kept) return;  // uniform over the workgroup
+#pragma unroll
+    for (int m = 0; m < 8; ++m) line[fft::pad16(t + m * TPL)] = acc[m];
+    __syncthreads();
+    fft::line_r8<LX, +1>(line, tw8, t);
+    float2* dst = a.T + (size_t)ky * a.w;
+    for (int x = t; x < a.w; x += TPL) dst[x] = line[fft::pad16(x + a.ox)];
+    __syncthreads();
+}
+
+template <int LX>
+__global__ __launch_bounds__(LX / 8) void dm_rows_h(DmArgs a, const float2* __restrict__ tw8) {
+    constexpr int TPL = LX / 8;
+    constexpr int PL = fft::padded_len(LX);
+    constexpr int CH = 16;
+    extern __shared__ float2 sm[];
+    float2* line = sm;
+    float2* Sa = sm + PL;
+    float2* Sb = Sa + LX;
+    const int t = threadIdx.x;
+    const int kya = blockIdx.x;
+    const int kyb = a.Ny - kya;
+    const bool has_b = kya != 0 && kya != a.Ny / 2;
+    const int sya = sgnidx(kya, a.Ny) + 1;
+    const int syb = sgnidx(kyb & (a.Ny - 1), a.Ny) + 1;
+    float2 Ra[8], Rb[8];
+    {
+        double tya = a.dcy * (double)sya * sya;
+        tya -= rint(tya);
+        double tyb = a.dcy * (double)syb * syb;
+        tyb -= rint(tyb);
+        const float dKf = (float)a.dK;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int kx = t + m * TPL;
+            const int sx = sgnidx(kx, LX);
+            const double sx2 = (double)sx * sx;
+            double txd = a.dcx * sx2;
+            txd -= rint(txd);
+            const float usx = a.ux * (float)(sx * sx);
+            Ra[m] = dm_transfer<false>(a, (float)txd, tya, usx, sya, a.dK, dKf, sx2);
+            Rb[m] = dm_transfer<false>(a, (float)txd, tyb, usx, syb, a.dK, dKf, sx2);
+            Sa[kx] = Sb[kx] = make_float2(0.f, 0.f);
+        }
+    }
+    const size_t lstride = (size_t)(a.Ny / 2 + 1) * a.w;
+    for (int cs = a.lo; cs <= a.hi; cs += CH) {
+        const int ce = min(a.hi, cs + CH - 1);
+        bool anyc = false;
+        for (int l = cs; l <= ce; ++l) anyc |= a.any[l - 1] != 0;
+        if (!anyc) continue;
+        float2 Ta[8], Tb[8];
+#pragma unroll
+        for (int m = 0; m < 8; ++m) Ta[m] = Tb[m] = make_float2(0.f, 0.f);
+        for (int l = ce; l >= cs; --l) {
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                Ta[m] = fft::cmul(Ta[m], Ra[m]);
+                Tb[m] = fft::cmul(Tb[m], Rb[m]);
+            }
+            if (!a.any[l - 1]) continue;
+            const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * a.w;
+            const unsigned char* fl = a.flags + (size_t)(l - 1) * a.ngroups;
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int x = t + m * TPL - a.ox;
+                const bool in = x >= 0 && x < a.w;
+                const float2 v = in ? src[x] : make_float2(0.f, 0.f);
+                const bool present = in && fl[x / GROUP];
+                line[fft::pad16(t + m * TPL)] = present ? v : make_float2(0.f, 0.f);
+            }
+            __syncthreads();
+            fft::line_r8<LX, -1>(line, tw8, t);
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int kx = t + m * TPL;
+                Ta[m] = fft::cadd(Ta[m], line[fft::pad16(kx)]);
+                const float2 gm = line[fft::pad16((LX - kx) & (LX - 1))];
+                Tb[m] = fft::cadd(Tb[m], make_float2(gm.x, -gm.y));
+            }
+            __syncthreads();
+        }
+        // anchor: S += H_cs T
+        const double cx = a.lc[3 * (cs - 1)], cy = a.lc[3 * (cs - 1) + 1], K = a.lc[3 * (cs - 1) + 2];
+        const float Kf = (float)K;
+        double tya = cy * (double)sya * sya;
+        tya -= rint(tya);
+        double tyb = cy * (double)syb * syb;
+        tyb -= rint(tyb);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int kx = t + m * TPL;
+            const int sx = sgnidx(kx, LX);
+            const double sx2 = (double)sx * sx;
+            double txd = cx * sx2;
+            txd -= rint(txd);
+            const float usx = a.ux * (float)(sx * sx);
+            const float2 ha = dm_transfer<false>(a, (float)txd, tya, usx, sya, K, Kf, sx2);
+            Sa[kx] = fft::cadd(Sa[kx], fft::cmul(ha, Ta[m]));
+            if (has_b) {
+                const float2 hb = dm_transfer<false>(a, (float)txd, tyb, usx, syb, K, Kf, sx2);
+                Sb[kx] = fft::cadd(Sb[kx], fft::cmul(hb, Tb[m]));
+            }
+        }
+    }
+    float2 acca[8], accb[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+        const int kx = t + m * TPL;
+        acca[m] = fft::cmul(Sa[kx], a.rho);
+        accb[m] = fft::cmul(Sb[kx], a.rho);
+    }
+    __syncthreads();
+    finish_row8<LX>(a, acca, kya, line, tw8, t);
+    if (has_b) finish_row8<LX>(a, accb, kyb, line, tw8, t);
 }
 
 // ---- final column pass ---------------------------------------------------------
@@ -377,6 +526,21 @@ template <int LX>
 int launch_dm_rows(oph_ctx* ctx, const DmArgs& a) {
     const float2* tw = oph_twiddles(ctx, LX);
     if (!tw) return OPH_ENOMEM;
+    if constexpr (LX >= 64 && LX <= 4096) {
+        if (a.horner) {
+            const float2* tw8 = oph_twiddles(ctx, LX, 8);
+            if (!tw8) return OPH_ENOMEM;
+            static bool attr = false;
+            if (!attr) {
+                hipFuncSetAttribute(reinterpret_cast<const void*>(&dm_rows_h<LX>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)rows_h_lds<LX>());
+                hipGetLastError();
+                attr = true;
+            }
+            OPH_LAUNCH(ctx, "dm_rows", dm_rows_h<LX>, dim3(a.Ny / 2 + 1), dim3(LX / 8), rows_h_lds<LX>(), a, tw8);
+            return OPH_OK;
+        }
+    }
     if (a.exact)
         OPH_LAUNCH(ctx, "dm_rows", (dm_rows<LX, true>), dim3(a.Ny / 2 + 1), dim3(LX / 16), 0, a, tw);
     else
@@ -498,9 +662,24 @@ extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, cons
     const double umax = lmax * lmax * (fxm * fxm + fym * fym);
     const bool exact = umax >= 0.5 || zmax * pow(umax, 6) * (21.0 / 1024.0) > 1e-7;
 
+    // equally spaced z (no field lens): the chunked recurrence of dm_rows_h
+    // applies when every step equals the first to far below a phase ulp
+    std::vector<double> zs(L);
+    for (int l = 0; l < L; ++l)
+        zs[l] = s->field_lens > 0.0 ? -s->field_lens * lev[l] / (lev[l] - s->field_lens) : lev[l];
+    const double dz = L > 1 ? zs[1] - zs[0] : 0.0;
+    double dev = 0.0;
+    for (int l = 1; l < L; ++l) dev = fmax(dev, fabs((zs[l] - zs[l - 1]) - dz));
+    const double phimax = lmax * (fxm * fxm + fym * fym) / 2.0 + 0.25 * umax * umax / lmax;  // turns per metre
+    const bool horner = !exact && L > 1 && hi > lo && dev * phimax < 1e-9;
+
     for (int c = 0; c < ncolor; ++c) {
         const double lam = lambdas[c];
         DmArgs a{};
+        a.horner = horner ? 1 : 0;
+        a.dcx = dz * lam / (2.0 * Sx * Sx);
+        a.dcy = dz * lam / (2.0 * Sy * Sy);
+        a.dK = dz / lam;
         a.w = w; a.h = h; a.Nx = Nx; a.Ny = Ny;
         a.ox = (Nx - w) / 2;
         a.oy = (Ny - h) / 2;

@@ -17,19 +17,20 @@
 #include "fft_engine.h"
 #include "fft_kernels.h"
 
-const float2* oph_twiddles(oph_ctx* ctx, int L) {
+const float2* oph_twiddles(oph_ctx* ctx, int L, int radix) {
     int lg = 0;
     while ((1 << lg) < L) ++lg;
-    if ((1 << lg) != L || lg >= 24) {
-        oph_fail(ctx, OPH_EINVAL, "FFT length %d is not a power of two", L);
+    if ((1 << lg) != L || lg >= 24 || (radix != 16 && radix != 8)) {
+        oph_fail(ctx, OPH_EINVAL, "FFT length %d / radix %d unsupported", L, radix);
         return nullptr;
     }
-    if (ctx->tw[lg]) return static_cast<const float2*>(ctx->tw[lg]);
-    // per-stage tables in the order fft::line walks the stages
+    void** cache = radix == 16 ? ctx->tw : ctx->tw8;
+    if (cache[lg]) return static_cast<const float2*>(cache[lg]);
+    // per-stage tables in the order the line transforms walk the stages
     std::vector<float2> h;
     int Ns = 1, rem = L;
     while (rem > 1) {
-        const int R = (rem % 16 == 0) ? 16 : rem;
+        const int R = (rem % radix == 0) ? radix : rem;
         if (Ns > 1)
             for (int r = 1; r < R; ++r)
                 for (int k = 0; k < Ns; ++k) {
@@ -52,7 +53,7 @@ const float2* oph_twiddles(oph_ctx* ctx, int L) {
         oph_fail(ctx, OPH_EDEVICE, "twiddle table upload failed");
         return nullptr;
     }
-    ctx->tw[lg] = d;
+    cache[lg] = d;
     return static_cast<const float2*>(d);
 }
 

@@ -256,6 +256,83 @@ __device__ __forceinline__ void line(float2* buf, const Twiddles<L>& T, int t) {
     if constexpr (RL > 1) stage<L, RL, SIGN, TPL>(buf, T.wl, Ns, t);
 }
 
+// Twiddles of one stage straight from the table (L1/L2-resident), for
+// kernels whose registers are needed elsewhere.  Stage tables are
+// concatenated from Ns = 16 on: radix-16 stage s (Ns = 16^s) starts at
+// sum_{s' < s} 15 * 16^s'; the last radix-RL stage follows them.
+template <int L, int R, int TPL>
+__device__ __forceinline__ void load_stage_tw(const float2* __restrict__ tw, int off, int Ns, int t,
+                                              float2 (&w)[L / R / TPL][R]) {
+#pragma unroll
+    for (int p = 0; p < L / R / TPL; ++p) {
+        const int k = (t + p * TPL) & (Ns - 1);
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[p][r] = tw[off + (r - 1) * Ns + k];
+    }
+}
+
+// line() with each stage's twiddles loaded from the table just before it.
+template <int L, int SIGN>
+__device__ __forceinline__ void line_tw(float2* buf, const float2* __restrict__ tw, int t) {
+    constexpr int TPL = L / 16;
+    constexpr int A = n16(L);
+    constexpr int RL = last_radix(L);
+    float2 w0[1][16];
+    stage<L, 16, SIGN, TPL>(buf, w0, 1, t);
+    int Ns = 16, off = 0;
+#pragma unroll
+    for (int s = 1; s < A; ++s) {
+        float2 w[1][16];
+        load_stage_tw<L, 16, TPL>(tw, off, Ns, t, w);
+        stage<L, 16, SIGN, TPL>(buf, w, Ns, t);
+        off += 15 * Ns;
+        Ns *= 16;
+    }
+    if constexpr (RL > 1) {
+        float2 w[L / RL / TPL][RL];
+        load_stage_tw<L, RL, TPL>(tw, off, Ns, t, w);
+        stage<L, RL, SIGN, TPL>(buf, w, Ns, t);
+    }
+}
+
+// Radix-8 variant: L/8 threads per line, each holding 8 elements
+// (t + m L/8, m = 0..7), radix-8 stages plus one radix-2/4 stage, twiddles
+// of each stage loaded from the radix-8 table just before it.  Half the
+// per-thread state of the radix-16 line, for kernels that keep a lot of
+// other data in registers.
+__host__ __device__ constexpr int n8(int L) {
+    int n = 0;
+    while (L >= 8 && L % 8 == 0) { L /= 8; ++n; }
+    return n;
+}
+__host__ __device__ constexpr int last_radix8(int L) {
+    while (L >= 8 && L % 8 == 0) L /= 8;
+    return L;
+}
+
+template <int L, int SIGN>
+__device__ __forceinline__ void line_r8(float2* buf, const float2* __restrict__ tw, int t) {
+    constexpr int TPL = L / 8;
+    constexpr int A = n8(L);
+    constexpr int RL = last_radix8(L);
+    float2 w0[1][8];
+    stage<L, 8, SIGN, TPL>(buf, w0, 1, t);
+    int Ns = 8, off = 0;
+#pragma unroll
+    for (int s = 1; s < A; ++s) {
+        float2 w[1][8];
+        load_stage_tw<L, 8, TPL>(tw, off, Ns, t, w);
+        stage<L, 8, SIGN, TPL>(buf, w, Ns, t);
+        off += 7 * Ns;
+        Ns *= 8;
+    }
+    if constexpr (RL > 1) {
+        float2 w[L / RL / TPL][RL];
+        load_stage_tw<L, RL, TPL>(tw, off, Ns, t, w);
+        stage<L, RL, SIGN, TPL>(buf, w, Ns, t);
+    }
+}
+
 // Convenience: load the twiddles and transform.
 template <int L, int SIGN>
 __device__ __forceinline__ void line(float2* buf, const float2* __restrict__ tw, int t) {
@@ -267,6 +344,7 @@ __device__ __forceinline__ void line(float2* buf, const float2* __restrict__ tw,
 }  // namespace fft
 
 // Device twiddle tables of all stages of a length-L transform (layout above,
-// FP64-computed, rounded to float), cached per length in the context.
-// Returns nullptr on failure (error recorded).
-const float2* oph_twiddles(oph_ctx* ctx, int L);
+// FP64-computed, rounded to float), cached per length in the context; radix
+// 16 (line, line_tw) or 8 (line_r8).  Returns nullptr on failure (error
+// recorded).
+const float2* oph_twiddles(oph_ctx* ctx, int L, int radix = 16);

@@ -143,6 +143,8 @@ void oph_destroy(oph_ctx* ctx) {
         if (ctx->slot_ptr[i]) hipFree(ctx->slot_ptr[i]);
     for (void* t : ctx->tw)
         if (t) hipFree(t);
+    for (void* t : ctx->tw8)
+        if (t) hipFree(t);
     for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
     if (ctx->ev_begin) hipEventDestroy(ctx->ev_begin);
     if (ctx->ev_end) hipEventDestroy(ctx->ev_end);

@@ -49,7 +49,8 @@ struct oph_ctx {
     size_t slot_size[SLOT_COUNT] = {};
     std::string err;
     int cu_count = 256;
-    void* tw[24] = {};  // FFT twiddle tables by log2(length)
+    void* tw[24] = {};  // FFT twiddle tables by log2(length), radix-16 stages
+    void* tw8[24] = {}; // the same for radix-8 stages
     unsigned* pending_overflow = nullptr;  // device flag left by an async WRP call
     std::vector<double> host_scratch;      // per-call host tables uploaded to the device
 };
