@@ -26,6 +26,7 @@
 // range), the small remainder in FP32.  When the truncated series would be
 // off by more than 1e-7 turns at the grid corner the kernel evaluates the
 // square root per element in FP64 instead.
+#include <algorithm>
 #include <cmath>
 
 #include "fft_engine.h"
@@ -180,11 +181,89 @@ __global__ __launch_bounds__(512) void asm_rows_inv(const float2* __restrict__ w
     }
 }
 
+// Persistent row passes for long rows (one line per workgroup): each
+// workgroup walks rows r, r + G, ...; the next row is fetched into registers
+// while the current one is transformed in LDS, so the global loads overlap
+// the transform (twiddles per stage from the table, to leave registers for
+// the prefetch).
+template <int L>
+__global__ __launch_bounds__(L / 16) void asm_rows_fwd_p(const float2* __restrict__ in, int nx, int ny, int ox,
+                                                         int oy, float2* __restrict__ work,
+                                                         const float2* __restrict__ tw) {
+    constexpr int TPL = L / 16;
+    constexpr int PL = fft::padded_len(L);
+    __shared__ float2 buf[PL];
+    const int t = threadIdx.x;
+    float2 nxt[16];
+    int r = blockIdx.x;
+#pragma unroll
+    for (int m = 0; m < 16; ++m) {
+        const int c = t + m * TPL - ox;
+        nxt[m] = (r < ny && c >= 0 && c < nx) ? in[(size_t)r * nx + c] : make_float2(0.f, 0.f);
+    }
+    for (; r < ny; r += gridDim.x) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m) buf[fft::lane_pad<L>(t, m)] = nxt[m];
+        __syncthreads();
+        const int rn = r + gridDim.x;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int c = t + m * TPL - ox;
+            nxt[m] = (rn < ny && c >= 0 && c < nx) ? in[(size_t)rn * nx + c] : make_float2(0.f, 0.f);
+        }
+        fft::line_tw<L, -1>(buf, tw, t);
+        float2* dst = work + (size_t)(oy + r) * L;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) dst[t + m * TPL] = buf[fft::lane_pad<L>(t, m)];
+        __syncthreads();
+    }
+}
+
+template <int L>
+__global__ __launch_bounds__(L / 16) void asm_rows_inv_p(const float2* __restrict__ work, int nx, int ny, int ox,
+                                                         int oy, float scale, float2* __restrict__ out,
+                                                         const float2* __restrict__ tw) {
+    constexpr int TPL = L / 16;
+    constexpr int PL = fft::padded_len(L);
+    __shared__ float2 buf[PL];
+    const int t = threadIdx.x;
+    float2 nxt[16];
+    int r = blockIdx.x;
+#pragma unroll
+    for (int m = 0; m < 16; ++m)
+        nxt[m] = r < ny ? work[(size_t)(oy + r) * L + t + m * TPL] : make_float2(0.f, 0.f);
+    for (; r < ny; r += gridDim.x) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m) buf[fft::lane_pad<L>(t, m)] = nxt[m];
+        __syncthreads();
+        const int rn = r + gridDim.x;
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            nxt[m] = rn < ny ? work[(size_t)(oy + rn) * L + t + m * TPL] : make_float2(0.f, 0.f);
+        fft::line_tw<L, +1>(buf, tw, t);
+        float2* dst = out + (size_t)r * nx;
+        for (int c = t; c < nx; c += TPL) {
+            const float2 v = buf[fft::pad16(c + ox)];
+            dst[c] = make_float2(v.x * scale, v.y * scale);
+        }
+        __syncthreads();
+    }
+}
+
+// resident workgroups per CU of the persistent row passes (register-bound
+// occupancy of each kernel, 4 waves per workgroup)
+constexpr int ROWS_FWD_PER_CU = 2, ROWS_INV_PER_CU = 3;
+
 template <int LX>
 int launch_rows_fwd(oph_ctx* ctx, const float2* in, int nx, int ny, int ox, int oy, float2* work) {
     const float2* tw = oph_twiddles(ctx, LX);
     if (!tw) return OPH_ENOMEM;
     constexpr int LPW = fftk::row_lines(LX);
+    if constexpr (LPW == 1) {
+        const int g = std::min(ny, ROWS_FWD_PER_CU * std::max(ctx->cu_count, 1));
+        OPH_LAUNCH(ctx, "asm_rows_fwd", asm_rows_fwd_p<LX>, dim3(g), dim3(LX / 16), 0, in, nx, ny, ox, oy, work, tw);
+        return OPH_OK;
+    }
     OPH_LAUNCH(ctx, "asm_rows_fwd", asm_rows_fwd<LX>, dim3((ny + LPW - 1) / LPW), dim3(LPW * (LX / 16)), 0, in,
                nx, ny, ox, oy, work, tw);
     return OPH_OK;
@@ -195,6 +274,12 @@ int launch_rows_inv(oph_ctx* ctx, const float2* work, int nx, int ny, int ox, in
     const float2* tw = oph_twiddles(ctx, LX);
     if (!tw) return OPH_ENOMEM;
     constexpr int LPW = fftk::row_lines(LX);
+    if constexpr (LPW == 1) {
+        const int g = std::min(ny, ROWS_INV_PER_CU * std::max(ctx->cu_count, 1));
+        OPH_LAUNCH(ctx, "asm_rows_inv", asm_rows_inv_p<LX>, dim3(g), dim3(LX / 16), 0, work, nx, ny, ox, oy, scale,
+                   out, tw);
+        return OPH_OK;
+    }
     OPH_LAUNCH(ctx, "asm_rows_inv", asm_rows_inv<LX>, dim3((ny + LPW - 1) / LPW), dim3(LPW * (LX / 16)), 0, work,
                nx, ny, ox, oy, scale, out, tw);
     return OPH_OK;
