@@ -23,8 +23,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "holograms/sec @4096² at rel-L2≤1e-3 vs FP64; point·pixel/s; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-VALU_LANE_OPS = 256 * 64 * 2.4e9          # unpacked FP32 lane-ops/s (3.93e13)
-RS_SLOTS_PER_PAIR = 21                     # issue-slot model of one exact RS pair
+VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9      # VALU issue peak, unpacked FP32 lane-ops/s (3.93e13)
+# Wave-level VALU instructions per launch at the bench sizes, from rocprofv3
+# SQ_INSTS_VALU (profiles/valu_counters_rs_wrp.md); valid for the kernels as
+# built here, re-measure after changing them.
+PROFILED_VALU_INSTS = {"rs_accumulate": 181585920, "asm_cols": 29720576, "wrp_gather": 38636949}
 
 
 def rel_l2(a, b):
@@ -61,21 +64,23 @@ def bench_rs(ctx, args):
     ctx.profile(False)
     pairs = 1000 * spec["nx"] * spec["ny"]
     k_avg = kms / max(n, 1) / 1e3
-    # accuracy of this very output against the FP64 oracle
-    from oracle import oracle as O
-    err = rel_l2(out.numpy(), O.rs(spec, x, y, z, a, ph))
-    peak_pairs = VALU_LANE_OPS / RS_SLOTS_PER_PAIR
+    lane_ops = PROFILED_VALU_INSTS["rs_accumulate"] * 64
     line = {
         "metric": METRIC, "value": args.steps / wall, "unit": "holograms/s",
         "ms_per_step": wall / args.steps * 1e3, "dtype": "f64-phase/f32-acc",
         "config": {"workload": "rs_pointcloud_1000pts_1024x1024_8um_532nm"},
-        "rel_l2_vs_fp64": err,
         "point_pixel_per_s": pairs / (wall / args.steps),
-        "roofline": {"bound": "valu", "achieved": pairs / k_avg / 1e12, "peak": peak_pairs / 1e12,
-                     "unit": "Tpair/s", "frac": pairs / k_avg / peak_pairs, "traffic": None},
+        # VALU issue: counted VALU lane-ops of rs_accumulate per launch / its
+        # measured duration, against the chip's VALU issue peak
+        "roofline": {"kernel": "rs_accumulate", "bound": "valu", "achieved": lane_ops / k_avg / 1e12,
+                     "peak": VALU_LANE_OPS / 1e12, "unit": "Tlane-op/s", "frac": lane_ops / k_avg / VALU_LANE_OPS,
+                     "traffic": None, "valu_lane_ops_per_pair": lane_ops / pairs},
     }
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_rs(spec)
+        line["cpu_baseline"], small, ref = cpu_rs(spec)
+        # the baseline's oracle field (the same points on a 1024 x 64 grid)
+        # doubles as the accuracy check of the kernel on that grid
+        line["rel_l2_vs_fp64_1024x64"] = rel_l2(ctx.pc_rs(small, x, y, z, a, ph), ref)
     return line
 
 
@@ -87,13 +92,13 @@ def cpu_rs(spec):
     s = dict(spec, ny=rows)
     cores = host_cores()
     t0 = time.perf_counter()
-    O.rs(s, x, y, z, a, ph, nthreads=cores)
+    ref = O.rs(s, x, y, z, a, ph, nthreads=cores)
     dt = time.perf_counter() - t0
     pairs = 1000 * spec["nx"] * rows
     full = 1000 * spec["nx"] * spec["ny"]
     return {"value": (pairs / dt) / full, "unit": "holograms/s", "cores": cores, "kind": "port",
             "sample": f"FP64 oracle, 1000 points x {rows} of {spec['ny']} rows "
-                      f"({pairs:.3g} point-pixel pairs, {dt:.2f} s), scaled by pair count"}
+                      f"({pairs:.3g} point-pixel pairs, {dt:.2f} s), scaled by pair count"}, s, ref
 
 
 def _time_steps(ctx, step, args, kernels):
@@ -170,6 +175,10 @@ def bench_wrp(ctx, args):
     line["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": PROFILED_TRAFFIC.get(dom),
                         "algorithmic_bytes": byt[dom]}
+    if dom in PROFILED_VALU_INSTS:
+        # the column pass is limited by its transform arithmetic: its VALU
+        # issue rate against the VALU peak, from counted instructions
+        line["roofline"]["valu_frac"] = PROFILED_VALU_INSTS[dom] * 64 / t / VALU_LANE_OPS
     pairs = 0
     # a plain device-to-device copy of the padded work-buffer size: the
     # achievable copy rate on this device (and the calibration kernel for the
@@ -187,12 +196,11 @@ def bench_wrp(ctx, args):
     del src, dst
     line["hbm_gbs_asm_total"] = sum(byt.values()) / (sum(per[k]["avg_us"] for k in byt if k in per) * 1e-6) / 1e9
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_wrp(x, y, z, a, zw, n_side, pitch, lam, dist, pad)
-    if args.check:
-        from oracle import oracle as O
-        w = O.wrp(n_side, n_side, pitch, lam, zw, x, y, z, a, nthreads=host_cores())
-        ref = O.asm(w, pitch, pitch, lam, dist, pad=pad, nthreads=host_cores())
-        line["rel_l2_vs_fp64"] = rel_l2(U, ref)
+        line["cpu_baseline"], m, ref = cpu_wrp(x, y, z, a, zw, n_side, pitch, lam, dist, pad)
+        # the baseline's oracle hologram (first m points) doubles as the
+        # accuracy check of the same hologram computed here
+        sub = ctx.wrp_hologram(n_side, pitch, lam, zw, dist, x[:m], y[:m], z[:m], a[:m], pad=pad)
+        line["rel_l2_vs_fp64_point_subset"] = rel_l2(sub, ref)
     del pairs
     return line
 
@@ -207,11 +215,11 @@ def cpu_wrp(x, y, z, a, zw, n_side, pitch, lam, dist, pad):
     w = O.wrp(n_side, n_side, pitch, lam, zw, x[:m], y[:m], z[:m], a[:m], nthreads=cores)
     t_wrp = (time.perf_counter() - t0) * (len(x) / m)
     t0 = time.perf_counter()
-    O.asm(w, pitch, pitch, lam, dist, pad=pad, nthreads=cores)
+    ref = O.asm(w, pitch, pitch, lam, dist, pad=pad, nthreads=cores)
     t_asm = time.perf_counter() - t0
     return {"value": 1.0 / (t_wrp + t_asm), "unit": "holograms/s", "cores": cores, "kind": "port",
             "sample": f"FP64 oracle: WRP of {m} of {len(x)} points ({t_wrp * m / len(x):.2f} s, scaled "
-                      f"by point count) + one full {n_side * pad}^2 padded ASM ({t_asm:.2f} s)"}
+                      f"by point count) + one full {n_side * pad}^2 padded ASM ({t_asm:.2f} s)"}, m, ref
 
 
 DM_KERNELS = ["dm_index", "dm_zero", "dm_cols", "dm_rows", "dm_final"]
@@ -249,9 +257,12 @@ def bench_depthmap(ctx, args):
         from oracle import oracle as O
         l0 = int(np.argmax(np.bincount(idx.ravel(), minlength=257)))
         t0 = time.perf_counter()
-        O.depthmap(img[1], dep, Nx, Ny, 8e-6, 8e-6, lams[1], 0.09, 0.11, L, layers=(l0, l0 + 3),
-                   nthreads=host_cores())
+        _, ref = O.depthmap(img[1], dep, Nx, Ny, 8e-6, 8e-6, lams[1], 0.09, 0.11, L, layers=(l0, l0 + 3),
+                            nthreads=host_cores())
         dt = time.perf_counter() - t0
+        # the baseline's oracle layers double as the accuracy check
+        sub = ctx.depthmap(w, h, Nx, Ny, 8e-6, 0.09, 0.11, L, [lams[1]], img[1:2], dep, layers=(l0, l0 + 3))[0]
+        line["rel_l2_vs_fp64_4_layers"] = rel_l2(sub, ref)
         line["cpu_baseline"] = {"value": 1.0 / (dt * 3 * layers / 4), "unit": "holograms/s (RGB)",
                                 "cores": host_cores(), "kind": "port",
                                 "sample": f"FP64 oracle, 4 populated layers of one colour at full size "
@@ -270,7 +281,8 @@ def bench_fresnel(ctx, args):
     d = [ctx.to_device(v) for v in (x, y, z, a, ph)]
     out = ctx.empty((n, n), np.complex64)
     ref = None
-    if not args.no_cpu or args.check:
+    if not args.no_cpu:
+        # the CPU baseline's oracle pixels double as the accuracy check
         from oracle import oracle as O
         rng = np.random.default_rng(0)
         rows, cols = rng.integers(0, n, (2, 256))
@@ -304,11 +316,14 @@ def bench_fresnel(ctx, args):
         res["valu"]["rel_l2_vs_fp64_sampled"] = rel_l2(out.numpy()[rows, cols], ref)
     for m in methods:
         res[m]["speedup_vs_valu"] = res[m]["holograms_per_s"] / res["valu"]["holograms_per_s"]
-    ok = [m for m in methods if res[m].get("rel_l2_vs_fp64_sampled", 0.0) <= 1e-3]
+    # the fastest method within rel-L2 1e-3 of FP64 (FP16 when the accuracy
+    # check did not run: its error, 2.6e-4 at this workload, is pinned by the
+    # GPU tests)
+    ok = [m for m in methods if res[m].get("rel_l2_vs_fp64_sampled", 0.0 if m == "fp16" else 1.0) <= 1e-3]
     best = max(ok, key=lambda m: res[m]["holograms_per_s"]) if ok else "fp16_split"
     line = {"value": res[best]["holograms_per_s"], "unit": "holograms/s", "method": best,
             "config": {"workload": "fresnel_1e6_points_64_planes_4096sq_8um_532nm"}, "methods": res}
-    if ref is not None and not args.no_cpu:
+    if ref is not None:
         pairs = 256 * len(x)
         line["cpu_baseline"] = {"value": pairs / dt / (n * n * len(x)), "unit": "holograms/s",
                                 "cores": host_cores(), "kind": "port",
@@ -416,9 +431,12 @@ def bench_learned(ctx, args):
         layers = int((counts > 0).sum())
         l0 = int(np.argmax(counts)) + 1
         t0 = time.perf_counter()
-        O.depthmap(img[0], dep, N, N, learned.PITCH, learned.PITCH, learned.LAMBDA, learned.NEAR, learned.FAR, 256,
-                   layers=(l0, l0 + 3), nthreads=host_cores())
+        _, ref = O.depthmap(img[0], dep, N, N, learned.PITCH, learned.PITCH, learned.LAMBDA, learned.NEAR,
+                            learned.FAR, 256, layers=(l0, l0 + 3), nthreads=host_cores())
         dt = time.perf_counter() - t0
+        sub = ctx.depthmap(n, n, N, N, learned.PITCH, learned.NEAR, learned.FAR, 256, lam, img, dep,
+                           layers=(l0, l0 + 3))[0]
+        line["rel_l2_vs_fp64_4_layers"] = rel_l2(sub, ref)
         line["cpu_baseline"] = {"value": 1.0 / (dt * layers / 4), "unit": "holograms/s (256-layer path)",
                                 "cores": host_cores(), "kind": "port",
                                 "sample": f"FP64 oracle, 4 populated layers of the 1024^2 scene on the 2048^2 grid "
@@ -438,7 +456,6 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--check", action="store_true", help="also report rel-L2 of the output vs the FP64 oracle")
     ap.add_argument("--only", action="store_true", help="run the selected workload only (no others)")
     ap.add_argument("--learn-train", type=int, default=64, help="learned: training scenes (1024^2 pairs)")
     ap.add_argument("--learn-steps", type=int, default=300, help="learned: Adam steps (batch 8, 128^2 patches)")
