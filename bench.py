@@ -346,15 +346,22 @@ def bench_learned(ctx, args):
     from openhologram_amd import learned
     from openhologram_amd.lib import Cnn
     n, N = 1024, 2048
+    log = lambda m: print(f"# learned: {m}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    xs, ts = learned.make_pairs(ctx, range(1000, 1000 + args.learn_train), n)
+    xs = np.empty((args.learn_train, n, n), np.complex64)
+    ts = np.empty((args.learn_train, n, n), np.complex64)
+    for b0 in range(0, args.learn_train, 256):   # in blocks, with a progress line each
+        b1 = min(args.learn_train, b0 + 256)
+        xs[b0:b1], ts[b0:b1] = learned.make_pairs(ctx, range(1000 + b0, 1000 + b1), n)
+        log(f"training pairs {b1}/{args.learn_train} ({time.perf_counter() - t0:.1f} s)")
     held = range(9000, 9000 + args.learn_eval)
     t_pairs = time.perf_counter() - t0
     scale = float(np.sqrt(np.mean(np.abs(xs) ** 2)))
     net = Cnn(ctx, blocks=8, channels=32, seed=1)
     ctx.profile(True)
     t0 = time.perf_counter()
-    hist = learned.train(ctx, net, xs, ts, scale, steps=args.learn_steps, batch=8, patch=128, lr=1e-3)
+    hist = learned.train(ctx, net, xs, ts, scale, steps=args.learn_steps, batch=8, patch=128, lr=1e-3,
+                         log=lambda m: log("train " + m))
     ctx.sync()
     t_train = time.perf_counter() - t0
     train_k = {}
@@ -367,16 +374,16 @@ def bench_learned(ctx, args):
     # held-out evaluation
     depths = learned.sampled_depths()
     ev = []
-    scenes_ = []
-    for s in held:
-        img, dep = learned.scenes.depth_scene(n, n, seed=s, colors=1)
-        scenes_.append((img, dep))
+    scenes_ = learned.make_scenes(held, n)
+    log(f"{len(scenes_)} held-out scenes generated")
+    for img, dep in scenes_:
         c = learned.field(ctx, img, dep, n, 16)
         f = learned.field(ctx, img, dep, n, 256)
         u = learned.correct(ctx, net, c, scale)
         ev.append((rel_l2(c, f), rel_l2(u, f), learned.recon_psnr(ctx, c, f, depths),
                    learned.recon_psnr(ctx, u, f, depths)))
     ev = np.array(ev)
+    log("held-out evaluation done")
     # timed, device resident: 256-layer field; 16-layer field + CNN
     img, dep = scenes_[0]
     dimg, ddep = ctx.to_device(img), ctx.to_device(dep)
@@ -457,9 +464,9 @@ def main():
     ap.add_argument("--workload", default=DEFAULT, choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", action="store_true", help="run the selected workload only (no others)")
-    ap.add_argument("--learn-train", type=int, default=64, help="learned: training scenes (1024^2 pairs)")
-    ap.add_argument("--learn-steps", type=int, default=300, help="learned: Adam steps (batch 8, 128^2 patches)")
-    ap.add_argument("--learn-eval", type=int, default=8, help="learned: held-out scenes")
+    ap.add_argument("--learn-train", type=int, default=2000, help="learned: training scenes (1024^2 pairs)")
+    ap.add_argument("--learn-steps", type=int, default=3000, help="learned: Adam steps (batch 8, 128^2 patches)")
+    ap.add_argument("--learn-eval", type=int, default=200, help="learned: held-out scenes")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): the hologram "

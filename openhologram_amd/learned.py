@@ -12,6 +12,8 @@ Everything here drives liboph_hip.so through the ctypes layer; the
 depth-map fields come from oph_depthmap (OPH_DM_OUT_FIELD), the
 reconstructions from oph_asm_propagate.
 """
+import os
+
 import numpy as np
 
 from . import scenes
@@ -32,14 +34,31 @@ def field(ctx, img, dep, n, layers, pad=2):
                         quant_mode=1)[0]
 
 
-def make_pairs(ctx, seeds, n, coarse_layers=16):
+def _scene(args):
+    n, s = args
+    return scenes.depth_scene(n, n, seed=int(s), colors=1)
+
+
+def make_scenes(seeds, n, workers=None):
+    """Seeded one-colour depth-map scenes, generated on host worker processes."""
+    seeds = [int(s) for s in seeds]
+    workers = workers or min(16, os.cpu_count() or 1)
+    if workers <= 1 or len(seeds) < 8:
+        return [_scene((n, s)) for s in seeds]
+    import multiprocessing as mp
+    with mp.get_context("fork").Pool(workers) as pool:
+        return pool.map(_scene, [(n, s) for s in seeds], chunksize=4)
+
+
+def make_pairs(ctx, seeds, n, coarse_layers=16, workers=None):
     """(coarse, full) complex fields for the seeded scenes."""
-    xs, ts = [], []
-    for s in seeds:
-        img, dep = scenes.depth_scene(n, n, seed=int(s), colors=1)
-        xs.append(field(ctx, img, dep, n, coarse_layers))
-        ts.append(field(ctx, img, dep, n, 256))
-    return np.stack(xs), np.stack(ts)
+    sc = make_scenes(seeds, n, workers)
+    xs = np.empty((len(sc), n, n), np.complex64)
+    ts = np.empty((len(sc), n, n), np.complex64)
+    for i, (img, dep) in enumerate(sc):
+        xs[i] = field(ctx, img, dep, n, coarse_layers)
+        ts[i] = field(ctx, img, dep, n, 256)
+    return xs, ts
 
 
 def to_channels(u, scale):
@@ -75,18 +94,17 @@ def recon_psnr(ctx, u, ref, depths):
 
 
 def train(ctx, net, xs, ts, scale, steps=300, batch=8, patch=128, lr=2e-3, seed=0, log=None):
-    """Adam on random patches of the training pairs; returns the loss history."""
+    """Adam on random patches of the training pairs (complex fields, cut and
+    converted per batch); returns the loss history."""
     rng = np.random.default_rng(seed)
-    X = to_channels(xs, scale)
-    T = to_channels(ts, scale)
-    n, H, W, _ = X.shape
+    n, H, W = xs.shape
     hist = []
     for it in range(steps):
         idx = rng.integers(0, n, batch)
         ys = rng.integers(0, H - patch + 1, batch)
         xs_ = rng.integers(0, W - patch + 1, batch)
-        xb = np.stack([X[i, y:y + patch, x:x + patch] for i, y, x in zip(idx, ys, xs_)])
-        tb = np.stack([T[i, y:y + patch, x:x + patch] for i, y, x in zip(idx, ys, xs_)])
+        xb = to_channels(np.stack([xs[i, y:y + patch, x:x + patch] for i, y, x in zip(idx, ys, xs_)]), scale)
+        tb = to_channels(np.stack([ts[i, y:y + patch, x:x + patch] for i, y, x in zip(idx, ys, xs_)]), scale)
         rate = lr * 0.5 * (1 + np.cos(np.pi * it / steps))   # cosine decay
         hist.append(net.train_step(xb, tb, rate))
         if log and (it % max(1, steps // 10) == 0 or it == steps - 1):
