@@ -4,9 +4,9 @@ A cheap hologram is the depth-map field computed with few depth layers (the
 depth map requantised to, e.g., 16 levels); the full one uses all 256.  A
 small residual CNN (oph_cnn_*) maps the first toward the second.  It is
 trained only on pairs this method generates; the full-layer method stays the
-judge: the reconstruction PSNR of the corrected hologram at sampled depths is
-checked against the full-layer hologram's, and below the threshold the frame
-falls back to the full-layer result.
+reference: the reconstruction PSNR of the corrected hologram at sampled
+depths is checked against the full-layer hologram's, and below the threshold
+the frame falls back to the full-layer result.
 
 Everything here drives liboph_hip.so through the ctypes layer; the
 depth-map fields come from oph_depthmap (OPH_DM_OUT_FIELD), the

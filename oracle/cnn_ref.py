@@ -1,5 +1,5 @@
 """FP64 numpy restatement of the residual CNN of the learned correction
-(forward, backward, loss) -- TEST INFRASTRUCTURE ONLY: the checker of the HIP
+(forward, backward, loss) -- TEST INFRASTRUCTURE ONLY: the reference of the HIP
 kernels' gradients.  The network has no counterpart in the reference.
 
 Parameter vector layout (as the C ABI): head W[C][9][2], b[C]; per block

@@ -2,8 +2,8 @@
  * oph_oracle.c -- CPU restatement of OpenHolo's hologram-generation algorithms.
  *
  * TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
- * calls this file: it is the checker that tests/, __graft_entry__.smoke() and
- * bench.py's cpu_baseline leg compare the HIP path against.
+ * calls this file: it is the CPU restatement that tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg compare the HIP path against.
  *
  * Every routine follows a reference file:line (paths relative to the reference
  * tree; K/ = OpenHolo_KETI/VS/Openholo_Generation_PointCloud_Library_beta_v1.0,

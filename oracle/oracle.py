@@ -2,7 +2,7 @@
 reference KETI sources (oracle/_ref/libketi_ref.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
-bench.py's cpu_baseline leg, as the checker.  The product package never imports
+bench.py's cpu_baseline leg, as the CPU restatement the GPU path is compared with.  The product package never imports
 this module.
 """
 import ctypes as C
