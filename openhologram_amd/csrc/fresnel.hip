@@ -162,8 +162,12 @@ __global__ void fresnel_sum_chunks(const float2* partial, int chunks, int64_t np
 // output row 8(v/4) + 4(l/32) + v%4, column l%32).
 
 constexpr int MT = 128;         // output tile (rows, columns)
-constexpr int KC = 32;          // points per chunk
-constexpr int KS = KC + 8;      // LDS row stride (elements), 16-byte aligned rows
+constexpr int KC = 32;          // point padding granularity (largest chunk)
+// points per chunk: 32 for the single-precision forms; 16 for the hi+lo
+// forms, whose four operand planes would otherwise need 83 KB of LDS and
+// leave one workgroup per CU (no overlap of generation and MFMA phases)
+template <bool SPLIT>
+__host__ __device__ constexpr int chunk_points() { return SPLIT ? 16 : 32; }
 
 template <typename T>
 struct Mfma;
@@ -207,10 +211,12 @@ template <typename T, bool SPLIT>
 __global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
     typedef typename Mfma<T>::V V;
     constexpr int NP = SPLIT ? 4 : 2;
+    constexpr int KCH = chunk_points<SPLIT>();
+    constexpr int KS = KCH + 8;   // LDS row stride (elements), 16-byte aligned rows
     __shared__ __attribute__((aligned(16))) T As[NP][MT][KS];
     __shared__ __attribute__((aligned(16))) T Bs[NP][MT][KS];
-    __shared__ double p_x[KC], p_y[KC], p_c[KC];
-    __shared__ float p_wr[KC], p_wi[KC];
+    __shared__ double p_x[KCH], p_y[KCH], p_c[KCH];
+    __shared__ float p_wr[KCH], p_wi[KCH];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -234,9 +240,9 @@ __global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
     const double Xc = (j0 + gr - a.nx / 2) * a.p;
     const bool row_ok = i0 + gr < a.ny, col_ok = j0 + gr < a.nx;
 
-    for (int64_t k0 = 0; k0 < a.npad; k0 += KC) {
+    for (int64_t k0 = 0; k0 < a.npad; k0 += KCH) {
         __syncthreads();
-        if (tid < KC) {
+        if (tid < KCH) {
             const FrPoint q = a.pts[k0 + tid];
             p_x[tid] = q.x;
             p_y[tid] = q.y;
@@ -245,10 +251,10 @@ __global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
             p_wi[tid] = q.wim;
         }
         __syncthreads();
-        // 16 points of this thread's row/column, written as 8-element vectors
+        // KCH/2 points of this thread's row/column, written as 8-element vectors
 #pragma unroll
-        for (int hb = 0; hb < 2; ++hb) {
-            const int kb0 = gh * 16 + hb * 8;
+        for (int hb = 0; hb < KCH / 16; ++hb) {
+            const int kb0 = gh * (KCH / 2) + hb * 8;
             V va[NP], vb[NP];
 #pragma unroll
             for (int e = 0; e < 8; ++e) {
@@ -289,7 +295,7 @@ __global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
         }
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < KC / 16; ++s) {
+        for (int s = 0; s < KCH / 16; ++s) {
             const int kb = 16 * s + 8 * (lane >> 5);
             V A[2][NP], B[2][NP];
 #pragma unroll
