@@ -36,16 +36,25 @@
 namespace {
 
 // per-axis table entry: x = frac(-z lambda f^2 / 2) (turns), y = lambda^2 f^2,
-// z = (|f| z lambda / (Np p / 2))^2 for the band limit, w = f (cycles/m)
-__global__ void asm_axis_table(float4* t, int n, double pitch, double lambda, double z) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+// z = (|f| z lambda / (Np p / 2))^2 for the band limit, w = f (cycles/m);
+// entries [0, nx) for the x axis, [nx, nx + ny) for the y axis
+__global__ void asm_axis_table(float4* t, int nx, double px, int ny, double py, double lambda, double z) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= nx + ny) return;
+    float4* dst = t + k;
+    int n = nx;
+    double pitch = px;
+    if (k >= nx) {
+        k -= nx;
+        n = ny;
+        pitch = py;
+    }
     const int kk = k < n / 2 ? k : k - n;
     const double f = kk / (n * pitch);
     double tt = -z * lambda * f * f * 0.5;
     tt -= rint(tt);
     const double bl = fabs(f) * z * lambda / (n * pitch * 0.5);
-    t[k] = make_float4((float)tt, (float)(lambda * lambda * f * f), (float)(bl * bl), (float)f);
+    *dst = make_float4((float)tt, (float)(lambda * lambda * f * f), (float)(bl * bl), (float)f);
 }
 
 struct AsmConst {
@@ -342,10 +351,8 @@ int oph_asm_device(oph_ctx* ctx, float2* field, int nx, int ny, int npx, int npy
     const int ox = npx / 2 - nx / 2, oy = npy / 2 - ny / 2;
     float4* tab = static_cast<float4*>(oph_scratch(ctx, SLOT_TABLE, sizeof(float4) * (npx + npy)));
     if (!tab) return OPH_ENOMEM;
-    OPH_LAUNCH(ctx, "asm_axis_table", asm_axis_table, dim3((npx + 255) / 256), dim3(256), 0, tab, npx, px,
-               lambda, z);
-    OPH_LAUNCH(ctx, "asm_axis_table", asm_axis_table, dim3((npy + 255) / 256), dim3(256), 0, tab + npx, npy, py,
-               lambda, z);
+    OPH_LAUNCH(ctx, "asm_axis_table", asm_axis_table, dim3((npx + npy + 255) / 256), dim3(256), 0, tab, npx, px,
+               npy, py, lambda, z);
     const double zl = z / lambda;
     AsmConst k;
     k.zl = zl;

@@ -13,7 +13,7 @@
 //
 // The reference scatters with a racing AddPixel; here the plane is gathered
 // per 64x64 tile, with no global atomics:
-//   1. wrp_bin_count   per chunk of 1024 points: the point records (centre,
+//   1. wrp_bin_count   per chunk of points: the point records (centre,
 //                      window, FP64-reduced phase constants) and per-tile
 //                      counts in LDS;
 //   2. wrp_bin_colscan per tile: prefix of the counts over chunks;
@@ -68,6 +68,7 @@ struct WrpArgs {
     WrpPoint* pts;
     unsigned* chunk_counts;  // [ntiles][nchunks]: counts, then exclusive prefix over chunks
     int nchunks;
+    int chunk;               // points per binning workgroup, a multiple of BIN_THREADS
     unsigned* totals;        // [ntiles]
     unsigned* offsets;       // [ntiles + 1]
     float* chunk_max;        // [nchunks] max |a/D|
@@ -77,8 +78,8 @@ struct WrpArgs {
     unsigned* overflow;
 };
 
-constexpr int CHUNK = 1024;          // points per binning workgroup
 constexpr int BIN_THREADS = 256;
+constexpr int MAX_CHUNKS = 1024;     // chunks per call (bounds the [tile][chunk] count matrix)
 constexpr int MAX_TILES = 16384;     // LDS counters of one binning workgroup
 
 __device__ __forceinline__ void tile_range(const WrpPoint& q, const WrpArgs& a, int& x0, int& x1, int& y0,
@@ -138,9 +139,9 @@ __global__ __launch_bounds__(BIN_THREADS) void wrp_bin_count(WrpArgs a) {
     __shared__ float wmax[BIN_THREADS / 64];
     for (int t = threadIdx.x; t < a.ntiles; t += BIN_THREADS) cnt[t] = 0u;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * CHUNK;
+    const int64_t base = (int64_t)blockIdx.x * a.chunk;
     float mx = 0.f;
-    for (int k = threadIdx.x; k < CHUNK; k += BIN_THREADS) {
+    for (int k = threadIdx.x; k < a.chunk; k += BIN_THREADS) {
         const int64_t i = base + k;
         if (i >= a.n) break;
         const WrpPoint q = make_point(a, i);
@@ -224,9 +225,9 @@ __global__ __launch_bounds__(BIN_THREADS) void wrp_bin_fill(WrpArgs a) {
     __shared__ unsigned cur[MAX_TILES];
     for (int t = threadIdx.x; t < a.ntiles; t += BIN_THREADS) cur[t] = 0u;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * CHUNK;
+    const int64_t base = (int64_t)blockIdx.x * a.chunk;
 
-    for (int k = threadIdx.x; k < CHUNK; k += BIN_THREADS) {
+    for (int k = threadIdx.x; k < a.chunk; k += BIN_THREADS) {
         const int64_t i = base + k;
         if (i >= a.n) break;
         const WrpPoint q = a.pts[i];
@@ -332,16 +333,22 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
             int ax0, ax1, ay0, ay1;
             abs_range(wx0, wx1, ax0, ax1);
             abs_range(wy0, wy1, ay0, ay1);
-            const int na = ax1 - ax0 + 1;
-            const int cnt = na * (ay1 - ay0 + 1);
+            const int na = ax1 - ax0 + 1, nay = ay1 - ay0 + 1;
+            // lane positions qq = lane + 64 j walk the na x nay grid of
+            // (|wx|, |wy|) row by row; advancing 64 positions is sy rows plus
+            // sx columns, so no division or multiply per position
             const float inv_na = 1.f / (float)na;
-            const int pbase = (ty - Y0) * TILE + (tx - X0);
-            for (int qq = lane; qq < cnt; qq += 64) {
-                const int iy = (int)(((float)qq + 0.5f) * inv_na);
-                const int ax = ax0 + (qq - iy * na), ay = ay0 + iy;
+            const int sy = (int)(64.5f * inv_na), sx = 64 - sy * na;
+            int iy = (int)(((float)lane + 0.5f) * inv_na);
+            int ix = lane - iy * na;
+            // byte address of the window centre in the accumulators
+            const int cbyte = ((ty - Y0) * TILE + (tx - X0)) * 4;
+            while (iy < nay) {
+                const int ax = ax0 + ix, ay = ay0 + iy;
                 float re, im;
                 if (!q.exact) {
-                    const float u = (float)(ax * ax + ay * ay) * q.p2dz2;
+                    const float fx = (float)ax, fy = (float)ay;
+                    const float u = fmaf(fx, fx, fy * fy) * q.p2dz2;
                     const float p = u * fmaf(u, fmaf(u, fmaf(u, -5.f / 128.f, 1.f / 16.f), -0.125f), 0.5f);
                     const float ph = fmaf(q.ks, p, q.f0);
                     const float am = q.ampd * fmaf(q.cd, p, 1.f);
@@ -362,24 +369,32 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                 }
                 const int ire = (int)rintf(re * scale);
                 const int iim = (int)rintf(im * scale);
-                const bool xp = ax < wx1, xm = ax > 0 && -ax >= wx0;
-                const bool yp = ay < wy1, ym = ay > 0 && -ay >= wy0;
-                const bool xpv = xp && ax >= wx0, ypv = yp && ay >= wy0;
-                if (xpv && ypv) {
-                    atomicAdd(&acc_re[pbase + ay * TILE + ax], ire);
-                    atomicAdd(&acc_im[pbase + ay * TILE + ax], iim);
+                const bool xp = ax >= wx0 && ax < wx1, xm = ax > 0 && -ax >= wx0;
+                const bool yp = ay >= wy0 && ay < wy1, ym = ay > 0 && -ay >= wy0;
+                const int rp = cbyte + (ay << 8), rm = cbyte - (ay << 8), cx = ax << 2;
+                char* bre = reinterpret_cast<char*>(acc_re);
+                char* bim = reinterpret_cast<char*>(acc_im);
+                if (xp && yp) {
+                    atomicAdd(reinterpret_cast<int*>(bre + rp + cx), ire);
+                    atomicAdd(reinterpret_cast<int*>(bim + rp + cx), iim);
                 }
-                if (xm && ypv) {
-                    atomicAdd(&acc_re[pbase + ay * TILE - ax], ire);
-                    atomicAdd(&acc_im[pbase + ay * TILE - ax], iim);
+                if (xm && yp) {
+                    atomicAdd(reinterpret_cast<int*>(bre + rp - cx), ire);
+                    atomicAdd(reinterpret_cast<int*>(bim + rp - cx), iim);
                 }
-                if (xpv && ym) {
-                    atomicAdd(&acc_re[pbase - ay * TILE + ax], ire);
-                    atomicAdd(&acc_im[pbase - ay * TILE + ax], iim);
+                if (xp && ym) {
+                    atomicAdd(reinterpret_cast<int*>(bre + rm + cx), ire);
+                    atomicAdd(reinterpret_cast<int*>(bim + rm + cx), iim);
                 }
                 if (xm && ym) {
-                    atomicAdd(&acc_re[pbase - ay * TILE - ax], ire);
-                    atomicAdd(&acc_im[pbase - ay * TILE - ax], iim);
+                    atomicAdd(reinterpret_cast<int*>(bre + rm - cx), ire);
+                    atomicAdd(reinterpret_cast<int*>(bim + rm - cx), iim);
+                }
+                ix += sx;
+                iy += sy;
+                if (ix >= na) {
+                    ix -= na;
+                    ++iy;
                 }
             }
         }
@@ -427,7 +442,11 @@ int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, do
     a.mode = mode;
     const int ntiles = a.ntiles;
     const int64_t np = n > 0 ? n : 1;
-    const int nchunks = (int)((np + CHUNK - 1) / CHUNK);
+    // one point per thread while that keeps the chunk count bounded
+    int64_t chunk = (np + MAX_CHUNKS - 1) / MAX_CHUNKS;
+    chunk = (chunk + BIN_THREADS - 1) / BIN_THREADS * BIN_THREADS;
+    a.chunk = (int)chunk;
+    const int nchunks = (int)((np + chunk - 1) / chunk);
     a.nchunks = nchunks;
     a.pts = static_cast<WrpPoint*>(oph_scratch(ctx, SLOT_AUX0, sizeof(WrpPoint) * np));
     // bookkeeping: chunk counts, totals, offsets (ntiles + 1), overflow, chunk
