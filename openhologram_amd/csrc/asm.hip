@@ -133,12 +133,25 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
     const int cbase = g * B;
     fft::Twiddles<L> T;
     T.load(tw, threadIdx.x % TPL);
-    for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
-        const int b = idx % B, i = idx / B;
-        const int c = cbase + b;
-        const bool in_rows = i >= oy && i < oy + ny;
-        lds[b * PL + fft::pad16(i)] =
-            (in_rows && c < npx) ? work[(size_t)i * npx + c] : make_float2(0.f, 0.f);
+    if constexpr (B % 2 == 0) {
+        // two neighbouring columns per lane (16-byte accesses); npx is a
+        // power of two >= B, so every column of the group exists
+        for (int idx = threadIdx.x; idx < L * B / 2; idx += nthr) {
+            const int pr = idx % (B / 2), i = idx / (B / 2);
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (i >= oy && i < oy + ny)
+                v = *reinterpret_cast<const float4*>(work + (size_t)i * npx + cbase + 2 * pr);
+            lds[(2 * pr) * PL + fft::pad16(i)] = make_float2(v.x, v.y);
+            lds[(2 * pr + 1) * PL + fft::pad16(i)] = make_float2(v.z, v.w);
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
+            const int b = idx % B, i = idx / B;
+            const int c = cbase + b;
+            const bool in_rows = i >= oy && i < oy + ny;
+            lds[b * PL + fft::pad16(i)] =
+                (in_rows && c < npx) ? work[(size_t)i * npx + c] : make_float2(0.f, 0.f);
+        }
     }
     __syncthreads();
     const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
@@ -155,10 +168,18 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
     }
     __syncthreads();
     fft::line<L, +1>(line, T, t);
-    for (int idx = threadIdx.x; idx < ny * B; idx += nthr) {
-        const int bb = idx % B, i = oy + idx / B;
-        const int cc = cbase + bb;
-        if (cc < npx) work[(size_t)i * npx + cc] = lds[bb * PL + fft::pad16(i)];
+    if constexpr (B % 2 == 0) {
+        for (int idx = threadIdx.x; idx < ny * B / 2; idx += nthr) {
+            const int pr = idx % (B / 2), i = oy + idx / (B / 2);
+            const float2 u = lds[(2 * pr) * PL + fft::pad16(i)], v = lds[(2 * pr + 1) * PL + fft::pad16(i)];
+            *reinterpret_cast<float4*>(work + (size_t)i * npx + cbase + 2 * pr) = make_float4(u.x, u.y, v.x, v.y);
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < ny * B; idx += nthr) {
+            const int bb = idx % B, i = oy + idx / B;
+            const int cc = cbase + bb;
+            if (cc < npx) work[(size_t)i * npx + cc] = lds[bb * PL + fft::pad16(i)];
+        }
     }
 }
 
