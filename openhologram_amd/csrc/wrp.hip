@@ -42,7 +42,9 @@
 namespace {
 
 constexpr int TILE = 64;
-constexpr int GATHER_THREADS = 256;
+// threads per gather workgroup: 8 waves per tile (4 tiles fit per CU by LDS,
+// so 8 waves per SIMD hide the LDS-atomic and transcendental latencies)
+constexpr int GATHER_THREADS = 512;
 
 struct WrpPoint {
     int tx, ty, w, exact;
@@ -285,15 +287,17 @@ constexpr int REC_BATCH = 128;
 // once and added to the up-to-four mirrored pixels (+-ax, +-ay) that lie in
 // this tile's part of the window.  Sums are 32-bit fixed point with a scale
 // chosen per tile so that (list length) x (largest contribution) fits in
-// 2^30: exact integer adds, independent of the list order.
+// 2^30: exact integer adds, independent of the list order.  Both parts of a
+// pixel share one 64-bit word, 2^32 im + re, added with one 64-bit LDS
+// atomic: the word is exact modulo 2^64, and with |sum re|, |sum im| < 2^31
+// re is its sign-extended low half and im = (word - re) >> 32.
 __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
-    __shared__ int acc_re[TILE * TILE];
-    __shared__ int acc_im[TILE * TILE];
+    __shared__ unsigned long long acc[TILE * TILE];
     __shared__ GRec recs[REC_BATCH];
     const int tile = blockIdx.x;
     const int tyi = tile / g.ntx, txi = tile % g.ntx;
     const int X0 = txi * TILE, Y0 = tyi * TILE;
-    for (int i = threadIdx.x; i < TILE * TILE; i += GATHER_THREADS) acc_re[i] = acc_im[i] = 0;
+    for (int i = threadIdx.x; i < TILE * TILE; i += GATHER_THREADS) acc[i] = 0ull;
     unsigned b = g.offsets[tile], e = g.offsets[tile + 1];
     if (e > g.capacity) e = g.capacity;
     if (b > e) b = e;
@@ -341,8 +345,8 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
             const int sy = (int)(64.5f * inv_na), sx = 64 - sy * na;
             int iy = (int)(((float)lane + 0.5f) * inv_na);
             int ix = lane - iy * na;
-            // byte address of the window centre in the accumulators
-            const int cbyte = ((ty - Y0) * TILE + (tx - X0)) * 4;
+            // byte address of the window centre in the accumulator
+            const int cbyte = ((ty - Y0) * TILE + (tx - X0)) * 8;
             while (iy < nay) {
                 const int ax = ax0 + ix, ay = ay0 + iy;
                 float re, im;
@@ -369,27 +373,18 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                 }
                 const int ire = (int)rintf(re * scale);
                 const int iim = (int)rintf(im * scale);
+                // 2^32 im + re modulo 2^64
+                const unsigned long long pk =
+                    (unsigned long long)(long long)ire + ((unsigned long long)(long long)iim << 32);
                 const bool xp = ax >= wx0 && ax < wx1, xm = ax > 0 && -ax >= wx0;
                 const bool yp = ay >= wy0 && ay < wy1, ym = ay > 0 && -ay >= wy0;
-                const int rp = cbyte + (ay << 8), rm = cbyte - (ay << 8), cx = ax << 2;
-                char* bre = reinterpret_cast<char*>(acc_re);
-                char* bim = reinterpret_cast<char*>(acc_im);
-                if (xp && yp) {
-                    atomicAdd(reinterpret_cast<int*>(bre + rp + cx), ire);
-                    atomicAdd(reinterpret_cast<int*>(bim + rp + cx), iim);
-                }
-                if (xm && yp) {
-                    atomicAdd(reinterpret_cast<int*>(bre + rp - cx), ire);
-                    atomicAdd(reinterpret_cast<int*>(bim + rp - cx), iim);
-                }
-                if (xp && ym) {
-                    atomicAdd(reinterpret_cast<int*>(bre + rm + cx), ire);
-                    atomicAdd(reinterpret_cast<int*>(bim + rm + cx), iim);
-                }
-                if (xm && ym) {
-                    atomicAdd(reinterpret_cast<int*>(bre + rm - cx), ire);
-                    atomicAdd(reinterpret_cast<int*>(bim + rm - cx), iim);
-                }
+                // byte offsets of the mirrored pixels (8 bytes per pixel)
+                const int rp = cbyte + (ay << 9), rm = cbyte - (ay << 9), cx = ax << 3;
+                char* ba = reinterpret_cast<char*>(acc);
+                if (xp && yp) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rp + cx), pk);
+                if (xm && yp) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rp - cx), pk);
+                if (xp && ym) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rm + cx), pk);
+                if (xm && ym) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rm - cx), pk);
                 ix += sx;
                 iy += sy;
                 if (ix >= na) {
@@ -404,8 +399,11 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
     for (int p = threadIdx.x; p < TILE * TILE; p += GATHER_THREADS) {
         const int row = Y0 + p / TILE, col = X0 + p % TILE;
         if (row >= g.ny || col >= g.nx) continue;
-        const float re = (float)acc_re[p] * inv;
-        const float im = (float)acc_im[p] * inv;
+        const unsigned long long wv = acc[p];
+        const int sre = (int)(unsigned)wv;
+        const int sim = (int)((long long)(wv - (unsigned long long)(long long)sre) >> 32);
+        const float re = (float)sre * inv;
+        const float im = (float)sim * inv;
         float2* o = g.wrp + (size_t)row * g.nx + col;
         if (g.accumulate) {
             const float2 v = *o;
