@@ -37,24 +37,31 @@ namespace {
 
 // per-axis table entry: x = frac(-z lambda f^2 / 2) (turns), y = lambda^2 f^2,
 // z = (|f| z lambda / (Np p / 2))^2 for the band limit, w = f (cycles/m);
-// entries [0, nx) for the x axis, [nx, nx + ny) for the y axis
-__global__ void asm_axis_table(float4* t, int nx, double px, int ny, double py, double lambda, double z) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= nx + ny) return;
-    float4* dst = t + k;
-    int n = nx;
-    double pitch = px;
-    if (k >= nx) {
-        k -= nx;
-        n = ny;
-        pitch = py;
+// entries [0, nx) for the x axis, [nx, nx + ny) for the y axis.  The forward
+// row pass fills them (the column pass is the first reader), spread over its
+// threads, so they cost no launch of their own.
+struct TabJob {
+    float4* t;
+    int nx, ny;
+    double px, py, lambda, z;
+};
+
+__device__ void fill_axis_tables(const TabJob& j, int first, int step) {
+    for (int k = first; k < j.nx + j.ny; k += step) {
+        int kk = k, n = j.nx;
+        double pitch = j.px;
+        if (k >= j.nx) {
+            kk = k - j.nx;
+            n = j.ny;
+            pitch = j.py;
+        }
+        const int ks = kk < n / 2 ? kk : kk - n;
+        const double f = ks / (n * pitch);
+        double tt = -j.z * j.lambda * f * f * 0.5;
+        tt -= rint(tt);
+        const double bl = fabs(f) * j.z * j.lambda / (n * pitch * 0.5);
+        j.t[k] = make_float4((float)tt, (float)(j.lambda * j.lambda * f * f), (float)(bl * bl), (float)f);
     }
-    const int kk = k < n / 2 ? k : k - n;
-    const double f = kk / (n * pitch);
-    double tt = -z * lambda * f * f * 0.5;
-    tt -= rint(tt);
-    const double bl = fabs(f) * z * lambda / (n * pitch * 0.5);
-    *dst = make_float4((float)tt, (float)(lambda * lambda * f * f), (float)(bl * bl), (float)f);
 }
 
 struct AsmConst {
@@ -94,7 +101,8 @@ __device__ __forceinline__ float2 transfer(const float4 ax, const float4 ay, int
 template <int L>
 __global__ __launch_bounds__(512) void asm_rows_fwd(const float2* __restrict__ in, int nx, int ny, int ox,
                                                     int oy, float2* __restrict__ work,
-                                                    const float2* __restrict__ tw) {
+                                                    const float2* __restrict__ tw, TabJob tj) {
+    fill_axis_tables(tj, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
     constexpr int TPL = L / 16;
     constexpr int LPW = fftk::row_lines(L);
     constexpr int PL = fft::padded_len(L);
@@ -219,7 +227,8 @@ __global__ __launch_bounds__(512) void asm_rows_inv(const float2* __restrict__ w
 template <int L>
 __global__ __launch_bounds__(L / 16) void asm_rows_fwd_p(const float2* __restrict__ in, int nx, int ny, int ox,
                                                          int oy, float2* __restrict__ work,
-                                                         const float2* __restrict__ tw) {
+                                                         const float2* __restrict__ tw, TabJob tj) {
+    fill_axis_tables(tj, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
     constexpr int TPL = L / 16;
     constexpr int PL = fft::padded_len(L);
     __shared__ float2 buf[PL];
@@ -280,22 +289,136 @@ __global__ __launch_bounds__(L / 16) void asm_rows_inv_p(const float2* __restric
     }
 }
 
+// Persistent row passes for L = 16^a with the first and last transform
+// stages in registers: the prefetched row goes straight into the first
+// butterfly and the last butterfly straight to global memory (thread t holds
+// elements t + m L/16 in both), so the line passes through LDS only between
+// stages.  HALF: the data columns [ox, ox + nx) lie inside [L/4, 3L/4)
+// (twofold padding), so only m in [4, 12) can be nonzero on input (forward)
+// or is needed on output (inverse).
+template <int L, bool HALF>
+__global__ __launch_bounds__(L / 16) void asm_rows_fwd_f(const float2* __restrict__ in, int nx, int ny, int ox,
+                                                         int oy, float2* __restrict__ work,
+                                                         const float2* __restrict__ tw, TabJob tj) {
+    fill_axis_tables(tj, blockIdx.x * blockDim.x + threadIdx.x, gridDim.x * blockDim.x);
+    constexpr int TPL = L / 16;
+    constexpr int PL = fft::padded_len(L);
+    constexpr int M0 = HALF ? 4 : 0, M1 = HALF ? 12 : 16;
+    __shared__ float2 buf[PL];
+    const int t = threadIdx.x;
+    float2 nxt[M1 - M0];
+    int r = blockIdx.x;
+    auto fetch = [&](int row) {
+#pragma unroll
+        for (int m = M0; m < M1; ++m) {
+            const int c = t + m * TPL - ox;
+            nxt[m - M0] = (row < ny && c >= 0 && c < nx) ? in[(size_t)row * nx + c] : make_float2(0.f, 0.f);
+        }
+    };
+    fetch(r);
+    for (; r < ny; r += gridDim.x) {
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = (m >= M0 && m < M1) ? nxt[m - M0] : make_float2(0.f, 0.f);
+        fetch(r + gridDim.x);
+        __syncthreads();  // the previous row's last-stage reads are done
+        fft::stage0_from_regs<L, -1>(v, buf, t);
+        __syncthreads();
+        // opaque copy of the table pointer: the twiddle loads stay in the
+        // loop instead of pinning every stage's twiddles in registers
+        const float2* twl = tw;
+        asm volatile("" : "+s"(twl));
+        fft::middle_stages_tw<L, -1>(buf, twl, t);
+        fft::last_stage_to_regs<L, -1>(buf, twl, t, v);
+        float2* dst = work + (size_t)(oy + r) * L;
+#pragma unroll
+        for (int m = 0; m < 16; ++m) dst[t + m * TPL] = v[m];
+    }
+}
+
+template <int L, bool HALF>
+__global__ __launch_bounds__(L / 16) void asm_rows_inv_f(const float2* __restrict__ work, int nx, int ny, int ox,
+                                                         int oy, float scale, float2* __restrict__ out,
+                                                         const float2* __restrict__ tw) {
+    constexpr int TPL = L / 16;
+    constexpr int PL = fft::padded_len(L);
+    constexpr int M0 = HALF ? 4 : 0, M1 = HALF ? 12 : 16;
+    __shared__ float2 buf[PL];
+    const int t = threadIdx.x;
+    float2 nxt[16];
+    int r = blockIdx.x;
+    auto fetch = [&](int row) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            nxt[m] = row < ny ? work[(size_t)(oy + row) * L + t + m * TPL] : make_float2(0.f, 0.f);
+    };
+    fetch(r);
+    for (; r < ny; r += gridDim.x) {
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = nxt[m];
+        fetch(r + gridDim.x);
+        __syncthreads();
+        fft::stage0_from_regs<L, +1>(v, buf, t);
+        __syncthreads();
+        // opaque copy of the table pointer: the twiddle loads stay in the
+        // loop instead of pinning every stage's twiddles in registers
+        const float2* twl = tw;
+        asm volatile("" : "+s"(twl));
+        fft::middle_stages_tw<L, +1>(buf, twl, t);
+        fft::last_stage_to_regs<L, +1>(buf, twl, t, v);
+        float2* dst = out + (size_t)r * nx;
+#pragma unroll
+        for (int m = M0; m < M1; ++m) {
+            const int c = t + m * TPL - ox;
+            if (c >= 0 && c < nx) dst[c] = make_float2(v[m].x * scale, v[m].y * scale);
+        }
+    }
+}
+
 // resident workgroups per CU of the persistent row passes (register-bound
 // occupancy of each kernel, 4 waves per workgroup)
 constexpr int ROWS_FWD_PER_CU = 2, ROWS_INV_PER_CU = 3;
 
+// Workgroups of a persistent kernel resident per CU, from the runtime's
+// occupancy calculator (registers and LDS of the compiled kernel).
+template <typename K>
+int resident_per_cu(K kernel, int threads) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(kernel), threads, 0) !=
+            hipSuccess ||
+        n < 1) {
+        hipGetLastError();
+        n = 1;
+    }
+    return n;
+}
+
 template <int LX>
-int launch_rows_fwd(oph_ctx* ctx, const float2* in, int nx, int ny, int ox, int oy, float2* work) {
+int launch_rows_fwd(oph_ctx* ctx, const float2* in, int nx, int ny, int ox, int oy, float2* work, const TabJob& tj) {
     const float2* tw = oph_twiddles(ctx, LX);
     if (!tw) return OPH_ENOMEM;
     constexpr int LPW = fftk::row_lines(LX);
     if constexpr (LPW == 1) {
         const int g = std::min(ny, ROWS_FWD_PER_CU * std::max(ctx->cu_count, 1));
-        OPH_LAUNCH(ctx, "asm_rows_fwd", asm_rows_fwd_p<LX>, dim3(g), dim3(LX / 16), 0, in, nx, ny, ox, oy, work, tw);
+        if constexpr (fft::regs_fusable<LX>()) {
+            static const int occ_h = resident_per_cu(asm_rows_fwd_f<LX, true>, LX / 16);
+            static const int occ_f = resident_per_cu(asm_rows_fwd_f<LX, false>, LX / 16);
+            const int cus = std::max(ctx->cu_count, 1);
+            if (ox >= LX / 4 && ox + nx <= 3 * LX / 4)
+                OPH_LAUNCH(ctx, "asm_rows_fwd", (asm_rows_fwd_f<LX, true>), dim3(std::min(ny, occ_h * cus)),
+                           dim3(LX / 16), 0, in, nx, ny, ox, oy, work, tw, tj);
+            else
+                OPH_LAUNCH(ctx, "asm_rows_fwd", (asm_rows_fwd_f<LX, false>), dim3(std::min(ny, occ_f * cus)),
+                           dim3(LX / 16), 0, in, nx, ny, ox, oy, work, tw, tj);
+            return OPH_OK;
+        }
+        OPH_LAUNCH(ctx, "asm_rows_fwd", asm_rows_fwd_p<LX>, dim3(g), dim3(LX / 16), 0, in, nx, ny, ox, oy, work, tw,
+                   tj);
         return OPH_OK;
     }
     OPH_LAUNCH(ctx, "asm_rows_fwd", asm_rows_fwd<LX>, dim3((ny + LPW - 1) / LPW), dim3(LPW * (LX / 16)), 0, in,
-               nx, ny, ox, oy, work, tw);
+               nx, ny, ox, oy, work, tw, tj);
     return OPH_OK;
 }
 
@@ -306,6 +429,18 @@ int launch_rows_inv(oph_ctx* ctx, const float2* work, int nx, int ny, int ox, in
     constexpr int LPW = fftk::row_lines(LX);
     if constexpr (LPW == 1) {
         const int g = std::min(ny, ROWS_INV_PER_CU * std::max(ctx->cu_count, 1));
+        if constexpr (fft::regs_fusable<LX>()) {
+            static const int occ_h = resident_per_cu(asm_rows_inv_f<LX, true>, LX / 16);
+            static const int occ_f = resident_per_cu(asm_rows_inv_f<LX, false>, LX / 16);
+            const int cus = std::max(ctx->cu_count, 1);
+            if (ox >= LX / 4 && ox + nx <= 3 * LX / 4)
+                OPH_LAUNCH(ctx, "asm_rows_inv", (asm_rows_inv_f<LX, true>), dim3(std::min(ny, occ_h * cus)),
+                           dim3(LX / 16), 0, work, nx, ny, ox, oy, scale, out, tw);
+            else
+                OPH_LAUNCH(ctx, "asm_rows_inv", (asm_rows_inv_f<LX, false>), dim3(std::min(ny, occ_f * cus)),
+                           dim3(LX / 16), 0, work, nx, ny, ox, oy, scale, out, tw);
+            return OPH_OK;
+        }
         OPH_LAUNCH(ctx, "asm_rows_inv", asm_rows_inv_p<LX>, dim3(g), dim3(LX / 16), 0, work, nx, ny, ox, oy, scale,
                    out, tw);
         return OPH_OK;
@@ -372,8 +507,7 @@ int oph_asm_device(oph_ctx* ctx, float2* field, int nx, int ny, int npx, int npy
     const int ox = npx / 2 - nx / 2, oy = npy / 2 - ny / 2;
     float4* tab = static_cast<float4*>(oph_scratch(ctx, SLOT_TABLE, sizeof(float4) * (npx + npy)));
     if (!tab) return OPH_ENOMEM;
-    OPH_LAUNCH(ctx, "asm_axis_table", asm_axis_table, dim3((npx + npy + 255) / 256), dim3(256), 0, tab, npx, px,
-               npy, py, lambda, z);
+    const TabJob tj{tab, npx, npy, px, py, lambda, z};
     const double zl = z / lambda;
     AsmConst k;
     k.zl = zl;
@@ -390,7 +524,7 @@ int oph_asm_device(oph_ctx* ctx, float2* field, int nx, int ny, int npx, int npy
     const bool exact = umax >= 0.5 || fabs(zl) * pow(umax, 6) * (21.0 / 1024.0) > 1e-7;
     int rc;
     switch (npx) {
-#define OPH_RF(n) case n: rc = launch_rows_fwd<n>(ctx, field, nx, ny, ox, oy, work); break;
+#define OPH_RF(n) case n: rc = launch_rows_fwd<n>(ctx, field, nx, ny, ox, oy, work, tj); break;
         OPH_FFT_LENGTHS(OPH_RF)
 #undef OPH_RF
         default: return oph_fail(ctx, OPH_ENOTSUP, "ASM grid width %d unsupported", npx);

@@ -295,6 +295,65 @@ __device__ __forceinline__ void line_tw(float2* buf, const float2* __restrict__ 
     }
 }
 
+// First and last stages with the thread's elements in registers, for lines
+// of L = 16^a (one radix-16 butterfly per thread and stage, TPL = L/16).
+// Thread t's stage-0 inputs are elements t + r L/16 (r = 0..15) and its
+// last-stage outputs are the same elements, so a kernel can take the line
+// from global memory straight into the first butterfly and write the last
+// butterfly straight back, without staging the line in LDS.
+template <int L>
+__host__ __device__ constexpr bool regs_fusable() { return n16(L) >= 2 && last_radix(L) == 1; }
+
+// Stage 0 (Ns = 1) on v (elements t + r L/16), result written to the line.
+template <int L, int SIGN>
+__device__ __forceinline__ void stage0_from_regs(float2 (&v)[16], float2* buf, int t) {
+    static_assert(regs_fusable<L>(), "length must be a power of 16");
+    Dft<16, SIGN>::run(v);
+    float2* dst = buf + pad16(16 * t);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) dst[r] = v[r];
+}
+
+// Middle stages s = 1 .. a-2 of line_tw (twiddles from the table).
+template <int L, int SIGN>
+__device__ __forceinline__ void middle_stages_tw(float2* buf, const float2* __restrict__ tw, int t) {
+    constexpr int TPL = L / 16;
+    constexpr int A = n16(L);
+    int Ns = 16, off = 0;
+#pragma unroll
+    for (int s = 1; s < A - 1; ++s) {
+        float2 w[1][16];
+        load_stage_tw<L, 16, TPL>(tw, off, Ns, t, w);
+        stage<L, 16, SIGN, TPL>(buf, w, Ns, t);
+        off += 15 * Ns;
+        Ns *= 16;
+    }
+}
+
+// Last radix-16 stage (Ns = L/16) read from the line into v: on return v[r]
+// is output element t + r L/16.  Callers sync before reusing the line.
+template <int L, int SIGN>
+__device__ __forceinline__ void last_stage_to_regs(const float2* buf, const float2* __restrict__ tw, int t,
+                                                   float2 (&v)[16]) {
+    static_assert(regs_fusable<L>(), "length must be a power of 16");
+    constexpr int TPL = L / 16;
+    constexpr int A = n16(L);
+    int off = 0, Ns = 16;
+#pragma unroll
+    for (int s = 1; s < A - 1; ++s) {
+        off += 15 * Ns;
+        Ns *= 16;
+    }
+    const float2* src = buf + pad16(t);
+#pragma unroll
+    for (int r = 0; r < 16; ++r) v[r] = src[r * (TPL + TPL / 16)];
+    float2 w[1][16];
+    load_stage_tw<L, 16, TPL>(tw, off, Ns, t, w);
+#pragma unroll
+    for (int r = 1; r < 16; ++r) v[r] = SIGN < 0 ? cmul(v[r], w[0][r]) : cmulc(v[r], w[0][r]);
+    Dft<16, SIGN>::run(v);
+}
+
 // Radix-8 variant: L/8 threads per line, each holding 8 elements
 // (t + m L/8, m = 0..7), radix-8 stages plus one radix-2/4 stage, twiddles
 // of each stage loaded from the radix-8 table just before it.  Half the
