@@ -35,6 +35,7 @@
 // so no square root or division per pixel; points whose window makes the
 // truncated series worse than 1e-7 turns use an exact FP64 evaluation.
 #include <cmath>
+#include <type_traits>
 
 #include "oph_internal.h"
 #include "pipelines.h"
@@ -347,51 +348,63 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
             int ix = lane - iy * na;
             // byte address of the window centre in the accumulator
             const int cbyte = ((ty - Y0) * TILE + (tx - X0)) * 8;
-            while (iy < nay) {
-                const int ax = ax0 + ix, ay = ay0 + iy;
-                float re, im;
-                if (!q.exact) {
-                    const float fx = (float)ax, fy = (float)ay;
-                    const float u = fmaf(fx, fx, fy * fy) * q.p2dz2;
-                    const float p = u * fmaf(u, fmaf(u, fmaf(u, -5.f / 128.f, 1.f / 16.f), -0.125f), 0.5f);
-                    const float ph = fmaf(q.ks, p, q.f0);
-                    const float am = q.ampd * fmaf(q.cd, p, 1.f);
-                    re = am * cos_turns(ph);
-                    im = am * sin_turns(ph);
-                } else {
-                    const double dz = g.pts[q.idx].dz;
-                    const double dx = ax * g.pitch, dy = ay * g.pitch;
-                    const double sgn = dz > 0.0 ? 1.0 : -1.0;
-                    const double r = sgn * sqrt(dx * dx + dy * dy + dz * dz);
-                    double t = r * g.inv_lambda;
-                    t -= rint(t);
-                    // amplitude a/(r + 0.05) = (a/D) D/(r + 0.05), D = sign|dz| + 0.05
-                    const double D = sgn * fabs(dz) + 0.05;
-                    const float am = (float)((double)q.ampd * D / (r + 0.05));
-                    re = am * cos_turns((float)t);
-                    im = am * sin_turns((float)t);
+            // the point's amplitude with the tile's fixed-point scale folded in
+            const float amps = q.ampd * scale;
+            // one loop per evaluation path (the path is uniform over the point)
+            auto walk = [&](auto exact_tag) {
+                constexpr bool EXACT = decltype(exact_tag)::value;
+                while (iy < nay) {
+                    const int ax = ax0 + ix, ay = ay0 + iy;
+                    float re, im;
+                    if constexpr (!EXACT) {
+                        const float fx = (float)ax, fy = (float)ay;
+                        const float u = fmaf(fx, fx, fy * fy) * q.p2dz2;
+                        const float p = u * fmaf(u, fmaf(u, fmaf(u, -5.f / 128.f, 1.f / 16.f), -0.125f), 0.5f);
+                        const float ph = fmaf(q.ks, p, q.f0);
+                        const float am = amps * fmaf(q.cd, p, 1.f);
+                        re = am * cos_turns(ph);
+                        im = am * sin_turns(ph);
+                    } else {
+                        const double dz = g.pts[q.idx].dz;
+                        const double dx = ax * g.pitch, dy = ay * g.pitch;
+                        const double sgn = dz > 0.0 ? 1.0 : -1.0;
+                        const double r = sgn * sqrt(dx * dx + dy * dy + dz * dz);
+                        double t = r * g.inv_lambda;
+                        t -= rint(t);
+                        // amplitude a/(r + 0.05) = (a/D) D/(r + 0.05), D = sign|dz| + 0.05
+                        const double D = sgn * fabs(dz) + 0.05;
+                        const float am = (float)((double)amps * D / (r + 0.05));
+                        re = am * cos_turns((float)t);
+                        im = am * sin_turns((float)t);
+                    }
+                    // fixed point, truncated toward zero (any fixed rounding
+                    // keeps the sum exact; the unit is 2^-30 of the bound)
+                    const int ire = (int)re;
+                    const int iim = (int)im;
+                    // 2^32 im + re modulo 2^64
+                    const unsigned long long pk =
+                        (unsigned long long)(long long)ire + ((unsigned long long)(long long)iim << 32);
+                    const bool xp = ax >= wx0 && ax < wx1, xm = ax > 0 && -ax >= wx0;
+                    const bool yp = ay >= wy0 && ay < wy1, ym = ay > 0 && -ay >= wy0;
+                    // byte offsets of the mirrored pixels (8 bytes per pixel)
+                    const int rp = cbyte + (ay << 9), rm = cbyte - (ay << 9), cx = ax << 3;
+                    char* ba = reinterpret_cast<char*>(acc);
+                    if (xp && yp) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rp + cx), pk);
+                    if (xm && yp) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rp - cx), pk);
+                    if (xp && ym) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rm + cx), pk);
+                    if (xm && ym) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rm - cx), pk);
+                    ix += sx;
+                    iy += sy;
+                    if (ix >= na) {
+                        ix -= na;
+                        ++iy;
+                    }
                 }
-                const int ire = (int)rintf(re * scale);
-                const int iim = (int)rintf(im * scale);
-                // 2^32 im + re modulo 2^64
-                const unsigned long long pk =
-                    (unsigned long long)(long long)ire + ((unsigned long long)(long long)iim << 32);
-                const bool xp = ax >= wx0 && ax < wx1, xm = ax > 0 && -ax >= wx0;
-                const bool yp = ay >= wy0 && ay < wy1, ym = ay > 0 && -ay >= wy0;
-                // byte offsets of the mirrored pixels (8 bytes per pixel)
-                const int rp = cbyte + (ay << 9), rm = cbyte - (ay << 9), cx = ax << 3;
-                char* ba = reinterpret_cast<char*>(acc);
-                if (xp && yp) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rp + cx), pk);
-                if (xm && yp) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rp - cx), pk);
-                if (xp && ym) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rm + cx), pk);
-                if (xm && ym) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rm - cx), pk);
-                ix += sx;
-                iy += sy;
-                if (ix >= na) {
-                    ix -= na;
-                    ++iy;
-                }
-            }
+            };
+            if (q.exact)
+                walk(std::true_type{});
+            else
+                walk(std::false_type{});
         }
     }
     __syncthreads();
