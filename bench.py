@@ -127,7 +127,7 @@ def _time_steps(ctx, step, args, kernels):
     return min(wall, wall_noprof), per
 
 
-WRP_KERNELS = ["wrp_bin_count", "wrp_bin_colscan", "wrp_scan", "wrp_bin_fill", "wrp_gather",
+WRP_KERNELS = ["wrp_bin_count", "wrp_bin_colscan", "wrp_bin_fill", "wrp_gather",
                "asm_rows_fwd", "asm_cols", "asm_rows_inv"]
 
 

@@ -17,9 +17,9 @@
 //                      window, FP64-reduced phase constants) and per-tile
 //                      counts in LDS;
 //   2. wrp_bin_colscan per tile: prefix of the counts over chunks;
-//   3. wrp_scan        tile offsets;
-//   4. wrp_bin_fill    per chunk: point indices into the tile lists;
-//   5. wrp_gather      one workgroup per tile: each wave takes points of the
+//   3. wrp_bin_fill    per chunk: tile offsets (scan of the tile totals, in
+//                      LDS), then point indices into the tile lists;
+//   4. wrp_gather      one workgroup per tile: each wave takes points of the
 //                      list; a window value depends on (|wx|, |wy|) only, so
 //                      each is evaluated once and added to its mirrored
 //                      pixels; contributions are summed in LDS as 32-bit
@@ -142,7 +142,10 @@ __global__ __launch_bounds__(BIN_THREADS) void wrp_bin_count(WrpArgs a) {
     __shared__ float wmax[BIN_THREADS / 64];
     for (int t = threadIdx.x; t < a.ntiles; t += BIN_THREADS) cnt[t] = 0u;
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * a.chunk;
+    // chunks numbered XCD by XCD, so that neighbouring columns of the count
+    // matrix are written through the same L2
+    const int chunk = xcd_remap(blockIdx.x, gridDim.x);
+    const int64_t base = (int64_t)chunk * a.chunk;
     float mx = 0.f;
     for (int k = threadIdx.x; k < a.chunk; k += BIN_THREADS) {
         const int64_t i = base + k;
@@ -160,11 +163,11 @@ __global__ __launch_bounds__(BIN_THREADS) void wrp_bin_count(WrpArgs a) {
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = mx;
     __syncthreads();
     for (int t = threadIdx.x; t < a.ntiles; t += BIN_THREADS)
-        a.chunk_counts[(size_t)t * a.nchunks + blockIdx.x] = cnt[t];
+        a.chunk_counts[(size_t)t * a.nchunks + chunk] = cnt[t];
     if (threadIdx.x == 0) {
         float m = 0.f;
         for (int w = 0; w < BIN_THREADS / 64; ++w) m = fmaxf(m, wmax[w]);
-        a.chunk_max[blockIdx.x] = m;
+        a.chunk_max[chunk] = m;
     }
 }
 
@@ -183,53 +186,67 @@ __global__ __launch_bounds__(256) void wrp_bin_colscan(WrpArgs a) {
     }
     if (wave >= a.ntiles) return;
     unsigned* p = a.chunk_counts + (size_t)wave * a.nchunks;
+    // all of the tile's counts are loaded first (one memory round trip),
+    // then scanned 64 at a time
+    constexpr int PER_LANE = MAX_CHUNKS / 64;
+    unsigned v[PER_LANE];
+#pragma unroll
+    for (int j = 0; j < PER_LANE; ++j) {
+        const int c = j * 64 + lane;
+        v[j] = c < a.nchunks ? p[c] : 0u;
+    }
     unsigned run = 0;
-    for (int c0 = 0; c0 < a.nchunks; c0 += 64) {
-        const int c = c0 + lane;
-        const unsigned v = c < a.nchunks ? p[c] : 0u;
-        unsigned incl = v;
+#pragma unroll
+    for (int j = 0; j < PER_LANE; ++j) {
+        if (j * 64 >= a.nchunks) break;
+        unsigned incl = v[j];
         for (int off = 1; off < 64; off <<= 1) {
             const unsigned o = __shfl_up(incl, off);
             if (lane >= off) incl += o;
         }
-        if (c < a.nchunks) p[c] = run + incl - v;
+        const int c = j * 64 + lane;
+        if (c < a.nchunks) p[c] = run + incl - v[j];
         run += __shfl(incl, 63);
     }
     if (lane == 0) a.totals[wave] = run;
 }
 
-// exclusive scan of counts into offsets[0..n], one workgroup of 1024 threads
-__global__ __launch_bounds__(1024) void wrp_scan(const unsigned* counts, unsigned* offsets, int n) {
-    __shared__ unsigned part[1024];
-    const int t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int b = t * per, e = min(n, b + per);
-    unsigned s = 0;
-    for (int i = b; i < e; ++i) s += counts[i];
-    part[t] = s;
+// pass 3: per chunk, point indices into the tile lists (slot = tile offset +
+// chunk prefix + rank within the chunk from an LDS counter).  Every
+// workgroup forms the tile offsets (exclusive prefix of the tile totals) in
+// LDS itself; chunk 0 also stores them for the gather.  Dynamic LDS: 2 ntiles
+// words.
+__global__ __launch_bounds__(BIN_THREADS) void wrp_bin_fill(WrpArgs a) {
+    extern __shared__ unsigned fill_lds[];
+    __shared__ unsigned part[BIN_THREADS];
+    unsigned* cur = fill_lds;
+    unsigned* offs = fill_lds + a.ntiles;
+    const int n = a.ntiles, t = threadIdx.x;
+    const int per = (n + BIN_THREADS - 1) / BIN_THREADS;
+    const int b = min(n, t * per), e = min(n, b + per);
+    unsigned sum = 0;
+    for (int i = b; i < e; ++i) {
+        cur[i] = 0u;
+        sum += a.totals[i];
+    }
+    part[t] = sum;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
+    for (int off = 1; off < BIN_THREADS; off <<= 1) {
         const unsigned v = t >= off ? part[t - off] : 0u;
         __syncthreads();
         part[t] += v;
         __syncthreads();
     }
-    unsigned run = part[t] - s;  // exclusive prefix of this chunk
+    const int chunk = xcd_remap(blockIdx.x, gridDim.x);
+    unsigned run = part[t] - sum;  // exclusive prefix of this thread's range
     for (int i = b; i < e; ++i) {
-        offsets[i] = run;
-        run += counts[i];
+        offs[i] = run;
+        if (chunk == 0) a.offsets[i] = run;
+        run += a.totals[i];
     }
-    if (t == 1023) offsets[n] = part[1023];
-}
-
-// pass 3: per chunk, point indices into the tile lists (slot = tile offset +
-// chunk prefix + rank within the chunk from an LDS counter)
-__global__ __launch_bounds__(BIN_THREADS) void wrp_bin_fill(WrpArgs a) {
-    __shared__ unsigned cur[MAX_TILES];
-    for (int t = threadIdx.x; t < a.ntiles; t += BIN_THREADS) cur[t] = 0u;
+    if (chunk == 0 && t == BIN_THREADS - 1) a.offsets[n] = part[BIN_THREADS - 1];
     __syncthreads();
-    const int64_t base = (int64_t)blockIdx.x * a.chunk;
-
+    const int64_t base = (int64_t)chunk * a.chunk;
     for (int k = threadIdx.x; k < a.chunk; k += BIN_THREADS) {
         const int64_t i = base + k;
         if (i >= a.n) break;
@@ -240,8 +257,8 @@ __global__ __launch_bounds__(BIN_THREADS) void wrp_bin_fill(WrpArgs a) {
         for (int ty = y0; ty <= y1; ++ty)
             for (int tx = x0; tx <= x1; ++tx) {
                 const int tile = ty * a.ntx + tx;
-                const unsigned slot = a.offsets[tile] + a.chunk_counts[(size_t)tile * a.nchunks + blockIdx.x] +
-                                      atomicAdd(&cur[tile], 1u);
+                const unsigned slot =
+                    offs[tile] + a.chunk_counts[(size_t)tile * a.nchunks + chunk] + atomicAdd(&cur[tile], 1u);
                 if (slot < a.capacity)
                     a.list[slot] = (unsigned)i;
                 else
@@ -480,8 +497,15 @@ int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, do
         a.capacity = (unsigned)(cap < 0xFFFFFFFFull ? cap : 0xFFFFFFFFull);
         OPH_LAUNCH(ctx, "wrp_bin_count", wrp_bin_count, dim3(nchunks), dim3(BIN_THREADS), 0, a);
         OPH_LAUNCH(ctx, "wrp_bin_colscan", wrp_bin_colscan, dim3((ntiles + 3) / 4), dim3(256), 0, a);
-        OPH_LAUNCH(ctx, "wrp_scan", wrp_scan, dim3(1), dim3(1024), 0, a.totals, a.offsets, ntiles);
-        OPH_LAUNCH(ctx, "wrp_bin_fill", wrp_bin_fill, dim3(nchunks), dim3(BIN_THREADS), 0, a);
+        static bool fill_attr = false;
+        if (!fill_attr) {
+            hipFuncSetAttribute(reinterpret_cast<const void*>(&wrp_bin_fill),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)(2 * sizeof(unsigned) * MAX_TILES));
+            hipGetLastError();
+            fill_attr = true;
+        }
+        OPH_LAUNCH(ctx, "wrp_bin_fill", wrp_bin_fill, dim3(nchunks), dim3(BIN_THREADS),
+                   2 * sizeof(unsigned) * (size_t)ntiles, a);
         GatherArgs g{};
         g.pts = a.pts;
         g.offsets = a.offsets;
