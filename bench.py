@@ -332,7 +332,7 @@ def bench_fresnel(ctx, args):
     return line
 
 
-CNN_KERNELS = ["conv3x3_fwd", "conv3x3_dw", "dw_reduce", "conv_flip", "relu_mask", "mse_grad", "sum_partials",
+CNN_KERNELS = ["conv3x3_fwd", "conv3x3_dw", "dw_reduce", "pack_weights", "relu_mask", "mse_grad", "sum_partials",
                "adam_step"]
 
 
@@ -399,18 +399,21 @@ def bench_learned(ctx, args):
     def learned_step():
         ctx.depthmap(n, n, N, N, learned.PITCH, learned.NEAR, learned.FAR, 16, lam, dimg, ddep, out="field",
                      quant_mode=1, rho=(1.0 / scale, 0.0), device=True, asynchronous=True, dst=c_out)
-        net.forward(c_out, out=y_out, device=True, shape=(1, n, n, 2))
+        net.forward(c_out, out=y_out, device=True, shape=(1, n, n, 2), bf16_acts=True)
     a2 = argparse.Namespace(**vars(args))
     a2.steps = max(5, args.steps // 2)
     w_full, per_full = _time_steps(ctx, full_step, a2, DM_KERNELS)
-    w_l, per_l = _time_steps(ctx, learned_step, a2, DM_KERNELS + ["conv3x3_fwd"])
+    w_l, per_l = _time_steps(ctx, learned_step, a2, DM_KERNELS + ["conv3x3_fwd", "resblock_fwd"])
     hz_full, hz_l = a2.steps / w_full, a2.steps / w_l
     passed = ev[:, 3] >= 35.0
     # with every frame audited, each frame pays the learned path plus the
     # 256-layer check (and its 6 reconstructions); a failing frame returns
     # the check's own 256-layer field
     conv_flops = 2.0 * n * n * 9 * (2 * 32 + 16 * 32 * 32 + 32 * 2)
-    kt = per_l.get("conv3x3_fwd", {})
+    # convolution time per frame: head and tail (conv3x3_fwd) plus the fused
+    # residual blocks (resblock_fwd, BF16 activations)
+    conv_us = sum(per_l[k]["launches_per_step"] * per_l[k]["avg_us"]
+                  for k in ("conv3x3_fwd", "resblock_fwd") if k in per_l)
     line = {"value": hz_l, "unit": "holograms/s (16-layer + CNN, no per-frame check)",
             "ms_per_step": 1e3 / hz_l,
             "config": {"workload": "learned_1024sq_grid2048_16_to_256_layers_8x32ch_532nm",
@@ -427,8 +430,8 @@ def bench_learned(ctx, args):
             "train": {"seconds": t_train, "pair_gen_seconds": t_pairs, "loss_first": hist[0],
                       "loss_last": float(np.mean(hist[-20:])), "kernels": train_k},
             "kernels": {"full": per_full, "learned": per_l}}
-    if kt:
-        line["conv_fwd_tflops"] = conv_flops / (kt["launches_per_step"] * kt["avg_us"] * 1e-6) / 1e12
+    if conv_us:
+        line["conv_fwd_tflops"] = conv_flops / (conv_us * 1e-6) / 1e12
     if not args.no_cpu:
         # the reference CPU path for this workload is the 256-layer hologram
         # (one colour at 1024^2): FP64 oracle on 4 populated layers, scaled

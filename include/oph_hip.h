@@ -265,6 +265,12 @@ int64_t oph_cnn_num_params(const oph_cnn* net);
 /* which = 0: parameters, 1: gradients of the last oph_cnn_gradients/train_step */
 int oph_cnn_get_params(oph_ctx* ctx, oph_cnn* net, float* host, int which);
 int oph_cnn_set_params(oph_ctx* ctx, oph_cnn* net, const float* host);
+/* flags: OPH_DEVICE_PTR, OPH_ASYNC, and OPH_CNN_BF16_ACTS: keep the
+ * activations between layers in BF16 (inference; the convolution operands
+ * are BF16-rounded in both forms, this also rounds the stored intermediate
+ * and residual tensors, halving their HBM traffic).  Training always uses
+ * float32 activations. */
+#define OPH_CNN_BF16_ACTS 0x1000
 int oph_cnn_forward(oph_ctx* ctx, oph_cnn* net, const float* x, float* y, int batch, int h, int w,
                     int flags);
 /* loss = mean over all elements of (y - target)^2; gradients only, or

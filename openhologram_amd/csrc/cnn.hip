@@ -20,6 +20,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <type_traits>
 #include <vector>
 
 #include "oph_internal.h"
@@ -38,6 +39,9 @@ struct oph_cnn {
     bf16x8* wf = nullptr;     // BF16 B fragments of every forward convolution
     bf16x8* wb = nullptr;     // ... of every input-gradient convolution (flipped taps, transposed channels)
     int64_t step = 0;
+    // parameter revision and the revision the forward fragments were packed
+    // from (inference skips the packing while they agree)
+    uint64_t params_rev = 1, wf_rev = 0;
     // activation workspace
     float* act = nullptr;
     size_t act_bytes = 0;
@@ -70,15 +74,20 @@ struct Layout {
 
 // ---- forward convolution (implicit GEMM) -------------------------------------
 
-struct ConvArgs {
-    const float* in;      // [B][H][W][CIN]
+// TI / TO: element type of the input / output (and residual) tensors,
+// float or __bf16 (inference keeps the activations between layers in BF16:
+// the MFMA operands are BF16-rounded either way)
+template <typename TI = float, typename TO = float>
+struct ConvArgsT {
+    const TI* in;         // [B][H][W][CIN]
     const bf16x8* wf;     // B fragments [KSTEPS][64 lanes] (pack_weights)
     const float* bias;    // [COUT] or null
-    const float* res;     // residual added after the bias (or null), [..][COUT]
-    float* out;           // [B][H][W][COUT]
+    const TO* res;        // residual added after the bias (or null), [..][COUT]
+    TO* out;              // [B][H][W][COUT]
     int B, H, W;
     int relu;
 };
+typedef ConvArgsT<float, float> ConvArgs;
 
 // A operand for lane (pixel p, k0..k0+7): taps/channels of the input patch
 template <int CIN>
@@ -124,11 +133,13 @@ __device__ __forceinline__ bf16x8 load_patch(const ConvArgs& a, int b, int y, in
 constexpr int CT_H = 8, CT_W = 32;
 constexpr int CT_WAVES = CT_H / 2;
 
+__host__ __device__ constexpr int ksteps_c(int cin) { return (9 * cin + 15) / 16; }
+
 template <int CIN>
 __host__ __device__ constexpr int tile_stride() { return CIN % 8 == 0 ? CIN + 8 : CIN; }
 
-template <int CIN, int COUT>
-__global__ __launch_bounds__(64 * CT_WAVES) void conv3x3_fwd(ConvArgs a) {
+template <int CIN, int COUT, typename TI = float, typename TO = float>
+__global__ __launch_bounds__(64 * CT_WAVES) void conv3x3_fwd(ConvArgsT<TI, TO> a) {
     constexpr int K = 9 * CIN;
     constexpr int KSTEPS = (K + 15) / 16;
     constexpr int PS = tile_stride<CIN>();
@@ -139,7 +150,17 @@ __global__ __launch_bounds__(64 * CT_WAVES) void conv3x3_fwd(ConvArgs a) {
     const size_t img = (size_t)b * a.H * a.W;
 
     // stage the tile (+ halo) as BF16; outside the image is zero padding
-    if constexpr (CIN % 4 == 0) {
+    if constexpr (std::is_same<TI, __bf16>::value && CIN % 8 == 0) {
+        constexpr int Q = CIN / 8;  // 16-byte vectors per pixel
+        for (int i = threadIdx.x; i < (CT_H + 2) * TWH * Q; i += 64 * CT_WAVES) {
+            const int pix = i / Q, q = i % Q;
+            const int yy = y0 + pix / TWH - 1, xx = x0 + pix % TWH - 1;
+            bf16x8 v = (bf16x8)(__bf16)0.f;
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                v = reinterpret_cast<const bf16x8*>(a.in + (img + (size_t)yy * a.W + xx) * CIN)[q];
+            *reinterpret_cast<bf16x8*>(tile + pix * PS + 8 * q) = v;
+        }
+    } else if constexpr (CIN % 4 == 0 && std::is_same<TI, float>::value) {
         constexpr int Q = CIN / 4;  // float4 per pixel
         for (int i = threadIdx.x; i < (CT_H + 2) * TWH * Q; i += 64 * CT_WAVES) {
             const int pix = i / Q, q = i % Q;
@@ -159,7 +180,7 @@ __global__ __launch_bounds__(64 * CT_WAVES) void conv3x3_fwd(ConvArgs a) {
             const int pix = i / CIN, c = i % CIN;
             const int yy = y0 + pix / TWH - 1, xx = x0 + pix % TWH - 1;
             float v = 0.f;
-            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = a.in[(img + (size_t)yy * a.W + xx) * CIN + c];
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W) v = (float)a.in[(img + (size_t)yy * a.W + xx) * CIN + c];
             tile[pix * PS + c] = (__bf16)v;
         }
     }
@@ -179,7 +200,13 @@ __global__ __launch_bounds__(64 * CT_WAVES) void conv3x3_fwd(ConvArgs a) {
         for (int h = 0; h < 2; ++h) {
             const int r = 2 * wave + h;
             bf16x8 A;
-            if constexpr (CIN % 8 == 0) {
+            if constexpr (CIN % 16 == 0) {
+                // k = 16 s + 8 kh stays within tap 16 s / CIN: compile-time
+                // tap and channel base, the lane's half added at run time
+                const int tap = (16 * s) / CIN, c = (16 * s) % CIN + 8 * kh;
+                const __bf16* src = tile + ((r + tap / 3) * TWH + col + tap % 3) * PS + c;
+                A = *reinterpret_cast<const bf16x8*>(src);
+            } else if constexpr (CIN % 8 == 0) {
                 const int tap = k0 / CIN, c = k0 % CIN;
                 if (tap < 9) {
                     const __bf16* src = tile + ((r + tap / 3) * TWH + col + tap % 3) * PS + c;
@@ -208,7 +235,7 @@ __global__ __launch_bounds__(64 * CT_WAVES) void conv3x3_fwd(ConvArgs a) {
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
             const int y = y0 + 2 * wave + h, x = x0 + 8 * (v >> 2) + 4 * kh + (v & 3);
-            rv[h][v] = (a.res && y < a.H && x < a.W) ? a.res[(img + (size_t)y * a.W + x) * COUT + n] : 0.f;
+            rv[h][v] = (a.res && y < a.H && x < a.W) ? (float)a.res[(img + (size_t)y * a.W + x) * COUT + n] : 0.f;
         }
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -220,7 +247,7 @@ __global__ __launch_bounds__(64 * CT_WAVES) void conv3x3_fwd(ConvArgs a) {
             if (x >= a.W) continue;
             float o = acc[h][v] + bias;
             if (a.relu) o = fmaxf(o, 0.f);
-            a.out[(img + (size_t)y * a.W + x) * COUT + n] = o + rv[h][v];
+            a.out[(img + (size_t)y * a.W + x) * COUT + n] = (TO)(o + rv[h][v]);
         }
     }
 }
@@ -435,15 +462,171 @@ __global__ void add_inplace(float* a, const float* b, int64_t n) {
     if (i < n) a[i] += b[i];
 }
 
+// ---- fused residual block (inference, BF16 activations) ----------------------
+// h' = h + conv2(relu(conv1(h))) for C = 32 channels in one kernel.  A
+// persistent workgroup of 4 waves (conv1's weight fragments in registers,
+// conv2's in LDS, loaded once) walks 8 x 32 output tiles; for each it
+// stages h on the tile
+// plus a two-pixel halo (12 x 36 pixels) in LDS, computes relu(conv1) on
+// the tile plus a one-pixel halo (10 x 34 = 340 pixels, taken as 11 flat
+// groups of 32 for the MFMA rows; pixels outside the image are the zero
+// padding of conv2) into a second LDS tile, then conv2 on the 8 rows of 32
+// with the residual read from the staged h.  HBM traffic per block: h read
+// once (plus halo), h' written once.  The MFMAs take the weights as the A
+// operand, so each lane ends with 4 runs of 4 consecutive channels of one
+// pixel (8-byte stores, one bounds check per pixel).  Same fragments,
+// BF16 rounding and FP32 accumulation as conv3x3_fwd.
+constexpr int RB_TH = 8, RB_TW = 32;
+constexpr int RB_IH = RB_TH + 4, RB_IW = RB_TW + 4;   // staged input
+constexpr int RB_MH = RB_TH + 2, RB_MW = RB_TW + 2;   // conv1 output
+constexpr int RB_MID = RB_MH * RB_MW;                 // 340 pixels
+constexpr int RB_GROUPS = (RB_MID + 31) / 32;         // 11
+constexpr int RB_PS = 32 + 8;                         // LDS pixel stride (bf16)
+
+struct ResArgs {
+    const __bf16* h;      // [B][H][W][32]
+    __bf16* out;          // [B][H][W][32], distinct from h
+    const bf16x8* w1;     // B fragments of conv1 / conv2 (pack_weights)
+    const bf16x8* w2;
+    const float* b1;
+    const float* b2;
+    int B, H, W;
+};
+
+__global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
+    constexpr int KSTEPS = ksteps_c(32);
+    __shared__ __attribute__((aligned(16))) __bf16 in_t[RB_IH * RB_IW * RB_PS];
+    __shared__ __attribute__((aligned(16))) __bf16 mid_t[RB_MID * RB_PS];
+    __shared__ bf16x8 w2_s[KSTEPS * 64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar branches
+    // conv1's fragments in registers, conv2's in LDS, loaded once for every
+    // tile this (persistent) workgroup takes
+    bf16x8 Bf[KSTEPS];
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) Bf[s] = a.w1[s * 64 + lane];
+    for (int i = threadIdx.x; i < KSTEPS * 64; i += 256) w2_s[i] = a.w2[i];
+    constexpr int GPW = (RB_GROUPS + 3) / 4;  // conv1 groups per wave (3)
+    const int tiles_x = (a.W + RB_TW - 1) / RB_TW, tiles_y = (a.H + RB_TH - 1) / RB_TH;
+    const int ntiles = tiles_x * tiles_y * a.B;
+    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
+        const int x0 = tx * RB_TW, y0 = ty * RB_TH;
+        const size_t img = (size_t)b * a.H * a.W;
+        // lane and wave indices through opaque copies: everything derived
+        // from them is recomputed per tile instead of being hoisted out of
+        // the loop into registers held for the kernel's lifetime
+        int lane_o = lane, wave_o = wave;
+        asm volatile("" : "+v"(lane_o), "+s"(wave_o));
+        const int px = lane_o & 31, kh = lane_o >> 5;
+        __syncthreads();  // the previous tile's LDS reads are done
+        for (int i = threadIdx.x; i < RB_IH * RB_IW * 4; i += 256) {
+            const int pix = i >> 2, q = i & 3;
+            const int yy = y0 + pix / RB_IW - 2, xx = x0 + pix % RB_IW - 2;
+            bf16x8 v = (bf16x8)(__bf16)0.f;
+            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                v = reinterpret_cast<const bf16x8*>(a.h + (img + (size_t)yy * a.W + xx) * 32)[q];
+            *reinterpret_cast<bf16x8*>(in_t + pix * RB_PS + 8 * q) = v;
+        }
+        __syncthreads();
+
+        // conv1 + bias + relu on the 340 halo pixels, groups wave, wave + 4,
+        // wave + 8 of 32 flat pixels (group 11 repeats the last pixel and is
+        // never stored).  MFMA with the weights as A: the result is
+        // [channel][pixel], register v of lane (px, kh) = channel
+        // 8 (v/4) + 4 kh + v%4 of pixel px.
+        {
+            f32x16 acc[GPW];
+            int base[GPW];  // LDS element offset of the lane's pixel, channel half
+#pragma unroll
+            for (int j = 0; j < GPW; ++j) {
+                acc[j] = (f32x16)0.f;
+                int p = (wave_o + 4 * j) * 32 + px;
+                if (p >= RB_MID) p = RB_MID - 1;
+                const int mr = (int)(((float)p + 0.5f) * (1.f / RB_MW));  // p / 34
+                base[j] = (mr * RB_IW + (p - mr * RB_MW)) * RB_PS + 8 * kh;
+            }
+            // k = 16 s + 8 kh: tap s / 2, channels 16 (s % 2) + 8 kh, so the
+            // per-step offset is a compile-time constant
+#pragma unroll
+            for (int s = 0; s < KSTEPS; ++s) {
+                const int tap = s >> 1;
+                const int off = ((tap / 3) * RB_IW + tap % 3) * RB_PS + 16 * (s & 1);
+#pragma unroll
+                for (int j = 0; j < GPW; ++j) {
+                    const bf16x8 X = *reinterpret_cast<const bf16x8*>(in_t + base[j] + off);
+                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bf[s], X, acc[j], 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < GPW; ++j) {
+                const int p = (wave_o + 4 * j) * 32 + px;
+                if (p >= RB_MID) continue;
+                const int mr = (int)(((float)p + 0.5f) * (1.f / RB_MW));
+                const int yy = y0 - 1 + mr, xx = x0 - 1 + (p - mr * RB_MW);
+                const bool inside = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;  // else conv2's zero padding
+                __bf16* dst = mid_t + p * RB_PS + 4 * kh;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const float4 bq = *reinterpret_cast<const float4*>(a.b1 + 8 * q + 4 * kh);
+                    bf16x4 o;
+                    o[0] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 0] + bq.x, 0.f) : 0.f);
+                    o[1] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 1] + bq.y, 0.f) : 0.f);
+                    o[2] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 2] + bq.z, 0.f) : 0.f);
+                    o[3] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 3] + bq.w, 0.f) : 0.f);
+                    *reinterpret_cast<bf16x4*>(dst + 8 * q) = o;
+                }
+            }
+        }
+        __syncthreads();
+
+        // conv2 + bias + residual on the 8 x 32 tile: rows wave and wave + 4
+        f32x16 acc[2] = {(f32x16)0.f, (f32x16)0.f};
+        const int base2 = (wave_o * RB_MW + px) * RB_PS + 8 * kh;  // row wave, column px
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) {
+            const int tap = s >> 1;
+            const int off = ((tap / 3) * RB_MW + tap % 3) * RB_PS + 16 * (s & 1);
+            const bf16x8 W2 = w2_s[s * 64 + lane_o];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                // row wave + 4 j
+                const bf16x8 X = *reinterpret_cast<const bf16x8*>(mid_t + base2 + 4 * j * RB_MW * RB_PS + off);
+                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W2, X, acc[j], 0, 0, 0);
+            }
+        }
+        const int x = x0 + px;
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int r = wave_o + 4 * j;
+            const int y = y0 + r;
+            if (y >= a.H || x >= a.W) continue;
+            __bf16* dst = a.out + (img + (size_t)y * a.W + x) * 32 + 4 * kh;
+            const __bf16* res = in_t + ((r + 2) * RB_IW + px + 2) * RB_PS + 4 * kh;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const float4 bq = *reinterpret_cast<const float4*>(a.b2 + 8 * q + 4 * kh);
+                const bf16x4 rv = *reinterpret_cast<const bf16x4*>(res + 8 * q);
+                bf16x4 o;
+                o[0] = (__bf16)(acc[j][4 * q + 0] + bq.x + (float)rv[0]);
+                o[1] = (__bf16)(acc[j][4 * q + 1] + bq.y + (float)rv[1]);
+                o[2] = (__bf16)(acc[j][4 * q + 2] + bq.z + (float)rv[2]);
+                o[3] = (__bf16)(acc[j][4 * q + 3] + bq.w + (float)rv[3]);
+                *reinterpret_cast<bf16x4*>(dst + 8 * q) = o;
+            }
+        }
+    }
+}
+
 // ---- host helpers ------------------------------------------------------------------
 
-template <int CIN, int COUT>
-int conv_fwd(oph_ctx* ctx, const float* in, const bf16x8* wf, const float* bias, const float* res, float* out,
-             int B, int H, int W, int relu) {
-    ConvArgs a{in, wf, bias, res, out, B, H, W, relu};
+template <int CIN, int COUT, typename TI = float, typename TO = float>
+int conv_fwd(oph_ctx* ctx, const TI* in, const bf16x8* wf, const float* bias,
+             const typename std::remove_reference<TO>::type* res, TO* out, int B, int H, int W, int relu) {
+    ConvArgsT<TI, TO> a{in, wf, bias, res, out, B, H, W, relu};
     const dim3 grid((unsigned)((W + CT_W - 1) / CT_W), (unsigned)((H + CT_H - 1) / CT_H), (unsigned)B);
     if (B > 65535 || grid.y > 65535) return oph_fail(ctx, OPH_EINVAL, "convolution batch or height too large");
-    OPH_LAUNCH(ctx, "conv3x3_fwd", (conv3x3_fwd<CIN, COUT>), grid, dim3(64 * CT_WAVES), 0, a);
+    OPH_LAUNCH(ctx, "conv3x3_fwd", (conv3x3_fwd<CIN, COUT, TI, TO>), grid, dim3(64 * CT_WAVES), 0, a);
     return OPH_OK;
 }
 
@@ -507,6 +690,7 @@ int pack(oph_ctx* ctx, oph_cnn* net, bool transposed) {
     }
     OPH_LAUNCH(ctx, "pack_weights", pack_weights, dim3((unsigned)((64 * ksteps(32) + 255) / 256), (unsigned)n),
                dim3(256), 0, net->params, transposed ? net->wb : net->wf, plan);
+    if (!transposed) net->wf_rev = net->params_rev;
     return OPH_OK;
 }
 
@@ -568,7 +752,7 @@ int forward(oph_ctx* ctx, oph_cnn* net, const float* x, int B, int H, int W, Act
     const Frags F{net->blocks};
     const bf16x8* WF = net->wf;
     int rc;
-    if ((rc = pack(ctx, net, false))) return rc;
+    if (net->wf_rev != net->params_rev && (rc = pack(ctx, net, false))) return rc;
     if ((rc = conv_fwd<2, 32>(ctx, x, WF + F.head(), P + L.head_b(), nullptr, A.h[0], B, H, W, 0))) return rc;
     for (int b = 0; b < net->blocks; ++b) {
         if ((rc = conv_fwd<32, 32>(ctx, A.h[b], WF + F.conv1(b), P + L.b1(b), nullptr, A.a[b + 1], B, H, W, 1)))
@@ -577,6 +761,30 @@ int forward(oph_ctx* ctx, oph_cnn* net, const float* x, int B, int H, int W, Act
             return rc;
     }
     return conv_fwd<32, 2>(ctx, A.h[net->blocks], WF + F.tail(), P + L.tail_b(), x, A.y, B, H, W, 0);
+}
+
+// Inference forward with BF16 activations between layers: head conv, the
+// fused residual blocks (ReLU outputs stay in LDS), tail conv; x and y stay
+// float32 and y is written straight to `y`.
+int forward_bf16(oph_ctx* ctx, oph_cnn* net, const float* x, float* y, int B, int H, int W, Acts& A) {
+    const Layout L{net->ch, net->blocks};
+    const float* P = net->params;
+    const Frags F{net->blocks};
+    const bf16x8* WF = net->wf;
+    int rc;
+    if (net->wf_rev != net->params_rev && (rc = pack(ctx, net, false))) return rc;
+    auto h = [&](int b) { return reinterpret_cast<__bf16*>(A.h[b]); };
+    if ((rc = conv_fwd<2, 32, float, __bf16>(ctx, x, WF + F.head(), P + L.head_b(), nullptr, h(0), B, H, W, 0)))
+        return rc;
+    // persistent: two workgroups per CU (LDS-bound), each walking tiles
+    const int64_t ntiles = (int64_t)((W + RB_TW - 1) / RB_TW) * ((H + RB_TH - 1) / RB_TH) * B;
+    if (ntiles > INT32_MAX) return oph_fail(ctx, OPH_EINVAL, "too many pixels");
+    const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 2LL * std::max(ctx->cu_count, 1));
+    for (int b = 0; b < net->blocks; ++b) {
+        const ResArgs r{h(b), h(b + 1), WF + F.conv1(b), WF + F.conv2(b), P + L.b1(b), P + L.b2(b), B, H, W};
+        OPH_LAUNCH(ctx, "resblock_fwd", resblock_fwd, dim3(grid), dim3(256), 0, r);
+    }
+    return conv_fwd<32, 2, __bf16, float>(ctx, h(net->blocks), WF + F.tail(), P + L.tail_b(), x, y, B, H, W, 0);
 }
 
 // gradients of loss = mean (y - t)^2 into net->grads; loss into *loss
@@ -703,6 +911,7 @@ int oph_cnn_set_params(oph_ctx* ctx, oph_cnn* n, const float* host) {
     if (!n || !host) return oph_fail(ctx, OPH_EINVAL, "null argument");
     OPH_HIP_CHECK(ctx, hipMemcpyAsync(n->params, host, (size_t)n->nparams * 4, hipMemcpyHostToDevice, ctx->stream));
     OPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
+    n->params_rev += 1;
     n->step = 0;
     OPH_HIP_CHECK(ctx, hipMemsetAsync(n->m, 0, (size_t)n->nparams * 4, ctx->stream));
     OPH_HIP_CHECK(ctx, hipMemsetAsync(n->v, 0, (size_t)n->nparams * 4, ctx->stream));
@@ -727,8 +936,12 @@ int oph_cnn_forward(oph_ctx* ctx, oph_cnn* n, const float* x, float* y, int B, i
         dx = s;
         dyv = s + npix * 2;
     }
-    if ((rc = forward(ctx, n, dx, B, H, W, A))) return rc;
-    OPH_HIP_CHECK(ctx, hipMemcpyAsync(dyv, A.y, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    if (flags & OPH_CNN_BF16_ACTS) {
+        if ((rc = forward_bf16(ctx, n, dx, dyv, B, H, W, A))) return rc;
+    } else {
+        if ((rc = forward(ctx, n, dx, B, H, W, A))) return rc;
+        OPH_HIP_CHECK(ctx, hipMemcpyAsync(dyv, A.y, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    }
     if (!(flags & OPH_DEVICE_PTR))
         OPH_HIP_CHECK(ctx, hipMemcpyAsync(y, dyv, bytes, hipMemcpyDeviceToHost, ctx->stream));
     return oph_finish(ctx, flags);
@@ -757,6 +970,7 @@ static int train_or_grad(oph_ctx* ctx, oph_cnn* n, const float* x, const float* 
     if ((rc = forward(ctx, n, dx, B, H, W, A))) return rc;
     if ((rc = backward(ctx, n, dx, dt, B, H, W, A, loss))) return rc;
     if (update) {
+        n->params_rev += 1;
         n->step += 1;
         const float b1 = 0.9f, b2 = 0.999f;
         const float c1 = 1.f - powf(b1, (float)n->step), c2 = 1.f - powf(b2, (float)n->step);

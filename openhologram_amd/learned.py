@@ -114,7 +114,7 @@ def train(ctx, net, xs, ts, scale, steps=300, batch=8, patch=128, lr=2e-3, seed=
 
 def correct(ctx, net, coarse, scale):
     """CNN correction of one coarse field (H, W) complex -> complex64"""
-    y = net.forward(to_channels(coarse[None], scale))
+    y = net.forward(to_channels(coarse[None], scale), bf16_acts=True)
     return from_channels(y[0], scale)
 
 

@@ -15,6 +15,7 @@ HEADER = os.path.join(os.path.dirname(HERE), "include", "oph_hip.h")
 
 OPH_OK, OPH_EINVAL, OPH_ENOMEM, OPH_EDEVICE, OPH_EIO, OPH_ENUMERIC, OPH_ENOTSUP = 0, -1, -2, -3, -4, -5, -6
 DEVICE_PTR, ACCUMULATE, ASYNC = 0x1, 0x2, 0x4
+CNN_BF16_ACTS = 0x1000
 OUT_C128, OUT_C64, OUT_FRINGE = 0x00, 0x10, 0x20
 
 _ERRNAMES = {-1: "OPH_EINVAL", -2: "OPH_ENOMEM", -3: "OPH_EDEVICE", -4: "OPH_EIO",
@@ -392,16 +393,17 @@ class Cnn:
         assert p.size == self.n
         self.ctx.check(self.ctx.lib.oph_cnn_set_params(self.ctx.h, self.h, ptr(p)))
 
-    def forward(self, x, out=None, device=False, shape=None):
+    def forward(self, x, out=None, device=False, shape=None, bf16_acts=False):
         """y = net(x); x (B, H, W, 2) float32 (a complex64 (B, H, W) field
-        is the same memory).  `shape` overrides x.shape for device arrays."""
+        is the same memory).  `shape` overrides x.shape for device arrays.
+        bf16_acts: BF16 activations between layers (inference form)."""
         B, H, W, _ = shape or x.shape
         if not device:
             x = np.ascontiguousarray(x, np.float32)
         if out is None:
             out = self.ctx.empty((B, H, W, 2), np.float32) if device else np.zeros(x.shape, np.float32)
-        self.ctx.check(self.ctx.lib.oph_cnn_forward(self.ctx.h, self.h, ptr(x), ptr(out), B, H, W,
-                                                    DEVICE_PTR if device else 0))
+        flags = (DEVICE_PTR if device else 0) | (CNN_BF16_ACTS if bf16_acts else 0)
+        self.ctx.check(self.ctx.lib.oph_cnn_forward(self.ctx.h, self.h, ptr(x), ptr(out), B, H, W, flags))
         return out
 
     def gradients(self, x, t, device=False):

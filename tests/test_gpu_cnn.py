@@ -43,6 +43,29 @@ def test_cnn_forward_and_gradients_vs_oracle(ctx):
         assert _rel(g[o:o + n], gr[o:o + n]) < 0.1, name
 
 
+def test_cnn_forward_bf16_activations(ctx):
+    """Inference form with BF16 activations between layers: against the FP64
+    oracle with the float32 form's bound, and close to the float32 form."""
+    print("case: cnn forward with bf16 activations", flush=True)
+    rng = np.random.default_rng(4)
+    net = oph.lib.Cnn(ctx, blocks=2, channels=32, seed=3)
+    p = rng.normal(0, 0.05, net.n).astype(np.float32)
+    net.set_params(p)
+    x = rng.normal(0, 1, (2, 20, 36, 2)).astype(np.float32)
+    y32 = net.forward(x)
+    y16 = net.forward(x, bf16_acts=True)
+    yr = R.forward(p.astype(np.float64), x.astype(np.float64), 32, 2)
+    e_or = _rel(y16 - x, yr - x)
+    e_32 = _rel(y16 - x, y32 - x)
+    print("bf16-act forward rel vs oracle", e_or, "vs float32 form", e_32)
+    assert e_or < 2e-2 and e_32 < 1e-2
+    # the packed weights follow parameter updates
+    net.set_params(p * 0.5)
+    yh = net.forward(x, bf16_acts=True)
+    yrh = R.forward(p.astype(np.float64) * 0.5, x.astype(np.float64), 32, 2)
+    assert _rel(yh - x, yrh - x) < 2e-2
+
+
 def test_cnn_training_reduces_loss_and_persists(ctx, tmp_path):
     print("case: cnn training", flush=True)
     rng = np.random.default_rng(2)
