@@ -221,33 +221,63 @@ __global__ __launch_bounds__(64 * CT_WAVES) void conv3x3_fwd(ConvArgsT<TI, TO> a
                     A[e] = tap < 9 ? tile[((r + tap / 3) * TWH + col + tap % 3) * PS + c] : (__bf16)0.f;
                 }
             }
-            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, Bf[s], acc[h], 0, 0, 0);
+            acc[h] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bf[s], A, acc[h], 0, 0, 0);
         }
     }
-    if (n >= COUT) return;
-    const float bias = a.bias ? a.bias[n] : 0.f;
+    // epilogue: the weights are the MFMA A operand, so register v of lane
+    // (col, kh) is output channel 8 (v/4) + 4 kh + v%4 of pixel col: four
+    // runs of four consecutive channels per lane (vector stores, one bounds
+    // check per pixel)
+    constexpr int NV = COUT < 4 ? COUT : 4;  // channels per run (COUT is 2 or a multiple of 4)
+    static_assert(COUT < 4 || COUT % 4 == 0, "output channels");
+    const int x = x0 + col;
+    if (x >= a.W) return;
     // residuals first, all in flight together (out may not be assumed
     // distinct from res by the compiler, which would otherwise serialise
     // each load behind the previous store)
-    float rv[2][16];
+    float rv[2][4][4];
 #pragma unroll
     for (int h = 0; h < 2; ++h)
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int y = y0 + 2 * wave + h, x = x0 + 8 * (v >> 2) + 4 * kh + (v & 3);
-            rv[h][v] = (a.res && y < a.H && x < a.W) ? (float)a.res[(img + (size_t)y * a.W + x) * COUT + n] : 0.f;
+        for (int q = 0; q < 4; ++q) {
+            const int ch0 = 8 * q + 4 * kh, y = y0 + 2 * wave + h;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) rv[h][q][i] = 0.f;
+            if (!a.res || y >= a.H || ch0 >= COUT) continue;
+            const TO* r = a.res + (img + (size_t)y * a.W + x) * COUT + ch0;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) rv[h][q][i] = (float)r[i];
         }
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int y = y0 + 2 * wave + h;
-        if (y >= a.H) continue;
+    for (int q = 0; q < 4; ++q) {
+        const int ch0 = 8 * q + 4 * kh;
+        if (ch0 >= COUT) continue;
+        float bias[4] = {0.f, 0.f, 0.f, 0.f};
+        if (a.bias) {
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int x = x0 + 8 * (v >> 2) + 4 * kh + (v & 3);
-            if (x >= a.W) continue;
-            float o = acc[h][v] + bias;
-            if (a.relu) o = fmaxf(o, 0.f);
-            a.out[(img + (size_t)y * a.W + x) * COUT + n] = (TO)(o + rv[h][v]);
+            for (int i = 0; i < NV; ++i) bias[i] = a.bias[ch0 + i];
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int y = y0 + 2 * wave + h;
+            if (y >= a.H) continue;
+            TO o[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                float v = acc[h][4 * q + i] + bias[i];
+                if (a.relu) v = fmaxf(v, 0.f);
+                o[i] = (TO)(v + rv[h][q][i]);
+            }
+            TO* dst = a.out + (img + (size_t)y * a.W + x) * COUT + ch0;
+            if constexpr (NV == 4) {
+                if constexpr (std::is_same<TO, float>::value)
+                    *reinterpret_cast<float4*>(dst) = make_float4(o[0], o[1], o[2], o[3]);
+                else
+                    *reinterpret_cast<bf16x4*>(dst) = bf16x4{o[0], o[1], o[2], o[3]};
+            } else {
+#pragma unroll
+                for (int i = 0; i < NV; ++i) dst[i] = o[i];
+            }
         }
     }
 }
@@ -482,6 +512,7 @@ constexpr int RB_MH = RB_TH + 2, RB_MW = RB_TW + 2;   // conv1 output
 constexpr int RB_MID = RB_MH * RB_MW;                 // 340 pixels
 constexpr int RB_GROUPS = (RB_MID + 31) / 32;         // 11
 constexpr int RB_PS = 32 + 8;                         // LDS pixel stride (bf16)
+constexpr int RB_AHEAD = 2;                           // k-steps of LDS operands read ahead
 
 struct ResArgs {
     const __bf16* h;      // [B][H][W][32]
@@ -547,16 +578,29 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
                 base[j] = (mr * RB_IW + (p - mr * RB_MW)) * RB_PS + 8 * kh;
             }
             // k = 16 s + 8 kh: tap s / 2, channels 16 (s % 2) + 8 kh, so the
-            // per-step offset is a compile-time constant
+            // per-step offset is a compile-time constant.  Operands are read
+            // RB_AHEAD steps ahead of their MFMAs (LDS latency ~ 3 MFMAs).
+            bf16x8 X[RB_AHEAD + 1][GPW];
 #pragma unroll
-            for (int s = 0; s < KSTEPS; ++s) {
-                const int tap = s >> 1;
-                const int off = ((tap / 3) * RB_IW + tap % 3) * RB_PS + 16 * (s & 1);
+            for (int s = 0; s < KSTEPS + RB_AHEAD; ++s) {
+                if (s < KSTEPS) {
+                    const int tap = s >> 1;
+                    const int off = ((tap / 3) * RB_IW + tap % 3) * RB_PS + 16 * (s & 1);
 #pragma unroll
-                for (int j = 0; j < GPW; ++j) {
-                    const bf16x8 X = *reinterpret_cast<const bf16x8*>(in_t + base[j] + off);
-                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bf[s], X, acc[j], 0, 0, 0);
+                    for (int j = 0; j < GPW; ++j)
+                        X[s % (RB_AHEAD + 1)][j] = *reinterpret_cast<const bf16x8*>(in_t + base[j] + off);
                 }
+                const int u = s - RB_AHEAD;
+                if (u >= 0) {
+#pragma unroll
+                    for (int j = 0; j < GPW; ++j)
+                        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bf[u], X[u % (RB_AHEAD + 1)][j], acc[j], 0,
+                                                                         0, 0);
+                }
+                // keep the schedule: this step's reads, then the MFMAs of
+                // the step RB_AHEAD behind (masks: 0x100 LDS read, 0x8 MFMA)
+                if (s < KSTEPS) __builtin_amdgcn_sched_group_barrier(0x100, GPW, 0);
+                if (u >= 0) __builtin_amdgcn_sched_group_barrier(0x008, GPW, 0);
             }
 #pragma unroll
             for (int j = 0; j < GPW; ++j) {
@@ -583,17 +627,27 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
         // conv2 + bias + residual on the 8 x 32 tile: rows wave and wave + 4
         f32x16 acc[2] = {(f32x16)0.f, (f32x16)0.f};
         const int base2 = (wave_o * RB_MW + px) * RB_PS + 8 * kh;  // row wave, column px
+        bf16x8 X2[RB_AHEAD + 1][2], W2[RB_AHEAD + 1];
 #pragma unroll
-        for (int s = 0; s < KSTEPS; ++s) {
-            const int tap = s >> 1;
-            const int off = ((tap / 3) * RB_MW + tap % 3) * RB_PS + 16 * (s & 1);
-            const bf16x8 W2 = w2_s[s * 64 + lane_o];
+        for (int s = 0; s < KSTEPS + RB_AHEAD; ++s) {
+            if (s < KSTEPS) {
+                const int tap = s >> 1;
+                const int off = ((tap / 3) * RB_MW + tap % 3) * RB_PS + 16 * (s & 1);
+                W2[s % (RB_AHEAD + 1)] = w2_s[s * 64 + lane_o];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                // row wave + 4 j
-                const bf16x8 X = *reinterpret_cast<const bf16x8*>(mid_t + base2 + 4 * j * RB_MW * RB_PS + off);
-                acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W2, X, acc[j], 0, 0, 0);
+                for (int j = 0; j < 2; ++j)  // rows wave + 4 j
+                    X2[s % (RB_AHEAD + 1)][j] =
+                        *reinterpret_cast<const bf16x8*>(mid_t + base2 + 4 * j * RB_MW * RB_PS + off);
             }
+            const int u = s - RB_AHEAD;
+            if (u >= 0) {
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W2[u % (RB_AHEAD + 1)], X2[u % (RB_AHEAD + 1)][j],
+                                                                     acc[j], 0, 0, 0);
+            }
+            if (s < KSTEPS) __builtin_amdgcn_sched_group_barrier(0x100, 3, 1);
+            if (u >= 0) __builtin_amdgcn_sched_group_barrier(0x008, 2, 1);
         }
         const int x = x0 + px;
 #pragma unroll
