@@ -23,10 +23,10 @@
 //                      list; a window value depends on (|wx|, |wy|) only, so
 //                      each is evaluated once and added to its mirrored
 //                      pixels; contributions are summed in LDS as 32-bit
-//                      fixed point scaled per tile so the sum cannot
-//                      overflow (integer adds are associative, so the result
-//                      does not depend on the list order: deterministic
-//                      without sorting).
+//                      fixed point scaled per list segment so the sum
+//                      cannot overflow (integer adds are associative, so the
+//                      result does not depend on the list order:
+//                      deterministic without sorting).
 // Per pixel, with u = (wx^2 + wy^2) p^2 / dz^2 (<= 2 tan^2 ~ 2.2e-3 for the
 // grating-angle window):
 //   (r - |dz|)/|dz| = P(u) = u/2 - u^2/8 + u^3/16 - 5u^4/128,
@@ -304,143 +304,157 @@ constexpr int REC_BATCH = 128;
 // The window value depends on (|wx|, |wy|) only: each (ax, ay) is evaluated
 // once and added to the up-to-four mirrored pixels (+-ax, +-ay) that lie in
 // this tile's part of the window.  Sums are 32-bit fixed point with a scale
-// chosen per tile so that (list length) x (largest contribution) fits in
-// 2^30: exact integer adds, independent of the list order.  Both parts of a
-// pixel share one 64-bit word, 2^32 im + re, added with one 64-bit LDS
-// atomic: the word is exact modulo 2^64, and with |sum re|, |sum im| < 2^31
-// re is its sign-extended low half and im = (word - re) >> 32.
+// chosen so that (points summed) x (largest contribution) fits in 2^30:
+// exact integer adds, independent of the list order.  Both parts of a pixel
+// share one 64-bit word, 2^32 im + re, added with one 64-bit LDS atomic: the
+// word is exact modulo 2^64, and with |sum re|, |sum im| < 2^31 re is its
+// sign-extended low half and im = (word - re) >> 32.  A tile's list is
+// summed in segments of at most SEGMENT points (one segment for all but
+// dense tiles), each with its own scale, so the fixed-point unit stays below
+// 2^-19 of the largest contribution; segment sums are added to the tile's
+// output in segment order (deterministic).
+constexpr unsigned SEGMENT = 2048;
+
 __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
     __shared__ unsigned long long acc[TILE * TILE];
     __shared__ GRec recs[REC_BATCH];
     const int tile = blockIdx.x;
     const int tyi = tile / g.ntx, txi = tile % g.ntx;
     const int X0 = txi * TILE, Y0 = tyi * TILE;
-    for (int i = threadIdx.x; i < TILE * TILE; i += GATHER_THREADS) acc[i] = 0ull;
     unsigned b = g.offsets[tile], e = g.offsets[tile + 1];
     if (e > g.capacity) e = g.capacity;
     if (b > e) b = e;
     const float mx = *g.gmax;
-    const float bound = mx * (float)max(e - b, 1u);
-    const int S = bound > 0.f ? max(-60, min(60, 30 - (int)ceilf(log2f(bound)))) : 30;
-    const float scale = ldexpf(1.f, S);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     constexpr int NW = GATHER_THREADS / 64;
-    for (unsigned bs = b; bs < e; bs += REC_BATCH) {
-        const int cnt_b = (int)min((unsigned)REC_BATCH, e - bs);
-        __syncthreads();
-        if ((int)threadIdx.x < cnt_b) {
-            const unsigned idx = g.list[bs + threadIdx.x];
-            const WrpPoint q = g.pts[idx];
-            GRec r;
-            r.tx = (short)q.tx;
-            r.ty = (short)q.ty;
-            r.w = (short)q.w;
-            r.exact = (short)q.exact;
-            r.idx = (int)idx;
-            r.f0 = q.f0;
-            r.ks = q.ks;
-            r.p2dz2 = q.p2dz2;
-            r.ampd = q.ampd;
-            r.cd = q.cd;
-            recs[threadIdx.x] = r;
-        }
-        __syncthreads();
-        for (int kk = wave; kk < cnt_b; kk += NW) {
-            const GRec q = recs[kk];
-            const int tx = q.tx, ty = q.ty, w = q.w;
-            // window offsets of this tile's part: wx in [wx0, wx1), wy in [wy0, wy1)
-            const int wx0 = max(tx - w, max(X0, 0)) - tx, wx1 = min(tx + w, min(X0 + TILE, g.nx)) - tx;
-            const int wy0 = max(ty - w, max(Y0, 0)) - ty, wy1 = min(ty + w, min(Y0 + TILE, g.ny)) - ty;
-            if (wx1 <= wx0 || wy1 <= wy0) continue;
-            int ax0, ax1, ay0, ay1;
-            abs_range(wx0, wx1, ax0, ax1);
-            abs_range(wy0, wy1, ay0, ay1);
-            const int na = ax1 - ax0 + 1, nay = ay1 - ay0 + 1;
-            // lane positions qq = lane + 64 j walk the na x nay grid of
-            // (|wx|, |wy|) row by row; advancing 64 positions is sy rows plus
-            // sx columns, so no division or multiply per position
-            const float inv_na = 1.f / (float)na;
-            const int sy = (int)(64.5f * inv_na), sx = 64 - sy * na;
-            int iy = (int)(((float)lane + 0.5f) * inv_na);
-            int ix = lane - iy * na;
-            // byte address of the window centre in the accumulator
-            const int cbyte = ((ty - Y0) * TILE + (tx - X0)) * 8;
-            // the point's amplitude with the tile's fixed-point scale folded in
-            const float amps = q.ampd * scale;
-            // one loop per evaluation path (the path is uniform over the point)
-            auto walk = [&](auto exact_tag) {
-                constexpr bool EXACT = decltype(exact_tag)::value;
-                while (iy < nay) {
-                    const int ax = ax0 + ix, ay = ay0 + iy;
-                    float re, im;
-                    if constexpr (!EXACT) {
-                        const float fx = (float)ax, fy = (float)ay;
-                        const float u = fmaf(fx, fx, fy * fy) * q.p2dz2;
-                        const float p = u * fmaf(u, fmaf(u, fmaf(u, -5.f / 128.f, 1.f / 16.f), -0.125f), 0.5f);
-                        const float ph = fmaf(q.ks, p, q.f0);
-                        const float am = amps * fmaf(q.cd, p, 1.f);
-                        re = am * cos_turns(ph);
-                        im = am * sin_turns(ph);
-                    } else {
-                        const double dz = g.pts[q.idx].dz;
-                        const double dx = ax * g.pitch, dy = ay * g.pitch;
-                        const double sgn = dz > 0.0 ? 1.0 : -1.0;
-                        const double r = sgn * sqrt(dx * dx + dy * dy + dz * dz);
-                        double t = r * g.inv_lambda;
-                        t -= rint(t);
-                        // amplitude a/(r + 0.05) = (a/D) D/(r + 0.05), D = sign|dz| + 0.05
-                        const double D = sgn * fabs(dz) + 0.05;
-                        const float am = (float)((double)amps * D / (r + 0.05));
-                        re = am * cos_turns((float)t);
-                        im = am * sin_turns((float)t);
+    for (unsigned sb = b;; sb += SEGMENT) {
+        const unsigned se = min(e, sb + SEGMENT);
+        const float bound = mx * (float)max(se - sb, 1u);
+        const int S = bound > 0.f ? max(-60, min(60, 30 - (int)ceilf(log2f(bound)))) : 30;
+        const float scale = ldexpf(1.f, S);
+        __syncthreads();  // the previous segment's output reads are done
+        for (int i = threadIdx.x; i < TILE * TILE; i += GATHER_THREADS) acc[i] = 0ull;
+        for (unsigned bs = sb; bs < se; bs += REC_BATCH) {
+            const int cnt_b = (int)min((unsigned)REC_BATCH, se - bs);
+            __syncthreads();
+            if ((int)threadIdx.x < cnt_b) {
+                const unsigned idx = g.list[bs + threadIdx.x];
+                const WrpPoint q = g.pts[idx];
+                GRec r;
+                r.tx = (short)q.tx;
+                r.ty = (short)q.ty;
+                r.w = (short)q.w;
+                r.exact = (short)q.exact;
+                r.idx = (int)idx;
+                r.f0 = q.f0;
+                r.ks = q.ks;
+                r.p2dz2 = q.p2dz2;
+                r.ampd = q.ampd;
+                r.cd = q.cd;
+                recs[threadIdx.x] = r;
+            }
+            __syncthreads();
+            for (int kk = wave; kk < cnt_b; kk += NW) {
+                const GRec q = recs[kk];
+                const int tx = q.tx, ty = q.ty, w = q.w;
+                // window offsets of this tile's part: wx in [wx0, wx1), wy in [wy0, wy1)
+                const int wx0 = max(tx - w, max(X0, 0)) - tx, wx1 = min(tx + w, min(X0 + TILE, g.nx)) - tx;
+                const int wy0 = max(ty - w, max(Y0, 0)) - ty, wy1 = min(ty + w, min(Y0 + TILE, g.ny)) - ty;
+                if (wx1 <= wx0 || wy1 <= wy0) continue;
+                int ax0, ax1, ay0, ay1;
+                abs_range(wx0, wx1, ax0, ax1);
+                abs_range(wy0, wy1, ay0, ay1);
+                const int na = ax1 - ax0 + 1, nay = ay1 - ay0 + 1;
+                // lane positions qq = lane + 64 j walk the na x nay grid of
+                // (|wx|, |wy|) row by row; advancing 64 positions is sy rows
+                // plus sx columns, so no division or multiply per position
+                const float inv_na = 1.f / (float)na;
+                const int sy = (int)(64.5f * inv_na), sx = 64 - sy * na;
+                int iy = (int)(((float)lane + 0.5f) * inv_na);
+                int ix = lane - iy * na;
+                // byte address of the window centre in the accumulator
+                const int cbyte = ((ty - Y0) * TILE + (tx - X0)) * 8;
+                // the point's amplitude with the segment's fixed-point scale folded in
+                const float amps = q.ampd * scale;
+                // one loop per evaluation path (the path is uniform over the point)
+                auto walk = [&](auto exact_tag) {
+                    constexpr bool EXACT = decltype(exact_tag)::value;
+                    while (iy < nay) {
+                        const int ax = ax0 + ix, ay = ay0 + iy;
+                        float re, im;
+                        if constexpr (!EXACT) {
+                            const float fx = (float)ax, fy = (float)ay;
+                            const float u = fmaf(fx, fx, fy * fy) * q.p2dz2;
+                            const float p = u * fmaf(u, fmaf(u, fmaf(u, -5.f / 128.f, 1.f / 16.f), -0.125f), 0.5f);
+                            const float ph = fmaf(q.ks, p, q.f0);
+                            const float am = amps * fmaf(q.cd, p, 1.f);
+                            re = am * cos_turns(ph);
+                            im = am * sin_turns(ph);
+                        } else {
+                            const double dz = g.pts[q.idx].dz;
+                            const double dx = ax * g.pitch, dy = ay * g.pitch;
+                            const double sgn = dz > 0.0 ? 1.0 : -1.0;
+                            const double r = sgn * sqrt(dx * dx + dy * dy + dz * dz);
+                            double t = r * g.inv_lambda;
+                            t -= rint(t);
+                            // amplitude a/(r + 0.05) = (a/D) D/(r + 0.05), D = sign|dz| + 0.05
+                            const double D = sgn * fabs(dz) + 0.05;
+                            const float am = (float)((double)amps * D / (r + 0.05));
+                            re = am * cos_turns((float)t);
+                            im = am * sin_turns((float)t);
+                        }
+                        // fixed point, truncated toward zero (any fixed rounding
+                        // keeps the sum exact; the unit is 2^-30 of the bound)
+                        const int ire = (int)re;
+                        const int iim = (int)im;
+                        // 2^32 im + re modulo 2^64
+                        const unsigned long long pk =
+                            (unsigned long long)(long long)ire + ((unsigned long long)(long long)iim << 32);
+                        const bool xp = ax >= wx0 && ax < wx1, xm = ax > 0 && -ax >= wx0;
+                        const bool yp = ay >= wy0 && ay < wy1, ym = ay > 0 && -ay >= wy0;
+                        // byte offsets of the mirrored pixels (8 bytes per pixel)
+                        const int rp = cbyte + (ay << 9), rm = cbyte - (ay << 9), cx = ax << 3;
+                        char* ba = reinterpret_cast<char*>(acc);
+                        if (xp && yp) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rp + cx), pk);
+                        if (xm && yp) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rp - cx), pk);
+                        if (xp && ym) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rm + cx), pk);
+                        if (xm && ym) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rm - cx), pk);
+                        ix += sx;
+                        iy += sy;
+                        if (ix >= na) {
+                            ix -= na;
+                            ++iy;
+                        }
                     }
-                    // fixed point, truncated toward zero (any fixed rounding
-                    // keeps the sum exact; the unit is 2^-30 of the bound)
-                    const int ire = (int)re;
-                    const int iim = (int)im;
-                    // 2^32 im + re modulo 2^64
-                    const unsigned long long pk =
-                        (unsigned long long)(long long)ire + ((unsigned long long)(long long)iim << 32);
-                    const bool xp = ax >= wx0 && ax < wx1, xm = ax > 0 && -ax >= wx0;
-                    const bool yp = ay >= wy0 && ay < wy1, ym = ay > 0 && -ay >= wy0;
-                    // byte offsets of the mirrored pixels (8 bytes per pixel)
-                    const int rp = cbyte + (ay << 9), rm = cbyte - (ay << 9), cx = ax << 3;
-                    char* ba = reinterpret_cast<char*>(acc);
-                    if (xp && yp) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rp + cx), pk);
-                    if (xm && yp) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rp - cx), pk);
-                    if (xp && ym) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rm + cx), pk);
-                    if (xm && ym) atomicAdd(reinterpret_cast<unsigned long long*>(ba + rm - cx), pk);
-                    ix += sx;
-                    iy += sy;
-                    if (ix >= na) {
-                        ix -= na;
-                        ++iy;
-                    }
-                }
-            };
-            if (q.exact)
-                walk(std::true_type{});
-            else
-                walk(std::false_type{});
+                };
+                if (q.exact)
+                    walk(std::true_type{});
+                else
+                    walk(std::false_type{});
+            }
         }
-    }
-    __syncthreads();
-    const float inv = ldexpf(1.f, -S);
-    for (int p = threadIdx.x; p < TILE * TILE; p += GATHER_THREADS) {
-        const int row = Y0 + p / TILE, col = X0 + p % TILE;
-        if (row >= g.ny || col >= g.nx) continue;
-        const unsigned long long wv = acc[p];
-        const int sre = (int)(unsigned)wv;
-        const int sim = (int)((long long)(wv - (unsigned long long)(long long)sre) >> 32);
-        const float re = (float)sre * inv;
-        const float im = (float)sim * inv;
-        float2* o = g.wrp + (size_t)row * g.nx + col;
-        if (g.accumulate) {
-            const float2 v = *o;
-            *o = make_float2(re + v.x, im + v.y);
-        } else {
-            *o = make_float2(re, im);
+        __syncthreads();
+        // the segment's sums: written (first segment, unless accumulating)
+        // or added to the tile's output
+        const bool add = sb != b || g.accumulate;
+        const float inv = ldexpf(1.f, -S);
+        for (int p = threadIdx.x; p < TILE * TILE; p += GATHER_THREADS) {
+            const int row = Y0 + p / TILE, col = X0 + p % TILE;
+            if (row >= g.ny || col >= g.nx) continue;
+            const unsigned long long wv = acc[p];
+            const int sre = (int)(unsigned)wv;
+            const int sim = (int)((long long)(wv - (unsigned long long)(long long)sre) >> 32);
+            const float re = (float)sre * inv;
+            const float im = (float)sim * inv;
+            float2* o = g.wrp + (size_t)row * g.nx + col;
+            if (add) {
+                const float2 v = *o;
+                *o = make_float2(re + v.x, im + v.y);
+            } else {
+                *o = make_float2(re, im);
+            }
         }
+        if (se >= e) break;
     }
 }
 

@@ -61,6 +61,37 @@ def test_asm_pad1_and_errors(ctx):
         ctx.asm(u, 8e-6, 532e-9, 0.02, pad=3)
 
 
+@pytest.mark.parametrize("ny,nx,pad", [(32, 4096, 1), (16, 2048, 2)])
+def test_asm_4096_rows(ctx, ny, nx, pad):
+    """4096-point row transforms (first and last stages in registers): a
+    field filling the grid (pad 1) and the half-width data window (pad 2)."""
+    print(f"case: asm {ny}x{nx} pad {pad}", flush=True)
+    rng = np.random.default_rng(nx + pad)
+    u = (rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))).astype(np.complex64)
+    out = ctx.asm(u, 8e-6, 532e-9, 0.03, pad=pad)
+    ref = O.asm(u.astype(np.complex128), 8e-6, 8e-6, 532e-9, 0.03, pad=pad)
+    err = rel_l2(out, ref)
+    print("rel-L2", err)
+    assert err < 1e-5
+
+
+def test_wrp_many_points_and_linearity(ctx):
+    """300,000 points: binning chunks of two points per thread; against the
+    oracle, and the sum of two halves (accumulate) equals the whole."""
+    print("case: wrp 128x128 300k points", flush=True)
+    x, y, z, a, zw = scenes.wrp_points(300_000, seed=9)
+    x, y = x * 0.05, y * 0.05
+    out = ctx.wrp(128, 128, 8e-6, 532e-9, zw, x, y, z, a)
+    ref = O.wrp(128, 128, 8e-6, 532e-9, zw, x, y, z, a)
+    err = rel_l2(out, ref)
+    print("rel-L2", err)
+    assert err < 1e-5
+    h = 150_000
+    part = ctx.wrp(128, 128, 8e-6, 532e-9, zw, x[:h], y[:h], z[:h], a[:h])
+    both = ctx.wrp(128, 128, 8e-6, 532e-9, zw, x[h:], y[h:], z[h:], a[h:], out=part, accumulate=True)
+    assert rel_l2(both, out) < 1e-6
+
+
 @pytest.mark.parametrize("mode", [0, 1])
 def test_wrp_vs_oracle(ctx, mode):
     print(f"case: wrp 256x256 mode {mode}", flush=True)
