@@ -42,6 +42,7 @@ namespace {
 // threads, so they cost no launch of their own.
 struct TabJob {
     float4* t;
+    float2* t2;  // y axis again as (x, y) only: what the column pass reads without a band limit
     int nx, ny;
     double px, py, lambda, z;
 };
@@ -61,6 +62,7 @@ __device__ void fill_axis_tables(const TabJob& j, int first, int step) {
         tt -= rint(tt);
         const double bl = fabs(f) * j.z * j.lambda / (n * pitch * 0.5);
         j.t[k] = make_float4((float)tt, (float)(j.lambda * j.lambda * f * f), (float)(bl * bl), (float)f);
+        if (k >= j.nx) j.t2[kk] = make_float2((float)tt, (float)(j.lambda * j.lambda * f * f));
     }
 }
 
@@ -74,13 +76,16 @@ struct AsmConst {
 };
 
 // exp(i 2 pi (c0 + tx + ty - K poly(u))) or 0; EXACT: z/lambda sqrt(1-u) in FP64
+// (ay: the y entry's phase and lambda^2 f^2; its band-limit term is read
+// from the full table only when the band limit is on)
 template <bool EXACT>
-__device__ __forceinline__ float2 transfer(const float4 ax, const float4 ay, int c, int i, const AsmConst& k) {
+__device__ __forceinline__ float2 transfer(const float4 ax, const float2 ay, const float4* __restrict__ tyfull, int c,
+                                           int i, const AsmConst& k) {
     const float u = ax.y + ay.y;
     if (u >= 1.f) return make_float2(0.f, 0.f);
     if (k.band) {
         const float s = 1.f - u;
-        if (ax.z > s || ay.z > s) return make_float2(0.f, 0.f);
+        if (ax.z > s || tyfull[i].z > s) return make_float2(0.f, 0.f);
     }
     float ph;
     if (!EXACT) {
@@ -131,7 +136,8 @@ __global__ __launch_bounds__(512) void asm_rows_fwd(const float2* __restrict__ i
 template <int L, int B, bool EXACT>
 __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, int oy,
                                                  const float4* __restrict__ tabx,
-                                                 const float4* __restrict__ taby, AsmConst k,
+                                                 const float4* __restrict__ taby,
+                                                 const float2* __restrict__ taby2, AsmConst k,
                                                  const float2* __restrict__ tw) {
     constexpr int TPL = L / 16;
     constexpr int PL = fft::padded_len(L);
@@ -170,7 +176,7 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
 #pragma unroll
     for (int m = 0; m < 16; ++m) {
         const int i = t + m * TPL;
-        const float2 h = transfer<EXACT>(ax, taby[i], c, i, k);
+        const float2 h = transfer<EXACT>(ax, taby2[i], taby, c, i, k);
         float2* e = line + fft::lane_pad<L>(t, m);
         *e = fft::cmul(*e, h);
     }
@@ -461,8 +467,10 @@ int launch_cols_b(oph_ctx* ctx, float2* work, int npx, int ny, int oy, const flo
         hipGetLastError();
         attr = true;
     }
+    // the compact y table follows the two float4 tables (ty has LY entries)
+    const float2* ty2 = reinterpret_cast<const float2*>(ty + LY);
     OPH_LAUNCH(ctx, "asm_cols", (asm_cols<LY, B, EXACT>), dim3((npx + B - 1) / B), dim3(B * (LY / 16)), shm,
-               work, npx, ny, oy, tx, ty, k, tw);
+               work, npx, ny, oy, tx, ty, ty2, k, tw);
     return OPH_OK;
 }
 
@@ -505,9 +513,11 @@ int launch_cols(oph_ctx* ctx, bool exact, float2* work, int npx, int ny, int oy,
 int oph_asm_device(oph_ctx* ctx, float2* field, int nx, int ny, int npx, int npy, double px, double py,
                    double lambda, double z, int band_limit, float2* work) {
     const int ox = npx / 2 - nx / 2, oy = npy / 2 - ny / 2;
-    float4* tab = static_cast<float4*>(oph_scratch(ctx, SLOT_TABLE, sizeof(float4) * (npx + npy)));
+    // x and y tables (float4), then the y table again as float2
+    float4* tab = static_cast<float4*>(
+        oph_scratch(ctx, SLOT_TABLE, sizeof(float4) * (npx + npy) + sizeof(float2) * npy));
     if (!tab) return OPH_ENOMEM;
-    const TabJob tj{tab, npx, npy, px, py, lambda, z};
+    const TabJob tj{tab, reinterpret_cast<float2*>(tab + npx + npy), npx, npy, px, py, lambda, z};
     const double zl = z / lambda;
     AsmConst k;
     k.zl = zl;
