@@ -38,7 +38,11 @@
 //             transforms row ky along x once and obtains row Ny-ky from the
 //             same transform by symmetry, multiplies both by H and
 //             accumulates over all layers in registers; then the encode crop,
-//             the inverse row transforms, and the scene columns are stored;
+//             the inverse row transforms, and the scene columns are stored.
+//             Rows 0 and Ny/2 are real (transforms of real columns at the DC
+//             and Nyquist frequencies): workgroup 0 packs them into one
+//             complex row and separates the transform, so the pass is Ny/2
+//             workgroups (a multiple of the CU count) instead of Ny/2 + 1;
 //   dm_final  inverse column transform, 1/(Nx Ny), the shift phase and the
 //             real part (or the complex field) in the scene window.
 #include <cmath>
@@ -192,6 +196,14 @@ __global__ __launch_bounds__(2 * (LY / 16)) void dm_cols(DmArgs a, int layers_pe
 
 // ---- row pass ------------------------------------------------------------------
 
+// Z = FFT(a + i b) of two real rows a, b; on entry z = Z(k), zc = conj Z(-k).
+// On exit z = A(k) = (Z(k) + conj Z(-k))/2 and zc = B(k) = (Z(k) - conj Z(-k))/(2i).
+__device__ __forceinline__ void split_real_pair(float2& z, float2& zc) {
+    const float dx = z.x - zc.x, dy = z.y - zc.y;
+    z = make_float2(0.5f * (z.x + zc.x), 0.5f * (z.y + zc.y));
+    zc = make_float2(0.5f * dy, -0.5f * dx);
+}
+
 template <bool EXACT>
 __device__ __forceinline__ float2 dm_transfer(const DmArgs& a, float tx, double ty, float usx, int sy,
                                               double K, float Kf, double sx2d) {
@@ -245,11 +257,12 @@ __global__ __launch_bounds__(LX / 16) void dm_rows(DmArgs a, const float2* __res
     constexpr int PL = fft::padded_len(LX);
     __shared__ float2 line[PL];
     const int t = threadIdx.x;
-    const int kya = blockIdx.x;               // 0 .. Ny/2
-    const int kyb = a.Ny - kya;
-    const bool has_b = kya != 0 && kya != a.Ny / 2;
+    const int kya = blockIdx.x;               // 0 .. Ny/2 - 1
+    const bool pair0 = kya == 0;              // real rows 0 and Ny/2 packed as re/im
+    const int kyb = pair0 ? a.Ny / 2 : a.Ny - kya;
     const int sya = sgnidx(kya, a.Ny) + 1;
-    const int syb = sgnidx(kyb & (a.Ny - 1), a.Ny) + 1;
+    const int syb = sgnidx(kyb, a.Ny) + 1;
+    const size_t nyq = (size_t)(a.Ny / 2) * a.w;  // row Ny/2 relative to row 0
     fft::Twiddles<LX> Tw;
     Tw.load(twx, t);
     float2 acca[16], accb[16];
@@ -271,7 +284,10 @@ __global__ __launch_bounds__(LX / 16) void dm_rows(DmArgs a, const float2* __res
         for (int m = 0; m < 16; ++m) {
             const int x = t + m * TPL - a.ox;
             float2 v = make_float2(0.f, 0.f);
-            if (x >= 0 && x < a.w && fl[x / GROUP]) v = src[x];
+            if (x >= 0 && x < a.w && fl[x / GROUP]) {
+                v = src[x];
+                if (pair0) v.y = src[nyq + x].x;
+            }
             line[fft::lane_pad<LX>(t, m)] = v;
         }
         __syncthreads();
@@ -288,15 +304,14 @@ __global__ __launch_bounds__(LX / 16) void dm_rows(DmArgs a, const float2* __res
             double txd = cx * sx2[m];
             txd -= rint(txd);
             const float tx = (float)txd;
-            const float2 gv = line[fft::lane_pad<LX>(t, m)];
+            float2 gv = line[fft::lane_pad<LX>(t, m)];
+            const float2 gm = line[fft::pad16((LX - kx) & (LX - 1))];
+            float2 gmc = make_float2(gm.x, -gm.y);
+            if (pair0) split_real_pair(gv, gmc);
             const float2 ha = dm_transfer<EXACT>(a, tx, tya, usx[m], sya, K, Kf, sx2[m]);
             acca[m] = fft::cadd(acca[m], fft::cmul(ha, gv));
-            if (has_b) {
-                const float2 gm = line[fft::pad16((LX - kx) & (LX - 1))];
-                const float2 gmc = make_float2(gm.x, -gm.y);
-                const float2 hb = dm_transfer<EXACT>(a, tx, tyb, usx[m], syb, K, Kf, sx2[m]);
-                accb[m] = fft::cadd(accb[m], fft::cmul(hb, gmc));
-            }
+            const float2 hb = dm_transfer<EXACT>(a, tx, tyb, usx[m], syb, K, Kf, sx2[m]);
+            accb[m] = fft::cadd(accb[m], fft::cmul(hb, gmc));
         }
         __syncthreads();
     }
@@ -308,7 +323,7 @@ __global__ __launch_bounds__(LX / 16) void dm_rows(DmArgs a, const float2* __res
         accb[m] = fft::cmul(accb[m], a.rho);
     }
     finish_row<LX>(a, acca, kya, line, twx, t);
-    if (has_b) finish_row<LX>(a, accb, kyb, line, twx, t);
+    finish_row<LX>(a, accb, kyb, line, twx, t);
 }
 
 // Row pass for equally spaced layers (no field lens).  The transfer of
@@ -321,9 +336,14 @@ __global__ __launch_bounds__(LX / 16) void dm_rows(DmArgs a, const float2* __res
 // recurrence's rounding stays within ~16 ulp).  The row transforms are
 // radix-8 with LX/8 threads (8 elements per thread) so that R and T of both
 // rows fit in registers next to the transform; the chunk sums S accumulate
-// in LDS, each element touched only by its owner thread.
+// in LDS, each element touched only by its owner thread.  The twiddle table
+// and a bitmask of the present layers are copied to LDS at the start, so
+// that inside the layer loop only the prefetched rows are global loads.
 template <int LX>
-__host__ __device__ constexpr size_t rows_h_lds() { return (fft::padded_len(LX) + 2 * (size_t)LX) * sizeof(float2); }
+__host__ __device__ constexpr size_t rows_h_lds() {
+    return (fft::padded_len(LX) + 2 * (size_t)LX + fft::table_len8(LX)) * sizeof(float2) +
+           (size_t)(MAX_LAYERS / 32) * sizeof(unsigned);
+}
 
 // finish_row for the radix-8 layout (8 elements per thread)
 template <int LX>
@@ -351,21 +371,35 @@ __device__ __forceinline__ void finish_row8(const DmArgs& a, const float2 (&acc)
     __syncthreads();
 }
 
-template <int LX>
-__global__ __launch_bounds__(LX / 8) void dm_rows_h(DmArgs a, const float2* __restrict__ tw8) {
+// PAIR0: workgroup 0, the real rows 0 and Ny/2 packed as re/im.
+template <int LX, bool PAIR0>
+__device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __restrict__ tw8, float2* sm) {
     constexpr int TPL = LX / 8;
     constexpr int PL = fft::padded_len(LX);
     constexpr int CH = 16;
-    extern __shared__ float2 sm[];
+    constexpr int NTW = fft::table_len8(LX);
     float2* line = sm;
     float2* Sa = sm + PL;
     float2* Sb = Sa + LX;
+    float2* tws = Sb + LX;
+    unsigned* amask = reinterpret_cast<unsigned*>(tws + NTW);
     const int t = threadIdx.x;
-    const int kya = blockIdx.x;
-    const int kyb = a.Ny - kya;
-    const bool has_b = kya != 0 && kya != a.Ny / 2;
+    for (int i = t; i < NTW; i += TPL) tws[i] = tw8[i];
+    for (int i = t; i < MAX_LAYERS / 32; i += TPL) amask[i] = 0u;
+    __syncthreads();
+    for (int i = t; i < a.L; i += TPL)
+        if (a.any[i]) atomicOr(&amask[i >> 5], 1u << (i & 31));
+    __syncthreads();
+    auto present = [&](int l) -> bool { return (amask[(l - 1) >> 5] >> ((l - 1) & 31)) & 1u; };
+    const int kya = PAIR0 ? 0 : (int)blockIdx.x;  // 0 .. Ny/2 - 1
+    const int kyb = PAIR0 ? a.Ny / 2 : a.Ny - kya;
     const int sya = sgnidx(kya, a.Ny) + 1;
-    const int syb = sgnidx(kyb & (a.Ny - 1), a.Ny) + 1;
+    const int syb = sgnidx(kyb, a.Ny) + 1;
+    const size_t nyq = (size_t)(a.Ny / 2) * a.w;
+    // Thread t owns row a at kx = t + m TPL and row b at kb = -kx mod LX:
+    // row b's transform at kb is conj Z(kx) of the thread's own element of
+    // row a's transform Z, and the transfer depends on kx only through kx^2,
+    // which is the same at kb, so one set of per-element factors serves both.
     float2 Ra[8], Rb[8];
     {
         double tya = a.dcy * (double)sya * sya;
@@ -383,14 +417,44 @@ __global__ __launch_bounds__(LX / 8) void dm_rows_h(DmArgs a, const float2* __re
             const float usx = a.ux * (float)(sx * sx);
             Ra[m] = dm_transfer<false>(a, (float)txd, tya, usx, sya, a.dK, dKf, sx2);
             Rb[m] = dm_transfer<false>(a, (float)txd, tyb, usx, syb, a.dK, dKf, sx2);
-            Sa[kx] = Sb[kx] = make_float2(0.f, 0.f);
+            Sa[kx] = Sb[(LX - kx) & (LX - 1)] = make_float2(0.f, 0.f);
         }
     }
     const size_t lstride = (size_t)(a.Ny / 2 + 1) * a.w;
+    // Layers are taken chunk by chunk, top layer first within a chunk.  The
+    // row of the next present layer in that order is loaded into registers
+    // (values and presence flags) before the current layer's transform, so
+    // its latency hides behind the transform.
+    auto next_layer = [&](int l) -> int {
+        for (;;) {
+            const int c = a.lo + (l - a.lo) / CH * CH;
+            l = l > c ? l - 1 : (c + CH <= a.hi ? min(a.hi, c + 2 * CH - 1) : 0);
+            if (l == 0 || present(l)) return l;
+        }
+    };
+    float2 nv[8];
+    unsigned nf[8];
+    auto fetch = [&](int l) {
+        const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * a.w;
+        const unsigned char* fl = a.flags + (size_t)(l - 1) * a.ngroups;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const int x = t + m * TPL - a.ox;
+            const bool in = x >= 0 && x < a.w;
+            nf[m] = in ? fl[x / GROUP] : 0u;
+            if constexpr (PAIR0)
+                nv[m] = in ? make_float2(src[x].x, src[nyq + x].x) : make_float2(0.f, 0.f);
+            else
+                nv[m] = in ? src[x] : make_float2(0.f, 0.f);
+        }
+    };
+    int pl = min(a.hi, a.lo + CH - 1);
+    if (!present(pl)) pl = next_layer(pl);
+    if (pl) fetch(pl);
     for (int cs = a.lo; cs <= a.hi; cs += CH) {
         const int ce = min(a.hi, cs + CH - 1);
         bool anyc = false;
-        for (int l = cs; l <= ce; ++l) anyc |= a.any[l - 1] != 0;
+        for (int l = cs; l <= ce; ++l) anyc |= present(l);
         if (!anyc) continue;
         float2 Ta[8], Tb[8];
 #pragma unroll
@@ -401,27 +465,38 @@ __global__ __launch_bounds__(LX / 8) void dm_rows_h(DmArgs a, const float2* __re
                 Ta[m] = fft::cmul(Ta[m], Ra[m]);
                 Tb[m] = fft::cmul(Tb[m], Rb[m]);
             }
-            if (!a.any[l - 1]) continue;
-            const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * a.w;
-            const unsigned char* fl = a.flags + (size_t)(l - 1) * a.ngroups;
+            if (!present(l)) continue;
+            // l is the prefetched layer
+            float2 v[8];
 #pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                const int x = t + m * TPL - a.ox;
-                const bool in = x >= 0 && x < a.w;
-                const float2 v = in ? src[x] : make_float2(0.f, 0.f);
-                const bool present = in && fl[x / GROUP];
-                line[fft::pad16(t + m * TPL)] = present ? v : make_float2(0.f, 0.f);
-            }
-            __syncthreads();
-            fft::line_r8<LX, -1>(line, tw8, t);
+            for (int m = 0; m < 8; ++m) v[m] = nf[m] ? nv[m] : make_float2(0.f, 0.f);
+            pl = next_layer(l);
+            if (pl) fetch(pl);
+            __syncthreads();  // the previous transform's readers are done with the line
+            fft::line_r8_regs<LX, -1>(v, line, tws, t);
+            if constexpr (PAIR0) {
+                // A(kx) = (Z(kx) + conj Z(-kx))/2 for row 0 and, row Ny/2
+                // being real, B(-kx) = conj B(kx): exchange through the line
+                __syncthreads();
 #pragma unroll
-            for (int m = 0; m < 8; ++m) {
-                const int kx = t + m * TPL;
-                Ta[m] = fft::cadd(Ta[m], line[fft::pad16(kx)]);
-                const float2 gm = line[fft::pad16((LX - kx) & (LX - 1))];
-                Tb[m] = fft::cadd(Tb[m], make_float2(gm.x, -gm.y));
+                for (int m = 0; m < 8; ++m) line[fft::pad16(t + m * TPL)] = v[m];
+                __syncthreads();
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    const int kx = t + m * TPL;
+                    const float2 gm = line[fft::pad16((LX - kx) & (LX - 1))];
+                    float2 z = v[m], zc = make_float2(gm.x, -gm.y);
+                    split_real_pair(z, zc);
+                    Ta[m] = fft::cadd(Ta[m], z);
+                    Tb[m] = fft::cadd(Tb[m], make_float2(zc.x, -zc.y));
+                }
+            } else {
+#pragma unroll
+                for (int m = 0; m < 8; ++m) {
+                    Ta[m] = fft::cadd(Ta[m], v[m]);
+                    Tb[m] = fft::cadd(Tb[m], make_float2(v[m].x, -v[m].y));
+                }
             }
-            __syncthreads();
         }
         // anchor: S += H_cs T
         const double cx = a.lc[3 * (cs - 1)], cy = a.lc[3 * (cs - 1) + 1], K = a.lc[3 * (cs - 1) + 2];
@@ -440,12 +515,12 @@ __global__ __launch_bounds__(LX / 8) void dm_rows_h(DmArgs a, const float2* __re
             const float usx = a.ux * (float)(sx * sx);
             const float2 ha = dm_transfer<false>(a, (float)txd, tya, usx, sya, K, Kf, sx2);
             Sa[kx] = fft::cadd(Sa[kx], fft::cmul(ha, Ta[m]));
-            if (has_b) {
-                const float2 hb = dm_transfer<false>(a, (float)txd, tyb, usx, syb, K, Kf, sx2);
-                Sb[kx] = fft::cadd(Sb[kx], fft::cmul(hb, Tb[m]));
-            }
+            const float2 hb = dm_transfer<false>(a, (float)txd, tyb, usx, syb, K, Kf, sx2);
+            const int kb = (LX - kx) & (LX - 1);
+            Sb[kb] = fft::cadd(Sb[kb], fft::cmul(hb, Tb[m]));
         }
     }
+    __syncthreads();  // Sb entries were written by the threads owning -kx
     float2 acca[8], accb[8];
 #pragma unroll
     for (int m = 0; m < 8; ++m) {
@@ -453,9 +528,17 @@ __global__ __launch_bounds__(LX / 8) void dm_rows_h(DmArgs a, const float2* __re
         acca[m] = fft::cmul(Sa[kx], a.rho);
         accb[m] = fft::cmul(Sb[kx], a.rho);
     }
-    __syncthreads();
-    finish_row8<LX>(a, acca, kya, line, tw8, t);
-    if (has_b) finish_row8<LX>(a, accb, kyb, line, tw8, t);
+    finish_row8<LX>(a, acca, kya, line, tws, t);
+    finish_row8<LX>(a, accb, kyb, line, tws, t);
+}
+
+template <int LX>
+__global__ __launch_bounds__(LX / 8) void dm_rows_h(DmArgs a, const float2* __restrict__ tw8) {
+    extern __shared__ float2 sm[];
+    if (blockIdx.x == 0)
+        rows_h_body<LX, true>(a, tw8, sm);
+    else
+        rows_h_body<LX, false>(a, tw8, sm);
 }
 
 // ---- final column pass ---------------------------------------------------------
@@ -537,14 +620,14 @@ int launch_dm_rows(oph_ctx* ctx, const DmArgs& a) {
                 hipGetLastError();
                 attr = true;
             }
-            OPH_LAUNCH(ctx, "dm_rows", dm_rows_h<LX>, dim3(a.Ny / 2 + 1), dim3(LX / 8), rows_h_lds<LX>(), a, tw8);
+            OPH_LAUNCH(ctx, "dm_rows", dm_rows_h<LX>, dim3(a.Ny / 2), dim3(LX / 8), rows_h_lds<LX>(), a, tw8);
             return OPH_OK;
         }
     }
     if (a.exact)
-        OPH_LAUNCH(ctx, "dm_rows", (dm_rows<LX, true>), dim3(a.Ny / 2 + 1), dim3(LX / 16), 0, a, tw);
+        OPH_LAUNCH(ctx, "dm_rows", (dm_rows<LX, true>), dim3(a.Ny / 2), dim3(LX / 16), 0, a, tw);
     else
-        OPH_LAUNCH(ctx, "dm_rows", (dm_rows<LX, false>), dim3(a.Ny / 2 + 1), dim3(LX / 16), 0, a, tw);
+        OPH_LAUNCH(ctx, "dm_rows", (dm_rows<LX, false>), dim3(a.Ny / 2), dim3(LX / 16), 0, a, tw);
     return OPH_OK;
 }
 

@@ -369,6 +369,18 @@ __host__ __device__ constexpr int last_radix8(int L) {
     return L;
 }
 
+// Length of the radix-8 twiddle table of an L-point line (all stages).
+__host__ __device__ constexpr int table_len8(int L) {
+    int n = 0, Ns = 1, rem = L;
+    while (rem > 1) {
+        const int R = rem % 8 == 0 ? 8 : rem;
+        if (Ns > 1) n += (R - 1) * Ns;
+        Ns *= R;
+        rem /= R;
+    }
+    return n;
+}
+
 template <int L, int SIGN>
 __device__ __forceinline__ void line_r8(float2* buf, const float2* __restrict__ tw, int t) {
     constexpr int TPL = L / 8;
@@ -389,6 +401,54 @@ __device__ __forceinline__ void line_r8(float2* buf, const float2* __restrict__ 
         float2 w[L / RL / TPL][RL];
         load_stage_tw<L, RL, TPL>(tw, off, Ns, t, w);
         stage<L, RL, SIGN, TPL>(buf, w, Ns, t);
+    }
+}
+
+// line_r8 with the thread's elements in registers at both ends: on entry
+// v[m] is input element t + m L/8, on return output element t + m L/8.
+// Stage 0 (Ns = 1) takes v directly; for L = 8^a the last stage (Ns = L/8,
+// butterfly j = t) reads and writes exactly those elements, so its outputs
+// stay in v; otherwise the last stage goes through the line and v is read
+// back.  The caller syncs before the line is written again (other threads
+// may still read the last stage's inputs).
+template <int L, int SIGN>
+__device__ __forceinline__ void line_r8_regs(float2 (&v)[8], float2* buf, const float2* __restrict__ tw, int t) {
+    static_assert(L >= 64, "at least two radix-8 stages");
+    constexpr int TPL = L / 8;
+    constexpr int A = n8(L);
+    constexpr int RL = last_radix8(L);
+    Dft<8, SIGN>::run(v);
+    {
+        float2* dst = buf + pad16(8 * t);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) dst[r] = v[r];
+    }
+    __syncthreads();
+    int Ns = 8, off = 0;
+    constexpr int NMID = RL == 1 ? A - 2 : A - 1;
+#pragma unroll
+    for (int s = 0; s < NMID; ++s) {
+        float2 w[1][8];
+        load_stage_tw<L, 8, TPL>(tw, off, Ns, t, w);
+        stage<L, 8, SIGN, TPL>(buf, w, Ns, t);
+        off += 7 * Ns;
+        Ns *= 8;
+    }
+    if constexpr (RL == 1) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) v[r] = buf[pad16(t + r * TPL)];
+#pragma unroll
+        for (int r = 1; r < 8; ++r) {
+            const float2 w = tw[off + (r - 1) * TPL + t];
+            v[r] = SIGN < 0 ? cmul(v[r], w) : cmulc(v[r], w);
+        }
+        Dft<8, SIGN>::run(v);
+    } else {
+        float2 w[L / RL / TPL][RL];
+        load_stage_tw<L, RL, TPL>(tw, off, Ns, t, w);
+        stage<L, RL, SIGN, TPL>(buf, w, Ns, t);
+#pragma unroll
+        for (int m = 0; m < 8; ++m) v[m] = buf[pad16(t + m * TPL)];
     }
 }
 
