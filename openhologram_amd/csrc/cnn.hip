@@ -529,6 +529,7 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
     __shared__ __attribute__((aligned(16))) __bf16 in_t[RB_IH * RB_IW * RB_PS];
     __shared__ __attribute__((aligned(16))) __bf16 mid_t[RB_MID * RB_PS];
     __shared__ bf16x8 w2_s[KSTEPS * 64];
+    __shared__ float4 bias_s[16];  // b1, b2
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar branches
     // conv1's fragments in registers, conv2's in LDS, loaded once for every
@@ -537,9 +538,32 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
 #pragma unroll
     for (int s = 0; s < KSTEPS; ++s) Bf[s] = a.w1[s * 64 + lane];
     for (int i = threadIdx.x; i < KSTEPS * 64; i += 256) w2_s[i] = a.w2[i];
+    if (threadIdx.x < 16)
+        bias_s[threadIdx.x] = reinterpret_cast<const float4*>(threadIdx.x < 8 ? a.b1 : a.b2)[threadIdx.x & 7];
     constexpr int GPW = (RB_GROUPS + 3) / 4;  // conv1 groups per wave (3)
     const int tiles_x = (a.W + RB_TW - 1) / RB_TW, tiles_y = (a.H + RB_TH - 1) / RB_TH;
     const int ntiles = tiles_x * tiles_y * a.B;
+    // The staged input of the next tile (12 x 36 pixels x 64 bytes, 16-byte
+    // pieces i = threadIdx.x + 256 n) is loaded into registers while the
+    // current tile computes; the epilogues read only LDS, so no wait on
+    // global loads falls inside the convolutions.
+    constexpr int NPRE = (RB_IH * RB_IW * 4 + 255) / 256;  // 7
+    bf16x8 pre[NPRE];
+    auto fetch = [&](int tile) {
+        const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
+        const int x0 = tx * RB_TW, y0 = ty * RB_TH;
+        const size_t img = (size_t)b * a.H * a.W;
+#pragma unroll
+        for (int n = 0; n < NPRE; ++n) {
+            const int i = threadIdx.x + 256 * n;
+            const int pix = i >> 2, q = i & 3;
+            const int yy = y0 + pix / RB_IW - 2, xx = x0 + pix % RB_IW - 2;
+            pre[n] = (bf16x8)(__bf16)0.f;
+            if (i < RB_IH * RB_IW * 4 && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
+                pre[n] = reinterpret_cast<const bf16x8*>(a.h + (img + (size_t)yy * a.W + xx) * 32)[q];
+        }
+    };
+    if ((int)blockIdx.x < ntiles) fetch(blockIdx.x);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
         const int x0 = tx * RB_TW, y0 = ty * RB_TH;
@@ -551,15 +575,13 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
         asm volatile("" : "+v"(lane_o), "+s"(wave_o));
         const int px = lane_o & 31, kh = lane_o >> 5;
         __syncthreads();  // the previous tile's LDS reads are done
-        for (int i = threadIdx.x; i < RB_IH * RB_IW * 4; i += 256) {
-            const int pix = i >> 2, q = i & 3;
-            const int yy = y0 + pix / RB_IW - 2, xx = x0 + pix % RB_IW - 2;
-            bf16x8 v = (bf16x8)(__bf16)0.f;
-            if (yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-                v = reinterpret_cast<const bf16x8*>(a.h + (img + (size_t)yy * a.W + xx) * 32)[q];
-            *reinterpret_cast<bf16x8*>(in_t + pix * RB_PS + 8 * q) = v;
+#pragma unroll
+        for (int n = 0; n < NPRE; ++n) {
+            const int i = threadIdx.x + 256 * n;
+            if (i < RB_IH * RB_IW * 4) *reinterpret_cast<bf16x8*>(in_t + (i >> 2) * RB_PS + 8 * (i & 3)) = pre[n];
         }
         __syncthreads();
+        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
 
         // conv1 + bias + relu on the 340 halo pixels, groups wave, wave + 4,
         // wave + 8 of 32 flat pixels (group 11 repeats the last pixel and is
@@ -612,7 +634,7 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
                 __bf16* dst = mid_t + p * RB_PS + 4 * kh;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const float4 bq = *reinterpret_cast<const float4*>(a.b1 + 8 * q + 4 * kh);
+                    const float4 bq = bias_s[2 * q + kh];
                     bf16x4 o;
                     o[0] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 0] + bq.x, 0.f) : 0.f);
                     o[1] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 1] + bq.y, 0.f) : 0.f);
@@ -659,7 +681,7 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
             const __bf16* res = in_t + ((r + 2) * RB_IW + px + 2) * RB_PS + 4 * kh;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const float4 bq = *reinterpret_cast<const float4*>(a.b2 + 8 * q + 4 * kh);
+                const float4 bq = bias_s[8 + 2 * q + kh];
                 const bf16x4 rv = *reinterpret_cast<const bf16x4*>(res + 8 * q);
                 bf16x4 o;
                 o[0] = (__bf16)(acc[j][4 * q + 0] + bq.x + (float)rv[0]);

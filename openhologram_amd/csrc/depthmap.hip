@@ -62,7 +62,8 @@ struct DmArgs {
     int ngroups;
     const unsigned char* img;      // h x w, this colour
     const unsigned short* idx;     // h x w
-    float2* I;                     // [L][Ny/2 + 1][w]
+    float2* I;                     // [L][ngroups][Ny/2 + 1][GROUP]: each column group's
+                                   // rows contiguous (I_l(ky, x) at group x / GROUP)
     unsigned char* flags;          // [L][ngroups]
     unsigned char* any;            // [L]
     const double* lc;              // per layer: cx, cy, K (3 doubles)
@@ -124,9 +125,12 @@ __global__ __launch_bounds__(2 * (LY / 16)) void dm_cols(DmArgs a, int layers_pe
     float2* lines = reinterpret_cast<float2*>(smem);                 // 2 x PL
     unsigned* pix = reinterpret_cast<unsigned*>(lines + 2 * PL);     // h x GROUP: idx << 8 | img
     unsigned* mask = pix + (size_t)a.h * GROUP;                      // layer bitmask
-    const int g = blockIdx.x;
+    // XCD-aware: neighbouring column groups (whose 32-byte row segments share
+    // cache lines of I) run on the same XCD at the same time
+    const int lin = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int g = lin % gridDim.x;
     const int x0 = g * GROUP;
-    const int l_begin = a.lo + blockIdx.y * layers_per_wg;
+    const int l_begin = a.lo + lin / gridDim.x * layers_per_wg;
     const int l_end = min(a.hi, l_begin + layers_per_wg - 1);
     const int nwords = (a.L + 32) / 32;
     const int p = threadIdx.x / TPL, t = threadIdx.x % TPL;
@@ -152,7 +156,8 @@ __global__ __launch_bounds__(2 * (LY / 16)) void dm_cols(DmArgs a, int layers_pe
         pix[e] = v;
     }
     __syncthreads();
-    const size_t lstride = (size_t)(a.Ny / 2 + 1) * a.w;
+    const size_t gstride = (size_t)(a.Ny / 2 + 1) * GROUP;
+    const size_t lstride = gstride * a.ngroups;
     for (int l = l_begin; l <= l_end; ++l) {
         if (!((mask[l >> 5] >> (l & 31)) & 1u)) continue;
         // two real columns (x0 + 2p, x0 + 2p + 1) as one complex column
@@ -171,7 +176,7 @@ __global__ __launch_bounds__(2 * (LY / 16)) void dm_cols(DmArgs a, int layers_pe
         __syncthreads();
         fft::line<LY, -1>(line, Tw, t);
         // separate: A(k) = (Z(k) + conj Z(-k))/2, B(k) = (Z(k) - conj Z(-k))/(2i)
-        float2* dst = a.I + (size_t)(l - 1) * lstride;
+        float2* dst = a.I + (size_t)(l - 1) * lstride + (size_t)g * gstride;
         for (int e = threadIdx.x; e < (LY / 2 + 1) * GROUP; e += NT) {
             const int c = e % GROUP, ky = e / GROUP;
             const int x = x0 + c;
@@ -184,7 +189,7 @@ __global__ __launch_bounds__(2 * (LY / 16)) void dm_cols(DmArgs a, int layers_pe
                 v = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
             else
                 v = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
-            dst[(size_t)ky * a.w + x] = v;
+            dst[e] = v;
         }
         if (threadIdx.x == 0) {
             a.flags[(size_t)(l - 1) * a.ngroups + g] = 1;
@@ -257,12 +262,13 @@ __global__ __launch_bounds__(LX / 16) void dm_rows(DmArgs a, const float2* __res
     constexpr int PL = fft::padded_len(LX);
     __shared__ float2 line[PL];
     const int t = threadIdx.x;
-    const int kya = blockIdx.x;               // 0 .. Ny/2 - 1
+    const int kya = xcd_remap(blockIdx.x, gridDim.x);  // 0 .. Ny/2 - 1
     const bool pair0 = kya == 0;              // real rows 0 and Ny/2 packed as re/im
     const int kyb = pair0 ? a.Ny / 2 : a.Ny - kya;
     const int sya = sgnidx(kya, a.Ny) + 1;
     const int syb = sgnidx(kyb, a.Ny) + 1;
-    const size_t nyq = (size_t)(a.Ny / 2) * a.w;  // row Ny/2 relative to row 0
+    const size_t nyq = (size_t)(a.Ny / 2) * GROUP;  // row Ny/2 relative to row 0
+    const size_t gstride = (size_t)(a.Ny / 2 + 1) * GROUP;
     fft::Twiddles<LX> Tw;
     Tw.load(twx, t);
     float2 acca[16], accb[16];
@@ -275,18 +281,19 @@ __global__ __launch_bounds__(LX / 16) void dm_rows(DmArgs a, const float2* __res
         sx2[m] = (double)sx * sx;
         usx[m] = a.ux * (float)(sx * sx);
     }
-    const size_t lstride = (size_t)(a.Ny / 2 + 1) * a.w;
+    const size_t lstride = gstride * a.ngroups;
     for (int l = a.lo; l <= a.hi; ++l) {
         if (!a.any[l - 1]) continue;
-        const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * a.w;
+        const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * GROUP;
         const unsigned char* fl = a.flags + (size_t)(l - 1) * a.ngroups;
 #pragma unroll
         for (int m = 0; m < 16; ++m) {
             const int x = t + m * TPL - a.ox;
             float2 v = make_float2(0.f, 0.f);
             if (x >= 0 && x < a.w && fl[x / GROUP]) {
-                v = src[x];
-                if (pair0) v.y = src[nyq + x].x;
+                const size_t o = (size_t)(x / GROUP) * gstride + x % GROUP;
+                v = src[o];
+                if (pair0) v.y = src[nyq + o].x;
             }
             line[fft::lane_pad<LX>(t, m)] = v;
         }
@@ -373,7 +380,7 @@ __device__ __forceinline__ void finish_row8(const DmArgs& a, const float2 (&acc)
 
 // PAIR0: workgroup 0, the real rows 0 and Ny/2 packed as re/im.
 template <int LX, bool PAIR0>
-__device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __restrict__ tw8, float2* sm) {
+__device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __restrict__ tw8, float2* sm, int kya) {
     constexpr int TPL = LX / 8;
     constexpr int PL = fft::padded_len(LX);
     constexpr int CH = 16;
@@ -391,11 +398,11 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
         if (a.any[i]) atomicOr(&amask[i >> 5], 1u << (i & 31));
     __syncthreads();
     auto present = [&](int l) -> bool { return (amask[(l - 1) >> 5] >> ((l - 1) & 31)) & 1u; };
-    const int kya = PAIR0 ? 0 : (int)blockIdx.x;  // 0 .. Ny/2 - 1
     const int kyb = PAIR0 ? a.Ny / 2 : a.Ny - kya;
     const int sya = sgnidx(kya, a.Ny) + 1;
     const int syb = sgnidx(kyb, a.Ny) + 1;
-    const size_t nyq = (size_t)(a.Ny / 2) * a.w;
+    const size_t nyq = (size_t)(a.Ny / 2) * GROUP;
+    const size_t gstride = (size_t)(a.Ny / 2 + 1) * GROUP;
     // Thread t owns row a at kx = t + m TPL and row b at kb = -kx mod LX:
     // row b's transform at kb is conj Z(kx) of the thread's own element of
     // row a's transform Z, and the transfer depends on kx only through kx^2,
@@ -420,7 +427,7 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
             Sa[kx] = Sb[(LX - kx) & (LX - 1)] = make_float2(0.f, 0.f);
         }
     }
-    const size_t lstride = (size_t)(a.Ny / 2 + 1) * a.w;
+    const size_t lstride = gstride * a.ngroups;
     // Layers are taken chunk by chunk, top layer first within a chunk.  The
     // row of the next present layer in that order is loaded into registers
     // (values and presence flags) before the current layer's transform, so
@@ -435,17 +442,18 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
     float2 nv[8];
     unsigned nf[8];
     auto fetch = [&](int l) {
-        const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * a.w;
+        const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * GROUP;
         const unsigned char* fl = a.flags + (size_t)(l - 1) * a.ngroups;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
             const int x = t + m * TPL - a.ox;
             const bool in = x >= 0 && x < a.w;
+            const size_t o = (size_t)(x / GROUP) * gstride + x % GROUP;
             nf[m] = in ? fl[x / GROUP] : 0u;
             if constexpr (PAIR0)
-                nv[m] = in ? make_float2(src[x].x, src[nyq + x].x) : make_float2(0.f, 0.f);
+                nv[m] = in ? make_float2(src[o].x, src[nyq + o].x) : make_float2(0.f, 0.f);
             else
-                nv[m] = in ? src[x] : make_float2(0.f, 0.f);
+                nv[m] = in ? src[o] : make_float2(0.f, 0.f);
         }
     };
     int pl = min(a.hi, a.lo + CH - 1);
@@ -535,10 +543,13 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
 template <int LX>
 __global__ __launch_bounds__(LX / 8) void dm_rows_h(DmArgs a, const float2* __restrict__ tw8) {
     extern __shared__ float2 sm[];
-    if (blockIdx.x == 0)
-        rows_h_body<LX, true>(a, tw8, sm);
+    // XCD-aware: neighbouring rows, which share the cache lines of I, run on
+    // the same XCD at the same time
+    const int kya = xcd_remap(blockIdx.x, gridDim.x);  // 0 .. Ny/2 - 1
+    if (kya == 0)
+        rows_h_body<LX, true>(a, tw8, sm, 0);
     else
-        rows_h_body<LX, false>(a, tw8, sm);
+        rows_h_body<LX, false>(a, tw8, sm, kya);
 }
 
 // ---- final column pass ---------------------------------------------------------
@@ -704,7 +715,7 @@ extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, cons
 
     const size_t npx = (size_t)w * h;
     const int ngroups = (w + GROUP - 1) / GROUP;
-    const size_t lstride = (size_t)(Ny / 2 + 1) * w;
+    const size_t lstride = (size_t)(Ny / 2 + 1) * GROUP * ngroups;
     // device buffers
     unsigned char* dimg = const_cast<unsigned char*>(intensity);
     unsigned char* ddep = const_cast<unsigned char*>(depth);
