@@ -28,6 +28,17 @@
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+// Raw buffer resource over [p, p + bytes) (gfx9 word 3).  Buffer loads and
+// stores are range-checked by the hardware: an offset at or beyond `bytes`
+// reads zeros and writes nothing, so bounds are handled without branches
+// (and the compiler's wait counts stay exact across the tile loop).
+constexpr unsigned BUF_OOB = 0x80000000u;  // an offset beyond every buffer used here
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
 
 struct oph_cnn {
     int blocks = 8, ch = 32;
@@ -496,13 +507,14 @@ __global__ void add_inplace(float* a, const float* b, int64_t n) {
 // h' = h + conv2(relu(conv1(h))) for C = 32 channels in one kernel.  A
 // persistent workgroup of 4 waves (conv1's weight fragments in registers,
 // conv2's in LDS, loaded once) walks 8 x 32 output tiles; for each it
-// stages h on the tile
-// plus a two-pixel halo (12 x 36 pixels) in LDS, computes relu(conv1) on
-// the tile plus a one-pixel halo (10 x 34 = 340 pixels, taken as 11 flat
-// groups of 32 for the MFMA rows; pixels outside the image are the zero
+// stages h on the tile plus a two-pixel halo (12 x 36 pixels) in LDS,
+// computes relu(conv1) on the tile plus a one-pixel halo (10 x 34 pixels:
+// ten rows of 32 and a 10 x 2 strip; pixels outside the image are the zero
 // padding of conv2) into a second LDS tile, then conv2 on the 8 rows of 32
-// with the residual read from the staged h.  HBM traffic per block: h read
-// once (plus halo), h' written once.  The MFMAs take the weights as the A
+// with the residual read from the staged h.  Each wave takes consecutive
+// rows so that one LDS operand serves the taps of up to three output rows.
+// HBM traffic per block: h read once (plus halo), h' written once.  The
+// MFMAs take the weights as the A
 // operand, so each lane ends with 4 runs of 4 consecutive channels of one
 // pixel (8-byte stores, one bounds check per pixel).  Same fragments,
 // BF16 rounding and FP32 accumulation as conv3x3_fwd.
@@ -510,9 +522,8 @@ constexpr int RB_TH = 8, RB_TW = 32;
 constexpr int RB_IH = RB_TH + 4, RB_IW = RB_TW + 4;   // staged input
 constexpr int RB_MH = RB_TH + 2, RB_MW = RB_TW + 2;   // conv1 output
 constexpr int RB_MID = RB_MH * RB_MW;                 // 340 pixels
-constexpr int RB_GROUPS = (RB_MID + 31) / 32;         // 11
 constexpr int RB_PS = 32 + 8;                         // LDS pixel stride (bf16)
-constexpr int RB_AHEAD = 2;                           // k-steps of LDS operands read ahead
+constexpr int RB_NPRE = (RB_IH * RB_IW * 4 + 255) / 256;  // 16-byte pieces per thread (7)
 
 struct ResArgs {
     const __bf16* h;      // [B][H][W][32]
@@ -524,9 +535,201 @@ struct ResArgs {
     int B, H, W;
 };
 
+// Software pipelines of the two convolutions, unrolled at compile time: the
+// LDS operands of step S are read at step S and consumed by the MFMAs at
+// step S + AHEAD, and sched_group_barrier keeps that order (masks: 0x100 LDS
+// read, 0x008 MFMA).
+constexpr int RB_C1_AHEAD = 4, RB_C2_AHEAD = 3;
+
+// conv1 steps: S = (NR + 2) k + i takes operand k = 2 dx + half of input
+// row rb_c1_row<NR>(i); the row order makes consecutive MFMAs alternate
+// between the output rows' accumulators (no back-to-back dependent MFMAs):
+// NR = 3: rows 2, 0, 3, 1, 4; NR = 2: rows 0, 3, 1, 2.
+template <int NR>
+__host__ __device__ constexpr int rb_c1_row(int i) {
+    return NR == 3 ? (i == 0 ? 2 : i == 1 ? 0 : i == 2 ? 3 : i == 3 ? 1 : 4) : (i == 0 ? 0 : i == 1 ? 3 : i == 2 ? 1 : 2);
+}
+template <int NR>
+__host__ __device__ constexpr int rb_c1_mfmas(int u) {
+    const int ri = rb_c1_row<NR>(u % (NR + 2));
+    int n = 0;
+    for (int j = 0; j < NR; ++j) n += (ri - j >= 0 && ri - j <= 2) ? 1 : 0;
+    return n;
+}
+
+template <int NR, int S>
+__device__ __forceinline__ void rb_c1_pipe(const __bf16* src, bf16x8 (&X)[RB_C1_AHEAD + 1], f32x16 (&acc)[NR],
+                                           const bf16x8 (&Bf)[18]) {
+    constexpr int NS = (NR + 2) * 6, AH = RB_C1_AHEAD;
+    if constexpr (S < NS + AH) {
+        if constexpr (S < NS) {
+            constexpr int ri = rb_c1_row<NR>(S % (NR + 2)), k = S / (NR + 2), dx = k >> 1, half = k & 1;
+            X[S % (AH + 1)] = *reinterpret_cast<const bf16x8*>(src + (ri * RB_IW + dx) * RB_PS + 16 * half);
+        }
+        constexpr int U = S - AH;
+        if constexpr (U >= 0) {
+            constexpr int ri = rb_c1_row<NR>(U % (NR + 2)), k = U / (NR + 2), dx = k >> 1, half = k & 1;
+#pragma unroll
+            for (int j = 0; j < NR; ++j) {
+                const int dy = ri - j;
+                if (dy >= 0 && dy <= 2)
+                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bf[(3 * dy + dx) * 2 + half], X[U % (AH + 1)],
+                                                                     acc[j], 0, 0, 0);
+            }
+        }
+        if constexpr (S < NS) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        if constexpr (U >= 0) __builtin_amdgcn_sched_group_barrier(0x008, rb_c1_mfmas<NR>(U), 0);
+        rb_c1_pipe<NR, S + 1>(src, X, acc, Bf);
+    }
+}
+
+// conv2 step S = 6 dy + k: weight fragment S, operands of mid rows r0 + dy
+// (read at step S - 6, or here for dy = 0) and r0 + dy + 1 (read here)
+template <int S>
+__device__ __forceinline__ void rb_c2_pipe(const __bf16* src, const bf16x8* w2l, bf16x8 (&Xr)[4][6],
+                                           bf16x8 (&Wr)[RB_C2_AHEAD + 1], f32x16 (&acc)[2]) {
+    constexpr int NS = 18, AH = RB_C2_AHEAD;
+    if constexpr (S < NS + AH) {
+        if constexpr (S < NS) {
+            constexpr int dy = S / 6, k = S % 6, dx = k >> 1, half = k & 1;
+            Wr[S % (AH + 1)] = w2l[S * 64];
+            if constexpr (dy == 0) Xr[0][k] = *reinterpret_cast<const bf16x8*>(src + dx * RB_PS + 16 * half);
+            Xr[dy + 1][k] = *reinterpret_cast<const bf16x8*>(src + ((dy + 1) * RB_MW + dx) * RB_PS + 16 * half);
+        }
+        constexpr int U = S - AH;
+        if constexpr (U >= 0) {
+            constexpr int dy = U / 6, k = U % 6;
+            acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Wr[U % (AH + 1)], Xr[dy][k], acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Wr[U % (AH + 1)], Xr[dy + 1][k], acc[1], 0, 0, 0);
+        }
+        if constexpr (S < NS) __builtin_amdgcn_sched_group_barrier(0x100, S < 6 ? 3 : 2, 0);
+        if constexpr (U >= 0) __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        rb_c2_pipe<S + 1>(src, w2l, Xr, Wr, acc);
+    }
+}
+
+// conv1 (weights in registers) + bias + relu for NR consecutive rows of 32
+// pixels of the halo tile, mid rows r0 .. r0 + NR - 1, mid columns px.
+// Input row r0 + ri feeds output row r0 + j at tap row dy = ri - j, so
+// each 16-byte operand (input row, column offset dx, channel half) is read
+// from LDS once and used by up to 3 MFMAs.
+template <int NR>
+__device__ __forceinline__ void rb_conv1_rows(const __bf16* in_t, __bf16* mid_t, const bf16x8 (&Bf)[18],
+                                              const float4* bias_s, int r0, int px, int kh, int x0, int y0, int H,
+                                              int W) {
+    f32x16 acc[NR];
+#pragma unroll
+    for (int j = 0; j < NR; ++j) acc[j] = (f32x16)0.f;
+    const __bf16* src = in_t + (r0 * RB_IW + px) * RB_PS + 8 * kh;
+    bf16x8 X[RB_C1_AHEAD + 1];
+    rb_c1_pipe<NR, 0>(src, X, acc, Bf);
+    const int xx = x0 - 1 + px;
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+        const int mr = r0 + j;
+        const int yy = y0 - 1 + mr;
+        const bool inside = yy >= 0 && yy < H && xx >= 0 && xx < W;  // else conv2's zero padding
+        __bf16* dst = mid_t + (mr * RB_MW + px) * RB_PS + 4 * kh;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 bq = bias_s[2 * q + kh];
+            bf16x4 o;
+            o[0] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 0] + bq.x, 0.f) : 0.f);
+            o[1] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 1] + bq.y, 0.f) : 0.f);
+            o[2] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 2] + bq.z, 0.f) : 0.f);
+            o[3] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 3] + bq.w, 0.f) : 0.f);
+            *reinterpret_cast<bf16x4*>(dst + 8 * q) = o;
+        }
+    }
+}
+
+// conv1 for the 10 x 2 strip of mid columns 32, 33 (20 pixels; lanes 20..31
+// repeat the last one and store nothing).
+__device__ __forceinline__ void rb_conv1_strip(const __bf16* in_t, __bf16* mid_t, const bf16x8 (&Bf)[18],
+                                               const float4* bias_s, int px, int kh, int x0, int y0, int H, int W) {
+    const int f = px < 2 * RB_MH ? px : 2 * RB_MH - 1;
+    const int mr = f >> 1, mc = RB_TW + (f & 1);
+    const __bf16* src = in_t + (mr * RB_IW + mc) * RB_PS + 8 * kh;
+    // two accumulation chains (even and odd k steps) so that consecutive
+    // MFMAs do not depend on each other
+    f32x16 ac2[2] = {(f32x16)0.f, (f32x16)0.f};
+#pragma unroll
+    for (int s = 0; s < 18; ++s) {
+        const int tap = s >> 1;
+        const bf16x8 X =
+            *reinterpret_cast<const bf16x8*>(src + ((tap / 3) * RB_IW + tap % 3) * RB_PS + 16 * (s & 1));
+        ac2[s & 1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bf[s], X, ac2[s & 1], 0, 0, 0);
+    }
+    const f32x16 acc = ac2[0] + ac2[1];
+    if (px >= 2 * RB_MH) return;
+    const int yy = y0 - 1 + mr, xx = x0 - 1 + mc;
+    const bool inside = yy >= 0 && yy < H && xx >= 0 && xx < W;
+    __bf16* dst = mid_t + (mr * RB_MW + mc) * RB_PS + 4 * kh;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 bq = bias_s[2 * q + kh];
+        bf16x4 o;
+        o[0] = (__bf16)(inside ? fmaxf(acc[4 * q + 0] + bq.x, 0.f) : 0.f);
+        o[1] = (__bf16)(inside ? fmaxf(acc[4 * q + 1] + bq.y, 0.f) : 0.f);
+        o[2] = (__bf16)(inside ? fmaxf(acc[4 * q + 2] + bq.z, 0.f) : 0.f);
+        o[3] = (__bf16)(inside ? fmaxf(acc[4 * q + 3] + bq.w, 0.f) : 0.f);
+        *reinterpret_cast<bf16x4*>(dst + 8 * q) = o;
+    }
+}
+
+// Outputs of one wave for one tile (2 rows x 4 runs of 4 channels, BF16)
+// and their byte offsets in the image (out of range for pixels outside it).
+struct RbOut {
+    u32x2 v[2][4];
+    unsigned dst[2];
+};
+
+__device__ __forceinline__ void rb_store(const RbOut& o, __amdgpu_buffer_rsrc_t img) {
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) __builtin_amdgcn_raw_buffer_store_b64(o.v[j][q], img, o.dst[j] + 16 * q, 0, 0);
+}
+
+// conv2 (weights from LDS) + bias + residual for output rows r0, r0 + 1 of
+// the tile.  Taps are taken row by row (dy outer): the operands of mid row
+// r0 + dy + 1 serve output row r0 + 1 at dy and output row r0 at dy + 1, so
+// each is read once; each weight fragment is read once for both rows.
+__device__ __forceinline__ void rb_conv2_rows(const __bf16* in_t, const __bf16* mid_t, const bf16x8* w2_s,
+                                              const float4* bias_s, RbOut& out, int r0, int lane, int x0, int y0,
+                                              int H, int W) {
+    const int px = lane & 31, kh = lane >> 5;
+    f32x16 acc[2] = {(f32x16)0.f, (f32x16)0.f};
+    const __bf16* src = mid_t + (r0 * RB_MW + px) * RB_PS + 8 * kh;
+    bf16x8 Xr[4][6], Wr[RB_C2_AHEAD + 1];
+    rb_c2_pipe<0>(src, w2_s + lane, Xr, Wr, acc);
+    const int x = x0 + px;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int r = r0 + j;
+        const int y = y0 + r;
+        // pixels outside the image: offset out of range, store dropped
+        out.dst[j] = y < H && x < W ? (unsigned)((y * W + x) * 64 + 8 * kh) : BUF_OOB;
+        const __bf16* res = in_t + ((r + 2) * RB_IW + px + 2) * RB_PS + 4 * kh;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float4 bq = bias_s[8 + 2 * q + kh];
+            const bf16x4 rv = *reinterpret_cast<const bf16x4*>(res + 8 * q);
+            bf16x4 o;
+            o[0] = (__bf16)(acc[j][4 * q + 0] + bq.x + (float)rv[0]);
+            o[1] = (__bf16)(acc[j][4 * q + 1] + bq.y + (float)rv[1]);
+            o[2] = (__bf16)(acc[j][4 * q + 2] + bq.z + (float)rv[2]);
+            o[3] = (__bf16)(acc[j][4 * q + 3] + bq.w + (float)rv[3]);
+            out.v[j][q] = __builtin_bit_cast(u32x2, o);
+        }
+    }
+}
+
 __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
     constexpr int KSTEPS = ksteps_c(32);
-    __shared__ __attribute__((aligned(16))) __bf16 in_t[RB_IH * RB_IW * RB_PS];
+    // staged input plus room for the unused tail of the last 16-byte pieces
+    // (the staging writes are unconditional)
+    __shared__ __attribute__((aligned(16))) __bf16 in_t[(RB_NPRE * 256 / 4) * RB_PS];
     __shared__ __attribute__((aligned(16))) __bf16 mid_t[RB_MID * RB_PS];
     __shared__ bf16x8 w2_s[KSTEPS * 64];
     __shared__ float4 bias_s[16];  // b1, b2
@@ -540,29 +743,35 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
     for (int i = threadIdx.x; i < KSTEPS * 64; i += 256) w2_s[i] = a.w2[i];
     if (threadIdx.x < 16)
         bias_s[threadIdx.x] = reinterpret_cast<const float4*>(threadIdx.x < 8 ? a.b1 : a.b2)[threadIdx.x & 7];
-    constexpr int GPW = (RB_GROUPS + 3) / 4;  // conv1 groups per wave (3)
     const int tiles_x = (a.W + RB_TW - 1) / RB_TW, tiles_y = (a.H + RB_TH - 1) / RB_TH;
     const int ntiles = tiles_x * tiles_y * a.B;
     // The staged input of the next tile (12 x 36 pixels x 64 bytes, 16-byte
     // pieces i = threadIdx.x + 256 n) is loaded into registers while the
     // current tile computes; the epilogues read only LDS, so no wait on
     // global loads falls inside the convolutions.
-    constexpr int NPRE = (RB_IH * RB_IW * 4 + 255) / 256;  // 7
+    constexpr int NPRE = RB_NPRE;
     bf16x8 pre[NPRE];
+    const unsigned img_bytes = (unsigned)a.H * (unsigned)a.W * 64u;  // < 2^31 (checked by the launcher)
     auto fetch = [&](int tile) {
         const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
         const int x0 = tx * RB_TW, y0 = ty * RB_TH;
-        const size_t img = (size_t)b * a.H * a.W;
+        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(a.h + (size_t)b * a.H * a.W * 32, img_bytes);
 #pragma unroll
         for (int n = 0; n < NPRE; ++n) {
             const int i = threadIdx.x + 256 * n;
             const int pix = i >> 2, q = i & 3;
             const int yy = y0 + pix / RB_IW - 2, xx = x0 + pix % RB_IW - 2;
-            pre[n] = (bf16x8)(__bf16)0.f;
-            if (i < RB_IH * RB_IW * 4 && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W)
-                pre[n] = reinterpret_cast<const bf16x8*>(a.h + (img + (size_t)yy * a.W + xx) * 32)[q];
+            const bool ok = i < RB_IH * RB_IW * 4 && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+            const unsigned off = ok ? (unsigned)((yy * a.W + xx) * 64 + 16 * q) : BUF_OOB;
+            pre[n] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
         }
     };
+    // A tile's outputs are stored after the next tile's staging and before
+    // its prefetch, so that the prefetch is always the newest memory
+    // operation when the staging waits for it (the wait counter is in order)
+    // and the stores have a whole tile to complete.
+    RbOut pend;
+    __amdgpu_buffer_rsrc_t pend_img = buf_rsrc(a.out, img_bytes);
     if ((int)blockIdx.x < ntiles) fetch(blockIdx.x);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
@@ -578,120 +787,27 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
 #pragma unroll
         for (int n = 0; n < NPRE; ++n) {
             const int i = threadIdx.x + 256 * n;
-            if (i < RB_IH * RB_IW * 4) *reinterpret_cast<bf16x8*>(in_t + (i >> 2) * RB_PS + 8 * (i & 3)) = pre[n];
+            *reinterpret_cast<bf16x8*>(in_t + (i >> 2) * RB_PS + 8 * (i & 3)) = pre[n];
         }
         __syncthreads();
+        if (tile != (int)blockIdx.x) rb_store(pend, pend_img);
         if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
 
-        // conv1 + bias + relu on the 340 halo pixels, groups wave, wave + 4,
-        // wave + 8 of 32 flat pixels (group 11 repeats the last pixel and is
-        // never stored).  MFMA with the weights as A: the result is
-        // [channel][pixel], register v of lane (px, kh) = channel
-        // 8 (v/4) + 4 kh + v%4 of pixel px.
-        {
-            f32x16 acc[GPW];
-            int base[GPW];  // LDS element offset of the lane's pixel, channel half
-#pragma unroll
-            for (int j = 0; j < GPW; ++j) {
-                acc[j] = (f32x16)0.f;
-                int p = (wave_o + 4 * j) * 32 + px;
-                if (p >= RB_MID) p = RB_MID - 1;
-                const int mr = (int)(((float)p + 0.5f) * (1.f / RB_MW));  // p / 34
-                base[j] = (mr * RB_IW + (p - mr * RB_MW)) * RB_PS + 8 * kh;
-            }
-            // k = 16 s + 8 kh: tap s / 2, channels 16 (s % 2) + 8 kh, so the
-            // per-step offset is a compile-time constant.  Operands are read
-            // RB_AHEAD steps ahead of their MFMAs (LDS latency ~ 3 MFMAs).
-            bf16x8 X[RB_AHEAD + 1][GPW];
-#pragma unroll
-            for (int s = 0; s < KSTEPS + RB_AHEAD; ++s) {
-                if (s < KSTEPS) {
-                    const int tap = s >> 1;
-                    const int off = ((tap / 3) * RB_IW + tap % 3) * RB_PS + 16 * (s & 1);
-#pragma unroll
-                    for (int j = 0; j < GPW; ++j)
-                        X[s % (RB_AHEAD + 1)][j] = *reinterpret_cast<const bf16x8*>(in_t + base[j] + off);
-                }
-                const int u = s - RB_AHEAD;
-                if (u >= 0) {
-#pragma unroll
-                    for (int j = 0; j < GPW; ++j)
-                        acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bf[u], X[u % (RB_AHEAD + 1)][j], acc[j], 0,
-                                                                         0, 0);
-                }
-                // keep the schedule: this step's reads, then the MFMAs of
-                // the step RB_AHEAD behind (masks: 0x100 LDS read, 0x8 MFMA)
-                if (s < KSTEPS) __builtin_amdgcn_sched_group_barrier(0x100, GPW, 0);
-                if (u >= 0) __builtin_amdgcn_sched_group_barrier(0x008, GPW, 0);
-            }
-#pragma unroll
-            for (int j = 0; j < GPW; ++j) {
-                const int p = (wave_o + 4 * j) * 32 + px;
-                if (p >= RB_MID) continue;
-                const int mr = (int)(((float)p + 0.5f) * (1.f / RB_MW));
-                const int yy = y0 - 1 + mr, xx = x0 - 1 + (p - mr * RB_MW);
-                const bool inside = yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;  // else conv2's zero padding
-                __bf16* dst = mid_t + p * RB_PS + 4 * kh;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const float4 bq = bias_s[2 * q + kh];
-                    bf16x4 o;
-                    o[0] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 0] + bq.x, 0.f) : 0.f);
-                    o[1] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 1] + bq.y, 0.f) : 0.f);
-                    o[2] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 2] + bq.z, 0.f) : 0.f);
-                    o[3] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 3] + bq.w, 0.f) : 0.f);
-                    *reinterpret_cast<bf16x4*>(dst + 8 * q) = o;
-                }
-            }
-        }
+        // conv1 + bias + relu on the 10 x 34 halo tile: rows of 32 (mid
+        // columns 0..31) as 3, 3, 2, 2 consecutive rows per wave, and the
+        // 10 x 2 strip of columns 32, 33 as one flat group on wave 3
+        if (wave_o < 2)
+            rb_conv1_rows<3>(in_t, mid_t, Bf, bias_s, 3 * wave_o, px, kh, x0, y0, a.H, a.W);
+        else
+            rb_conv1_rows<2>(in_t, mid_t, Bf, bias_s, 6 + 2 * (wave_o - 2), px, kh, x0, y0, a.H, a.W);
+        if (wave_o == 3) rb_conv1_strip(in_t, mid_t, Bf, bias_s, px, kh, x0, y0, a.H, a.W);
         __syncthreads();
 
-        // conv2 + bias + residual on the 8 x 32 tile: rows wave and wave + 4
-        f32x16 acc[2] = {(f32x16)0.f, (f32x16)0.f};
-        const int base2 = (wave_o * RB_MW + px) * RB_PS + 8 * kh;  // row wave, column px
-        bf16x8 X2[RB_AHEAD + 1][2], W2[RB_AHEAD + 1];
-#pragma unroll
-        for (int s = 0; s < KSTEPS + RB_AHEAD; ++s) {
-            if (s < KSTEPS) {
-                const int tap = s >> 1;
-                const int off = ((tap / 3) * RB_MW + tap % 3) * RB_PS + 16 * (s & 1);
-                W2[s % (RB_AHEAD + 1)] = w2_s[s * 64 + lane_o];
-#pragma unroll
-                for (int j = 0; j < 2; ++j)  // rows wave + 4 j
-                    X2[s % (RB_AHEAD + 1)][j] =
-                        *reinterpret_cast<const bf16x8*>(mid_t + base2 + 4 * j * RB_MW * RB_PS + off);
-            }
-            const int u = s - RB_AHEAD;
-            if (u >= 0) {
-#pragma unroll
-                for (int j = 0; j < 2; ++j)
-                    acc[j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W2[u % (RB_AHEAD + 1)], X2[u % (RB_AHEAD + 1)][j],
-                                                                     acc[j], 0, 0, 0);
-            }
-            if (s < KSTEPS) __builtin_amdgcn_sched_group_barrier(0x100, 3, 1);
-            if (u >= 0) __builtin_amdgcn_sched_group_barrier(0x008, 2, 1);
-        }
-        const int x = x0 + px;
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int r = wave_o + 4 * j;
-            const int y = y0 + r;
-            if (y >= a.H || x >= a.W) continue;
-            __bf16* dst = a.out + (img + (size_t)y * a.W + x) * 32 + 4 * kh;
-            const __bf16* res = in_t + ((r + 2) * RB_IW + px + 2) * RB_PS + 4 * kh;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const float4 bq = bias_s[8 + 2 * q + kh];
-                const bf16x4 rv = *reinterpret_cast<const bf16x4*>(res + 8 * q);
-                bf16x4 o;
-                o[0] = (__bf16)(acc[j][4 * q + 0] + bq.x + (float)rv[0]);
-                o[1] = (__bf16)(acc[j][4 * q + 1] + bq.y + (float)rv[1]);
-                o[2] = (__bf16)(acc[j][4 * q + 2] + bq.z + (float)rv[2]);
-                o[3] = (__bf16)(acc[j][4 * q + 3] + bq.w + (float)rv[3]);
-                *reinterpret_cast<bf16x4*>(dst + 8 * q) = o;
-            }
-        }
+        // conv2 + bias + residual on the 8 x 32 tile: rows 2 wave, 2 wave + 1
+        rb_conv2_rows(in_t, mid_t, w2_s, bias_s, pend, 2 * wave_o, lane_o, x0, y0, a.H, a.W);
+        pend_img = buf_rsrc(a.out + img * 32, img_bytes);
     }
+    if ((int)blockIdx.x < ntiles) rb_store(pend, pend_img);
 }
 
 // ---- host helpers ------------------------------------------------------------------
@@ -855,6 +971,10 @@ int forward_bf16(oph_ctx* ctx, oph_cnn* net, const float* x, float* y, int B, in
     // persistent: two workgroups per CU (LDS-bound), each walking tiles
     const int64_t ntiles = (int64_t)((W + RB_TW - 1) / RB_TW) * ((H + RB_TH - 1) / RB_TH) * B;
     if (ntiles > INT32_MAX) return oph_fail(ctx, OPH_EINVAL, "too many pixels");
+    // the fused block addresses one image's activations as a range-checked
+    // buffer with 32-bit offsets
+    if ((int64_t)H * W * 64 >= (1LL << 31))
+        return oph_fail(ctx, OPH_ENOTSUP, "image %dx%d too large for the BF16 inference path (< 2^25 pixels)", H, W);
     const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 2LL * std::max(ctx->cu_count, 1));
     for (int b = 0; b < net->blocks; ++b) {
         const ResArgs r{h(b), h(b + 1), WF + F.conv1(b), WF + F.conv2(b), P + L.b1(b), P + L.b2(b), B, H, W};
