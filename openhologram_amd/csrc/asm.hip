@@ -301,7 +301,17 @@ __global__ __launch_bounds__(L / 16) void asm_rows_inv_p(const float2* __restric
 // elements t + m L/16 in both), so the line passes through LDS only between
 // stages.  HALF: the data columns [ox, ox + nx) lie inside [L/4, 3L/4)
 // (twofold padding), so only m in [4, 12) can be nonzero on input (forward)
-// or is needed on output (inverse).
+// or is needed on output (inverse).  The next row is loaded while the
+// current one is transformed: the stage twiddles of a thread do not depend
+// on the row and are loaded once (the middle stages' tables into LDS, the
+// last stage's 15 per thread into registers), and the row loads and
+// stores are range-checked buffer accesses without branches, so the only
+// memory wait in the loop is the one for the prefetched row (the wait
+// counter is in order; a global twiddle load or a branch around a load
+// would make that wait early).  The first row is peeled off the loop, so the
+// loop is entered only with a row's loads followed by a row's stores
+// outstanding: the wait before each row's first butterfly then covers the
+// loads and not the previous row's stores.
 template <int L, bool HALF>
 __global__ __launch_bounds__(L / 16) void asm_rows_fwd_f(const float2* __restrict__ in, int nx, int ny, int ox,
                                                          int oy, float2* __restrict__ work,
@@ -311,34 +321,43 @@ __global__ __launch_bounds__(L / 16) void asm_rows_fwd_f(const float2* __restric
     constexpr int PL = fft::padded_len(L);
     constexpr int M0 = HALF ? 4 : 0, M1 = HALF ? 12 : 16;
     __shared__ float2 buf[PL];
+    constexpr int TWO = fft::last_stage_tw_offset<L>();
+    __shared__ float2 twm[TWO > 0 ? TWO : 1];
     const int t = threadIdx.x;
+    for (int i = t; i < TWO; i += TPL) twm[i] = tw[i];
+    float2 wl[16];
+#pragma unroll
+    for (int r = 1; r < 16; ++r) wl[r] = tw[TWO + (r - 1) * TPL + t];
+    // the field is at most L x L complex64 (128 MiB for L = 4096)
+    const __amdgpu_buffer_rsrc_t src = oph_buf_rsrc(in, (unsigned)((size_t)nx * ny * sizeof(float2)));
     float2 nxt[M1 - M0];
     int r = blockIdx.x;
     auto fetch = [&](int row) {
 #pragma unroll
         for (int m = M0; m < M1; ++m) {
             const int c = t + m * TPL - ox;
-            nxt[m - M0] = (row < ny && c >= 0 && c < nx) ? in[(size_t)row * nx + c] : make_float2(0.f, 0.f);
+            const bool ok = row < ny && c >= 0 && c < nx;
+            nxt[m - M0] = oph_buf_load_f2(src, ok ? (unsigned)(((size_t)row * nx + c) * sizeof(float2)) : OPH_BUF_OOB);
         }
     };
-    fetch(r);
-    for (; r < ny; r += gridDim.x) {
+    auto row = [&](int r) {
         float2 v[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m) v[m] = (m >= M0 && m < M1) ? nxt[m - M0] : make_float2(0.f, 0.f);
-        fetch(r + gridDim.x);
         __syncthreads();  // the previous row's last-stage reads are done
         fft::stage0_from_regs<L, -1>(v, buf, t);
+        fetch(r + gridDim.x);  // into the registers this row was read from
         __syncthreads();
-        // opaque copy of the table pointer: the twiddle loads stay in the
-        // loop instead of pinning every stage's twiddles in registers
-        const float2* twl = tw;
-        asm volatile("" : "+s"(twl));
-        fft::middle_stages_tw<L, -1>(buf, twl, t);
-        fft::last_stage_to_regs<L, -1>(buf, twl, t, v);
+        fft::middle_stages_tw<L, -1>(buf, twm, t);
+        fft::last_stage_to_regs_w<L, -1>(buf, wl, t, v);
         float2* dst = work + (size_t)(oy + r) * L;
 #pragma unroll
         for (int m = 0; m < 16; ++m) dst[t + m * TPL] = v[m];
+    };
+    fetch(r);
+    if (r < ny) {
+        row(r);
+        for (r += gridDim.x; r < ny; r += gridDim.x) row(r);
     }
 }
 
@@ -350,35 +369,50 @@ __global__ __launch_bounds__(L / 16) void asm_rows_inv_f(const float2* __restric
     constexpr int PL = fft::padded_len(L);
     constexpr int M0 = HALF ? 4 : 0, M1 = HALF ? 12 : 16;
     __shared__ float2 buf[PL];
+    constexpr int TWO = fft::last_stage_tw_offset<L>();
+    __shared__ float2 twm[TWO > 0 ? TWO : 1];
     const int t = threadIdx.x;
+    for (int i = t; i < TWO; i += TPL) twm[i] = tw[i];
+    // last-stage twiddles W^(r t) rebuilt per row from W^t and W^(4t) (two
+    // registers pairs instead of 15: four waves per SIMD)
+    const float2 w1 = tw[TWO + t], w4 = tw[TWO + 3 * TPL + t];
+    // rows oy .. oy + ny - 1 of the work grid and the output field, each at
+    // most L x L complex64 (128 MiB for L = 4096)
+    const __amdgpu_buffer_rsrc_t src =
+        oph_buf_rsrc(work + (size_t)oy * L, (unsigned)((size_t)ny * L * sizeof(float2)));
+    const __amdgpu_buffer_rsrc_t dst = oph_buf_rsrc(out, (unsigned)((size_t)nx * ny * sizeof(float2)));
     float2 nxt[16];
     int r = blockIdx.x;
     auto fetch = [&](int row) {
 #pragma unroll
         for (int m = 0; m < 16; ++m)
-            nxt[m] = row < ny ? work[(size_t)(oy + row) * L + t + m * TPL] : make_float2(0.f, 0.f);
+            nxt[m] = oph_buf_load_f2(
+                src, row < ny ? (unsigned)(((size_t)row * L + t + m * TPL) * sizeof(float2)) : OPH_BUF_OOB);
     };
-    fetch(r);
-    for (; r < ny; r += gridDim.x) {
+    auto row = [&](int r) {
         float2 v[16];
 #pragma unroll
         for (int m = 0; m < 16; ++m) v[m] = nxt[m];
-        fetch(r + gridDim.x);
         __syncthreads();
         fft::stage0_from_regs<L, +1>(v, buf, t);
+        fetch(r + gridDim.x);  // into the registers this row was read from
         __syncthreads();
-        // opaque copy of the table pointer: the twiddle loads stay in the
-        // loop instead of pinning every stage's twiddles in registers
-        const float2* twl = tw;
-        asm volatile("" : "+s"(twl));
-        fft::middle_stages_tw<L, +1>(buf, twl, t);
-        fft::last_stage_to_regs<L, +1>(buf, twl, t, v);
-        float2* dst = out + (size_t)r * nx;
+        fft::middle_stages_tw<L, +1>(buf, twm, t);
+        float2 wl[16];
+        fft::twiddle_powers(w1, w4, wl);
+        fft::last_stage_to_regs_w<L, +1>(buf, wl, t, v);
 #pragma unroll
         for (int m = M0; m < M1; ++m) {
             const int c = t + m * TPL - ox;
-            if (c >= 0 && c < nx) dst[c] = make_float2(v[m].x * scale, v[m].y * scale);
+            const bool ok = c >= 0 && c < nx;
+            oph_buf_store_f2(make_float2(v[m].x * scale, v[m].y * scale), dst,
+                             ok ? (unsigned)(((size_t)r * nx + c) * sizeof(float2)) : OPH_BUF_OOB);
         }
+    };
+    fetch(r);
+    if (r < ny) {
+        row(r);
+        for (r += gridDim.x; r < ny; r += gridDim.x) row(r);
     }
 }
 

@@ -31,14 +31,6 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 
-// Raw buffer resource over [p, p + bytes) (gfx9 word 3).  Buffer loads and
-// stores are range-checked by the hardware: an offset at or beyond `bytes`
-// reads zeros and writes nothing, so bounds are handled without branches
-// (and the compiler's wait counts stay exact across the tile loop).
-constexpr unsigned BUF_OOB = 0x80000000u;  // an offset beyond every buffer used here
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, unsigned bytes) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
-}
 
 struct oph_cnn {
     int blocks = 8, ch = 32;
@@ -709,7 +701,7 @@ __device__ __forceinline__ void rb_conv2_rows(const __bf16* in_t, const __bf16* 
         const int r = r0 + j;
         const int y = y0 + r;
         // pixels outside the image: offset out of range, store dropped
-        out.dst[j] = y < H && x < W ? (unsigned)((y * W + x) * 64 + 8 * kh) : BUF_OOB;
+        out.dst[j] = y < H && x < W ? (unsigned)((y * W + x) * 64 + 8 * kh) : OPH_BUF_OOB;
         const __bf16* res = in_t + ((r + 2) * RB_IW + px + 2) * RB_PS + 4 * kh;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -755,14 +747,14 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
     auto fetch = [&](int tile) {
         const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
         const int x0 = tx * RB_TW, y0 = ty * RB_TH;
-        const __amdgpu_buffer_rsrc_t rs = buf_rsrc(a.h + (size_t)b * a.H * a.W * 32, img_bytes);
+        const __amdgpu_buffer_rsrc_t rs = oph_buf_rsrc(a.h + (size_t)b * a.H * a.W * 32, img_bytes);
 #pragma unroll
         for (int n = 0; n < NPRE; ++n) {
             const int i = threadIdx.x + 256 * n;
             const int pix = i >> 2, q = i & 3;
             const int yy = y0 + pix / RB_IW - 2, xx = x0 + pix % RB_IW - 2;
             const bool ok = i < RB_IH * RB_IW * 4 && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-            const unsigned off = ok ? (unsigned)((yy * a.W + xx) * 64 + 16 * q) : BUF_OOB;
+            const unsigned off = ok ? (unsigned)((yy * a.W + xx) * 64 + 16 * q) : OPH_BUF_OOB;
             pre[n] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
         }
     };
@@ -771,7 +763,7 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
     // operation when the staging waits for it (the wait counter is in order)
     // and the stores have a whole tile to complete.
     RbOut pend;
-    __amdgpu_buffer_rsrc_t pend_img = buf_rsrc(a.out, img_bytes);
+    __amdgpu_buffer_rsrc_t pend_img = oph_buf_rsrc(a.out, img_bytes);
     if ((int)blockIdx.x < ntiles) fetch(blockIdx.x);
     for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
@@ -805,7 +797,7 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
 
         // conv2 + bias + residual on the 8 x 32 tile: rows 2 wave, 2 wave + 1
         rb_conv2_rows(in_t, mid_t, w2_s, bias_s, pend, 2 * wave_o, lane_o, x0, y0, a.H, a.W);
-        pend_img = buf_rsrc(a.out + img * 32, img_bytes);
+        pend_img = oph_buf_rsrc(a.out + img * 32, img_bytes);
     }
     if ((int)blockIdx.x < ntiles) rb_store(pend, pend_img);
 }

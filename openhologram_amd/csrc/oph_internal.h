@@ -123,6 +123,29 @@ __device__ __forceinline__ double rsqrt_f64(double s) {
 
 struct f2 { float x, y; };
 
+// Raw buffer resource over [p, p + bytes), bytes < 2^31 (gfx9 word 3).
+// Buffer loads and stores are range-checked by the hardware: an offset at
+// or beyond `bytes` reads zeros and writes nothing, so bounds are handled
+// without branches, which keeps the compiler's memory wait counts exact
+// across a loop (the counter is in order: a branch around a load or store
+// makes the next wait conservative).
+constexpr unsigned OPH_BUF_OOB = 0x80000000u;  // an offset beyond every buffer made here
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t oph_buf_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)bytes, 0x00020000);
+}
+// (the builtins' vectors are converted as a whole with __builtin_bit_cast;
+// subscripting the returned vector lost its second element here)
+typedef float oph_f32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned oph_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 oph_buf_load_f2(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    const oph_f32x2 v = __builtin_bit_cast(oph_f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void oph_buf_store_f2(float2 x, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    const oph_f32x2 v = {x.x, x.y};
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(oph_u32x2, v), r, off, 0, 0);
+}
+
 // Workgroup index remap for the 8 XCDs of MI355X: consecutive hardware ids go
 // round-robin over XCDs; this returns a logical id such that each XCD receives
 // a contiguous range of logical ids (neighbouring tiles then share that XCD's
