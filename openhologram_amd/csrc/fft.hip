@@ -3,7 +3,9 @@
 // 285-322, DM/Hologram/src/HologramKernel.cu:189-225).
 //
 // Row pass: one LDS line per L/16 threads, several lines per workgroup for
-// short rows; rows are read and written with unit stride.
+// short rows; rows are read and written with unit stride.  For L = 4096 a
+// persistent kernel loads the next row into registers while it transforms
+// the current one (first and last stages on registers).
 // Column pass: a workgroup stages B neighbouring columns (B*8-byte runs per
 // row) in LDS and transforms them together; workgroup ids are remapped so
 // neighbouring column groups run on the same XCD and share its L2.
@@ -100,6 +102,63 @@ __global__ __launch_bounds__(512) void fft_rows_kernel(float2* data, int rows, i
     }
 }
 
+// Persistent row transform for L = 16^3: thread t holds elements t + m L/16
+// in the first stage's inputs and in the last stage's outputs, so a row goes
+// from global memory straight into the first butterfly and from the last
+// butterfly straight back.  The next row is loaded (into the registers the
+// current row was read from) while the current one is transformed; the
+// middle stage's twiddles sit in LDS and the last stage's 15 per thread are
+// rebuilt from two powers, so no global load but the row's is issued in the
+// loop, and rows are range-checked buffer accesses without branches.  With
+// the first row peeled off the loop, the memory wait before each row's
+// first butterfly covers that row's loads only (the wait counter is in
+// order).  In place: each row is loaded and stored by one workgroup.
+template <int L, int SIGN>
+__global__ __launch_bounds__(L / 16) void fft_rows_f(float2* data, int rows, int stride, float scale,
+                                                     const float2* __restrict__ tw) {
+    static_assert(fft::regs_fusable<L>() && L / 16 >= 64, "L = 16^a, a >= 3");
+    constexpr int TPL = L / 16;
+    constexpr int PL = fft::padded_len(L);
+    constexpr int TWO = fft::last_stage_tw_offset<L>();
+    __shared__ float2 buf[PL];
+    __shared__ float2 twm[TWO];
+    const int t = threadIdx.x;
+    for (int i = t; i < TWO; i += TPL) twm[i] = tw[i];
+    const float2 w1 = tw[TWO + t], w4 = tw[TWO + 3 * TPL + t];
+    // the whole matrix, below 2^31 bytes (checked by the launcher)
+    const __amdgpu_buffer_rsrc_t R = oph_buf_rsrc(data, (unsigned)((size_t)rows * stride * sizeof(float2)));
+    float2 in[16];
+    auto fetch = [&](int row) {
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            in[m] = oph_buf_load_f2(
+                R, row < rows ? (unsigned)(((size_t)row * stride + t + m * TPL) * sizeof(float2)) : OPH_BUF_OOB);
+    };
+    auto row = [&](int r) {
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) v[m] = in[m];
+        __syncthreads();  // the previous row's last-stage reads are done
+        fft::stage0_from_regs<L, SIGN>(v, buf, t);
+        fetch(r + gridDim.x);
+        __syncthreads();
+        fft::middle_stages_tw<L, SIGN>(buf, twm, t);
+        float2 wl[16];
+        fft::twiddle_powers(w1, w4, wl);
+        fft::last_stage_to_regs_w<L, SIGN>(buf, wl, t, v);
+#pragma unroll
+        for (int m = 0; m < 16; ++m)
+            oph_buf_store_f2(make_float2(v[m].x * scale, v[m].y * scale), R,
+                             (unsigned)(((size_t)r * stride + t + m * TPL) * sizeof(float2)));
+    };
+    int r = blockIdx.x;
+    fetch(r);
+    if (r < rows) {
+        row(r);
+        for (r += gridDim.x; r < rows; r += gridDim.x) row(r);
+    }
+}
+
 template <int L, int SIGN, int B>
 __global__ __launch_bounds__(1024) void fft_cols_kernel(float2* data, int ncols, int stride, float scale,
                                                         const float2* __restrict__ tw) {
@@ -133,6 +192,24 @@ template <int L, int SIGN>
 int launch_rows(oph_ctx* ctx, float2* data, int rows, int stride, float scale) {
     const float2* tw = oph_twiddles(ctx, L);
     if (!tw) return OPH_ENOMEM;
+    if constexpr (fft::regs_fusable<L>() && L / 16 >= 64) {
+        if ((size_t)rows * stride * sizeof(float2) < (1u << 31)) {
+            // persistent: as many workgroups as are resident at once
+            static int per_cu = 0;
+            if (!per_cu) {
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                        &per_cu, reinterpret_cast<const void*>(&fft_rows_f<L, SIGN>), L / 16, 0) != hipSuccess ||
+                    per_cu < 1) {
+                    hipGetLastError();
+                    per_cu = 1;
+                }
+            }
+            const int grid = std::min(rows, per_cu * std::max(ctx->cu_count, 1));
+            OPH_LAUNCH(ctx, "fft_rows", (fft_rows_f<L, SIGN>), dim3(grid), dim3(L / 16), 0, data, rows, stride,
+                       scale, tw);
+            return OPH_OK;
+        }
+    }
     constexpr int LPW = fftk::row_lines(L);
     const int grid = (rows + LPW - 1) / LPW;
     OPH_LAUNCH(ctx, "fft_rows", (fft_rows_kernel<L, SIGN>), dim3(grid), dim3(LPW * (L / 16)), 0, data, rows,
