@@ -137,7 +137,7 @@ WRP_KERNELS = ["wrp_bin_count", "wrp_bin_colscan", "wrp_bin_fill", "wrp_gather",
 # FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); summary in
 # profiles/hbm_traffic_wrp_depthmap.md.  Valid for the kernels as built here;
 # re-measure after changing them.
-PROFILED_TRAFFIC = {"asm_cols": (66.4 + 73.4) * 2**20, "asm_rows_fwd": (32.4 + 64.1) * 2**20,
+PROFILED_TRAFFIC = {"asm_cols": (65.6 + 74.8) * 2**20, "asm_rows_fwd": (32.4 + 64.2) * 2**20,
                     "asm_rows_inv": (64.4 + 32.0) * 2**20}
 
 
@@ -454,8 +454,40 @@ def bench_learned(ctx, args):
     return line
 
 
+def bench_fft2d(ctx, args):
+    """In-place forward 2-D FFT of a 4096^2 complex64 array (oph_fft2d): a
+    row pass and a column pass, each reading and writing the whole array
+    (256 MiB per pass).  The roofline is each pass's algorithmic bytes over
+    its average launch time (HIP events on the library's stream)."""
+    n = 4096
+    rng = np.random.default_rng(1)
+    x = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64)
+    d = ctx.to_device(x)
+    wall, per = _time_steps(ctx, lambda: ctx.fft2d(d, device=True, asynchronous=True), args,
+                            ["fft_rows", "fft_cols"])
+    moved = 2 * n * n * 8
+    roof = {}
+    for k in ("fft_rows", "fft_cols"):
+        if k in per:
+            gbs = moved / (per[k]["avg_us"] * 1e-6) / 1e9
+            roof[k] = {"kernel": k, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": gbs / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": moved}
+    # accuracy against numpy's FFT (double precision) on the same array
+    f = ctx.fft2d(x)
+    t0 = time.perf_counter()
+    ref = np.fft.fft2(x.astype(np.complex128))
+    dt = time.perf_counter() - t0
+    line = {"metric": METRIC, "value": args.steps / wall, "unit": "transforms/s", "ms_per_step": wall / args.steps * 1e3,
+            "dtype": "f32", "config": {"workload": "fft2d_4096x4096_c64_inplace_forward"}, "kernels": per,
+            "roofline": roof.get("fft_cols"), "roofline_rows": roof.get("fft_rows"),
+            "rel_l2_vs_numpy_f64": rel_l2(f, ref),
+            "cpu_baseline": {"value": 1.0 / dt, "unit": "transforms/s", "cores": 1, "kind": "port",
+                             "sample": f"numpy.fft.fft2 in complex128 on the same 4096^2 array ({dt:.2f} s)"}}
+    return line
+
+
 WORKLOADS = {"rs": bench_rs, "wrp": bench_wrp, "depthmap": bench_depthmap, "fresnel": bench_fresnel,
-             "learned": bench_learned}
+             "learned": bench_learned, "fft2d": bench_fft2d}
 DEFAULT = "wrp"
 
 
