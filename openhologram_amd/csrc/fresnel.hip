@@ -240,17 +240,22 @@ __global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
     const double Xc = (j0 + gr - a.nx / 2) * a.p;
     const bool row_ok = i0 + gr < a.ny, col_ok = j0 + gr < a.nx;
 
+    // the next chunk's points are loaded into registers while the current
+    // chunk is generated and multiplied (the loop issues no other memory
+    // operation, so the wait for them falls at the next chunk's start)
+    FrPoint nq{};
+    if (tid < KCH) nq = a.pts[tid];
     for (int64_t k0 = 0; k0 < a.npad; k0 += KCH) {
         __syncthreads();
         if (tid < KCH) {
-            const FrPoint q = a.pts[k0 + tid];
-            p_x[tid] = q.x;
-            p_y[tid] = q.y;
-            p_c[tid] = q.c;
-            p_wr[tid] = q.wre;
-            p_wi[tid] = q.wim;
+            p_x[tid] = nq.x;
+            p_y[tid] = nq.y;
+            p_c[tid] = nq.c;
+            p_wr[tid] = nq.wre;
+            p_wi[tid] = nq.wim;
         }
         __syncthreads();
+        if (tid < KCH && k0 + KCH < a.npad) nq = a.pts[k0 + KCH + tid];
         // KCH/2 points of this thread's row/column, written as 8-element vectors
 #pragma unroll
         for (int hb = 0; hb < KCH / 16; ++hb) {
