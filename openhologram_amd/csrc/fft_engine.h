@@ -330,36 +330,6 @@ __device__ __forceinline__ void middle_stages_tw(float2* buf, const float2* __re
     }
 }
 
-// Middle stages and last stage with all twiddles already in registers
-// (Twiddles<L>, loaded once by a persistent kernel), so that no global load
-// is issued inside the transform.
-template <int L, int SIGN>
-__device__ __forceinline__ void middle_stages_reg(float2* buf, const Twiddles<L>& Tw, int t) {
-    constexpr int TPL = L / 16;
-    constexpr int A = n16(L);
-    int Ns = 16;
-#pragma unroll
-    for (int s = 1; s < A - 1; ++s) {
-        stage<L, 16, SIGN, TPL>(buf, Tw.w16[s - 1], Ns, t);
-        Ns *= 16;
-    }
-}
-
-template <int L, int SIGN>
-__device__ __forceinline__ void last_stage_to_regs_reg(const float2* buf, const Twiddles<L>& Tw, int t,
-                                                       float2 (&v)[16]) {
-    static_assert(regs_fusable<L>(), "length must be a power of 16");
-    constexpr int TPL = L / 16;
-    constexpr int A = n16(L);
-    const float2* src = buf + pad16(t);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = src[r * (TPL + TPL / 16)];
-#pragma unroll
-    for (int r = 1; r < 16; ++r)
-        v[r] = SIGN < 0 ? cmul(v[r], Tw.w16[A - 2][0][r]) : cmulc(v[r], Tw.w16[A - 2][0][r]);
-    Dft<16, SIGN>::run(v);
-}
-
 // Last stage with the thread's 15 twiddles (k = t, Ns = L/16) in wl[1..15].
 template <int L, int SIGN>
 __device__ __forceinline__ void last_stage_to_regs_w(const float2* buf, const float2 (&wl)[16], int t,
@@ -401,30 +371,6 @@ __host__ __device__ constexpr int last_stage_tw_offset() {
         Ns *= 16;
     }
     return off;
-}
-
-// Last radix-16 stage (Ns = L/16) read from the line into v: on return v[r]
-// is output element t + r L/16.  Callers sync before reusing the line.
-template <int L, int SIGN>
-__device__ __forceinline__ void last_stage_to_regs(const float2* buf, const float2* __restrict__ tw, int t,
-                                                   float2 (&v)[16]) {
-    static_assert(regs_fusable<L>(), "length must be a power of 16");
-    constexpr int TPL = L / 16;
-    constexpr int A = n16(L);
-    int off = 0, Ns = 16;
-#pragma unroll
-    for (int s = 1; s < A - 1; ++s) {
-        off += 15 * Ns;
-        Ns *= 16;
-    }
-    const float2* src = buf + pad16(t);
-#pragma unroll
-    for (int r = 0; r < 16; ++r) v[r] = src[r * (TPL + TPL / 16)];
-    float2 w[1][16];
-    load_stage_tw<L, 16, TPL>(tw, off, Ns, t, w);
-#pragma unroll
-    for (int r = 1; r < 16; ++r) v[r] = SIGN < 0 ? cmul(v[r], w[0][r]) : cmulc(v[r], w[0][r]);
-    Dft<16, SIGN>::run(v);
 }
 
 // Radix-8 variant: L/8 threads per line, each holding 8 elements
