@@ -66,6 +66,27 @@ def test_cnn_forward_bf16_activations(ctx):
     assert _rel(yh - x, yrh - x) < 2e-2
 
 
+def test_cnn_bf16_activations_many_tiles(ctx):
+    """Fused residual blocks on more 8 x 32 tiles than resident workgroups
+    (each workgroup takes several tiles, storing a tile's outputs after the
+    next tile's staging), ragged width, batch 2: against the float32 form,
+    which the oracle tests pin."""
+    print("case: cnn bf16 activations, 2 x 256 x 520 (1088 tiles)", flush=True)
+    rng = np.random.default_rng(7)
+    net = oph.lib.Cnn(ctx, blocks=2, channels=32, seed=9)
+    net.set_params(rng.normal(0, 0.05, net.n).astype(np.float32))
+    x = rng.normal(0, 1, (2, 256, 520, 2)).astype(np.float32)
+    y32 = net.forward(x)
+    y16 = net.forward(x, bf16_acts=True)
+    e = _rel(y16 - x, y32 - x)
+    print("bf16-act forward rel vs float32 form", e)
+    assert e < 1e-2
+    # per image and per edge region, so a misplaced tile cannot hide in the norm
+    for b in range(2):
+        for sl in (np.s_[:, :8, :], np.s_[:, -8:, :], np.s_[:, :, -8:]):
+            assert _rel((y16 - x)[b][sl[1:]], (y32 - x)[b][sl[1:]]) < 2e-2
+
+
 def test_cnn_training_reduces_loss_and_persists(ctx, tmp_path):
     print("case: cnn training", flush=True)
     rng = np.random.default_rng(2)
