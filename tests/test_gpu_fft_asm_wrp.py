@@ -75,6 +75,23 @@ def test_asm_4096_rows(ctx, ny, nx, pad):
     assert err < 1e-5
 
 
+def test_asm_many_rows_per_workgroup(ctx):
+    """2048^2 field on the 4096^2 grid: the persistent row passes take
+    several rows per workgroup (next row prefetched across the transform)
+    and the column pass runs its full grid; against the FP64 oracle."""
+    print("case: asm 2048x2048 pad 2", flush=True)
+    rng = np.random.default_rng(21)
+    u = (rng.standard_normal((2048, 2048)) + 1j * rng.standard_normal((2048, 2048))).astype(np.complex64)
+    out = ctx.asm(u, 8e-6, 532e-9, 0.1, pad=2)
+    ref = O.asm(u.astype(np.complex128), 8e-6, 8e-6, 532e-9, 0.1, pad=2)
+    err = rel_l2(out, ref)
+    print("rel-L2", err)
+    assert err < 1e-5
+    # rows far apart in the grid (the first and last rows of each workgroup's walk)
+    for rows in (np.s_[:64], np.s_[-64:], np.s_[1000:1100]):
+        assert rel_l2(out[rows], ref[rows]) < 1e-5
+
+
 def test_wrp_many_points_and_linearity(ctx):
     """300,000 points: binning chunks of two points per thread; against the
     oracle, and the sum of two halves (accumulate) equals the whole."""
