@@ -419,6 +419,10 @@ __global__ __launch_bounds__(L / 16) void asm_rows_inv_f(const float2* __restric
 // resident workgroups per CU of the persistent row passes (register-bound
 // occupancy of each kernel, 4 waves per workgroup)
 constexpr int ROWS_FWD_PER_CU = 2, ROWS_INV_PER_CU = 3;
+// workgroups per CU of the register-fused row passes: at most 3, measured
+// faster than the occupancy maximum of 4 (2048 rows of 4096: forward 17.4
+// against 19.0 us, inverse 18.9 against 19.2 us; 2 per CU: 19.0 and 20.1 us)
+constexpr int ROWS_F_MAX_PER_CU = 3;
 
 // Workgroups of a persistent kernel resident per CU, from the runtime's
 // occupancy calculator (registers and LDS of the compiled kernel).
@@ -442,8 +446,8 @@ int launch_rows_fwd(oph_ctx* ctx, const float2* in, int nx, int ny, int ox, int 
     if constexpr (LPW == 1) {
         const int g = std::min(ny, ROWS_FWD_PER_CU * std::max(ctx->cu_count, 1));
         if constexpr (fft::regs_fusable<LX>()) {
-            static const int occ_h = resident_per_cu(asm_rows_fwd_f<LX, true>, LX / 16);
-            static const int occ_f = resident_per_cu(asm_rows_fwd_f<LX, false>, LX / 16);
+            static const int occ_h = std::min(ROWS_F_MAX_PER_CU, resident_per_cu(asm_rows_fwd_f<LX, true>, LX / 16));
+            static const int occ_f = std::min(ROWS_F_MAX_PER_CU, resident_per_cu(asm_rows_fwd_f<LX, false>, LX / 16));
             const int cus = std::max(ctx->cu_count, 1);
             if (ox >= LX / 4 && ox + nx <= 3 * LX / 4)
                 OPH_LAUNCH(ctx, "asm_rows_fwd", (asm_rows_fwd_f<LX, true>), dim3(std::min(ny, occ_h * cus)),
@@ -470,8 +474,8 @@ int launch_rows_inv(oph_ctx* ctx, const float2* work, int nx, int ny, int ox, in
     if constexpr (LPW == 1) {
         const int g = std::min(ny, ROWS_INV_PER_CU * std::max(ctx->cu_count, 1));
         if constexpr (fft::regs_fusable<LX>()) {
-            static const int occ_h = resident_per_cu(asm_rows_inv_f<LX, true>, LX / 16);
-            static const int occ_f = resident_per_cu(asm_rows_inv_f<LX, false>, LX / 16);
+            static const int occ_h = std::min(ROWS_F_MAX_PER_CU, resident_per_cu(asm_rows_inv_f<LX, true>, LX / 16));
+            static const int occ_f = std::min(ROWS_F_MAX_PER_CU, resident_per_cu(asm_rows_inv_f<LX, false>, LX / 16));
             const int cus = std::max(ctx->cu_count, 1);
             if (ox >= LX / 4 && ox + nx <= 3 * LX / 4)
                 OPH_LAUNCH(ctx, "asm_rows_inv", (asm_rows_inv_f<LX, true>), dim3(std::min(ny, occ_h * cus)),
