@@ -268,8 +268,9 @@ int oph_cnn_set_params(oph_ctx* ctx, oph_cnn* net, const float* host);
 /* flags: OPH_DEVICE_PTR, OPH_ASYNC, and OPH_CNN_BF16_ACTS: keep the
  * activations between layers in BF16 (inference; the convolution operands
  * are BF16-rounded in both forms, this also rounds the stored intermediate
- * and residual tensors, halving their HBM traffic).  Training always uses
- * float32 activations. */
+ * and residual tensors, halving their HBM traffic; each residual block is one
+ * kernel, for images below 2^25 pixels, OPH_ENOTSUP above).  Training always
+ * uses float32 activations. */
 #define OPH_CNN_BF16_ACTS 0x1000
 int oph_cnn_forward(oph_ctx* ctx, oph_cnn* net, const float* x, float* y, int batch, int h, int w,
                     int flags);
