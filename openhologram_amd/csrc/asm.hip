@@ -76,19 +76,22 @@ struct AsmConst {
 };
 
 // exp(i 2 pi (c0 + tx + ty - K poly(u))) or 0; EXACT: z/lambda sqrt(1-u) in FP64
-// (ay: the y entry's phase and lambda^2 f^2; its band-limit term is read
-// from the full table only when the band limit is on)
-template <bool EXACT>
+// (ay: the y entry's phase and lambda^2 f^2; BAND: the band-limit term of the
+// y entry is read from the full table).  Branch-free: the phase is evaluated
+// everywhere and the zero selected afterwards, so a caller's unrolled loop
+// is one basic block whose table loads can all be in flight together.
+template <bool EXACT, bool BAND>
 __device__ __forceinline__ float2 transfer(const float4 ax, const float2 ay, const float4* __restrict__ tyfull, int c,
                                            int i, const AsmConst& k) {
     const float u = ax.y + ay.y;
-    if (u >= 1.f) return make_float2(0.f, 0.f);
-    if (k.band) {
+    bool zero = u >= 1.f;
+    if constexpr (BAND) {
         const float s = 1.f - u;
-        if (ax.z > s || tyfull[i].z > s) return make_float2(0.f, 0.f);
+        const float bz = tyfull[i].z;  // (loaded unconditionally: no branch around the load)
+        zero = zero | (ax.z > s) | (bz > s);
     }
     float ph;
-    if (!EXACT) {
+    if constexpr (!EXACT) {
         const float p = u * u * fmaf(u, fmaf(u, fmaf(u, 7.f / 256.f, 5.f / 128.f), 1.f / 16.f), 0.125f);
         ph = k.c0 + ax.x + ay.x - k.K * p;
     } else {
@@ -96,11 +99,26 @@ __device__ __forceinline__ float2 transfer(const float4 ax, const float2 ay, con
         const int ky = i < k.npy / 2 ? i : i - k.npy;
         const double fx = kx * k.dfx, fy = ky * k.dfy;
         const double uu = k.lam2 * (fx * fx + fy * fy);
-        if (uu >= 1.0) return make_float2(0.f, 0.f);
-        const double t = k.zl * sqrt(1.0 - uu);
+        zero = zero | (uu >= 1.0);
+        const double t = k.zl * sqrt(fmax(1.0 - uu, 0.0));
         ph = (float)(t - rint(t));
     }
-    return make_float2(cos_turns(ph), sin_turns(ph));
+    const float cs = cos_turns(ph), sn = sin_turns(ph);
+    return zero ? make_float2(0.f, 0.f) : make_float2(cs, sn);
+}
+
+// Multiplies N of the thread's column elements, v[j] = row t + (m0 + j) L/16,
+// by the transfer function; the N table entries are loaded before any is used.
+template <int L, bool EXACT, bool BAND, int N>
+__device__ __forceinline__ void apply_transfer(float2 (&v)[N], const float4 ax, const float2* __restrict__ taby2,
+                                               const float4* __restrict__ taby, int c, int t, int m0,
+                                               const AsmConst& k) {
+    float2 ay[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) ay[j] = taby2[t + (m0 + j) * (L / 16)];
+#pragma unroll
+    for (int j = 0; j < N; ++j)
+        v[j] = fft::cmul(v[j], transfer<EXACT, BAND>(ax, ay[j], taby, c, t + (m0 + j) * (L / 16), k));
 }
 
 template <int L>
@@ -133,6 +151,10 @@ __global__ __launch_bounds__(512) void asm_rows_fwd(const float2* __restrict__ i
     }
 }
 
+// Column lengths whose transform is register-fused around the transfer.
+template <int L>
+__host__ __device__ constexpr bool asm_cols_fused() { return fft::regs_fusable<L>() && fft::n16(L) == 3; }
+
 template <int L, int B, bool EXACT>
 __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, int oy,
                                                  const float4* __restrict__ tabx,
@@ -145,8 +167,19 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
     const int nthr = TPL * B;
     const int g = xcd_remap(blockIdx.x, gridDim.x);
     const int cbase = g * B;
-    fft::Twiddles<L> T;
-    T.load(tw, threadIdx.x % TPL);
+    // 4096-point columns (three radix-16 stages): the middle stage's table
+    // (240 entries, k = t mod 16) in LDS before the lines, the last stage's 15
+    // twiddles (k = t) read from the table just before each use; other
+    // lengths hold every stage's twiddles in registers
+    constexpr bool FUSED = asm_cols_fused<L>();
+    fft::Twiddles<FUSED ? 16 : L> T;
+    float2* tw1 = lds;  // first, so every read of it is one address plus an immediate
+    float2* lines = lds + (FUSED ? 256 : 0);
+    if constexpr (FUSED) {
+        for (int e = threadIdx.x; e < 15 * 16; e += nthr) tw1[e] = tw[e];
+    } else {
+        T.load(tw, threadIdx.x % TPL);
+    }
     if constexpr (B % 2 == 0) {
         // two neighbouring columns per lane (16-byte accesses); npx is a
         // power of two >= B, so every column of the group exists
@@ -155,44 +188,77 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
             if (i >= oy && i < oy + ny)
                 v = *reinterpret_cast<const float4*>(work + (size_t)i * npx + cbase + 2 * pr);
-            lds[(2 * pr) * PL + fft::pad16(i)] = make_float2(v.x, v.y);
-            lds[(2 * pr + 1) * PL + fft::pad16(i)] = make_float2(v.z, v.w);
+            lines[(2 * pr) * PL + fft::pad16(i)] = make_float2(v.x, v.y);
+            lines[(2 * pr + 1) * PL + fft::pad16(i)] = make_float2(v.z, v.w);
         }
     } else {
         for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
             const int b = idx % B, i = idx / B;
             const int c = cbase + b;
             const bool in_rows = i >= oy && i < oy + ny;
-            lds[b * PL + fft::pad16(i)] =
+            lines[b * PL + fft::pad16(i)] =
                 (in_rows && c < npx) ? work[(size_t)i * npx + c] : make_float2(0.f, 0.f);
         }
     }
     __syncthreads();
     const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
-    float2* line = lds + b * PL;
-    fft::line<L, -1>(line, T, t);
+    float2* line = lines + b * PL;
     const int c = cbase + b;
     const float4 ax = tabx[c < npx ? c : 0];
+    if constexpr (FUSED) {
+        // the forward transform's last butterfly leaves elements t + m L/16
+        // in registers, which are the inverse's first butterfly's inputs:
+        // the transfer is applied in between, without a pass through LDS
+        float2 w[1][16];
+        fft::stage<L, 16, -1, TPL>(line, w, 1, t);
 #pragma unroll
-    for (int m = 0; m < 16; ++m) {
-        const int i = t + m * TPL;
-        const float2 h = transfer<EXACT>(ax, taby2[i], taby, c, i, k);
-        float2* e = line + fft::lane_pad<L>(t, m);
-        *e = fft::cmul(*e, h);
+        for (int r = 1; r < 16; ++r) w[0][r] = tw1[(r - 1) * 16 + (t & 15)];
+        fft::stage<L, 16, -1, TPL>(line, w, 16, t);
+        float2 v[16];
+        // (buffer loads: one lane offset, the per-r part in the instruction)
+        const auto twl = oph_buf_rsrc(tw + fft::last_stage_tw_offset<L>(), 15 * TPL * sizeof(float2));
+#pragma unroll
+        for (int r = 1; r < 16; ++r) w[0][r] = oph_buf_load_f2(twl, t * sizeof(float2), (r - 1) * TPL * sizeof(float2));
+        fft::last_stage_to_regs_w<L, -1>(line, w[0], t, v);
+        if (k.band) apply_transfer<L, EXACT, true>(v, ax, taby2, taby, c, t, 0, k);
+        else apply_transfer<L, EXACT, false>(v, ax, taby2, taby, c, t, 0, k);
+        __syncthreads();  // every lane has read the line
+        fft::stage0_from_regs<L, +1>(v, line, t);
+        __syncthreads();
+#pragma unroll
+        for (int r = 1; r < 16; ++r) w[0][r] = tw1[(r - 1) * 16 + (t & 15)];
+        fft::stage<L, 16, +1, TPL>(line, w, 16, t);
+#pragma unroll
+        for (int r = 1; r < 16; ++r) w[0][r] = oph_buf_load_f2(twl, t * sizeof(float2), (r - 1) * TPL * sizeof(float2));
+        fft::stage<L, 16, +1, TPL>(line, w, 16 * 16, t);
+    } else {
+        fft::line<L, -1>(line, T, t);
+        // (a few rows at a time: every stage's twiddles stay in registers)
+        constexpr int CH = L > 4096 ? 2 : 4;
+#pragma unroll
+        for (int m0 = 0; m0 < 16; m0 += CH) {
+            float2 v[CH];
+#pragma unroll
+            for (int j = 0; j < CH; ++j) v[j] = line[fft::lane_pad<L>(t, m0 + j)];
+            if (k.band) apply_transfer<L, EXACT, true>(v, ax, taby2, taby, c, t, m0, k);
+            else apply_transfer<L, EXACT, false>(v, ax, taby2, taby, c, t, m0, k);
+#pragma unroll
+            for (int j = 0; j < CH; ++j) line[fft::lane_pad<L>(t, m0 + j)] = v[j];
+        }
+        __syncthreads();
+        fft::line<L, +1>(line, T, t);
     }
-    __syncthreads();
-    fft::line<L, +1>(line, T, t);
     if constexpr (B % 2 == 0) {
         for (int idx = threadIdx.x; idx < ny * B / 2; idx += nthr) {
             const int pr = idx % (B / 2), i = oy + idx / (B / 2);
-            const float2 u = lds[(2 * pr) * PL + fft::pad16(i)], v = lds[(2 * pr + 1) * PL + fft::pad16(i)];
+            const float2 u = lines[(2 * pr) * PL + fft::pad16(i)], v = lines[(2 * pr + 1) * PL + fft::pad16(i)];
             *reinterpret_cast<float4*>(work + (size_t)i * npx + cbase + 2 * pr) = make_float4(u.x, u.y, v.x, v.y);
         }
     } else {
         for (int idx = threadIdx.x; idx < ny * B; idx += nthr) {
             const int bb = idx % B, i = oy + idx / B;
             const int cc = cbase + bb;
-            if (cc < npx) work[(size_t)i * npx + cc] = lds[bb * PL + fft::pad16(i)];
+            if (cc < npx) work[(size_t)i * npx + cc] = lines[bb * PL + fft::pad16(i)];
         }
     }
 }
@@ -497,7 +563,7 @@ int launch_rows_inv(oph_ctx* ctx, const float2* work, int nx, int ny, int ox, in
 template <int LY, int B, bool EXACT>
 int launch_cols_b(oph_ctx* ctx, float2* work, int npx, int ny, int oy, const float4* tx, const float4* ty,
                   const AsmConst& k, const float2* tw) {
-    const size_t shm = (size_t)B * fft::padded_len(LY) * sizeof(float2);
+    const size_t shm = ((size_t)B * fft::padded_len(LY) + (asm_cols_fused<LY>() ? 256 : 0)) * sizeof(float2);
     static bool attr = false;
     if (!attr) {
         hipFuncSetAttribute(reinterpret_cast<const void*>(&asm_cols<LY, B, EXACT>),

@@ -141,6 +141,11 @@ __device__ __forceinline__ float2 oph_buf_load_f2(__amdgpu_buffer_rsrc_t r, unsi
     const oph_f32x2 v = __builtin_bit_cast(oph_f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
     return make_float2(v.x, v.y);
 }
+// (soff: a wave-uniform byte offset, kept in a scalar register)
+__device__ __forceinline__ float2 oph_buf_load_f2(__amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+    const oph_f32x2 v = __builtin_bit_cast(oph_f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
+    return make_float2(v.x, v.y);
+}
 __device__ __forceinline__ void oph_buf_store_f2(float2 x, __amdgpu_buffer_rsrc_t r, unsigned off) {
     const oph_f32x2 v = {x.x, x.y};
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(oph_u32x2, v), r, off, 0, 0);

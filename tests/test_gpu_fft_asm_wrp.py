@@ -75,6 +75,26 @@ def test_asm_4096_rows(ctx, ny, nx, pad):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("nx,pitch,lam,z,band", [
+    (64, 8e-6, 633e-9, 0.1, True),          # band limit
+    (32, 1e-6, 532e-9, 0.002, False),       # FP64 transfer
+    (32, 1e-6, 532e-9, 0.002, True),        # FP64 transfer and band limit
+    (32, 0.35e-6, 532e-9, 0.0005, False),   # evanescent corners (transfer zero)
+])
+def test_asm_4096_columns(ctx, nx, pitch, lam, z, band):
+    """2048-row fields: 4096-point column transforms with the transfer applied
+    between the forward and inverse butterflies in registers, in every
+    transfer form."""
+    print(f"case: asm 2048x{nx} p={pitch} z={z} band={band}", flush=True)
+    rng = np.random.default_rng(nx + int(band))
+    u = (rng.standard_normal((2048, nx)) + 1j * rng.standard_normal((2048, nx))).astype(np.complex64)
+    out = ctx.asm(u, pitch, lam, z, pad=2, band_limit=band)
+    ref = O.asm(u.astype(np.complex128), pitch, pitch, lam, z, pad=2, band_limit=band)
+    err = rel_l2(out, ref)
+    print("rel-L2", err)
+    assert err < 1e-5
+
+
 def test_asm_many_rows_per_workgroup(ctx):
     """2048^2 field on the 4096^2 grid: the persistent row passes take
     several rows per workgroup (next row prefetched across the transform)
