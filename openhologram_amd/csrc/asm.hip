@@ -175,14 +175,33 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
     fft::Twiddles<FUSED ? 16 : L> T;
     float2* tw1 = lds;  // first, so every read of it is one address plus an immediate
     float2* lines = lds + (FUSED ? 256 : 0);
-    if constexpr (FUSED) {
-        for (int e = threadIdx.x; e < 15 * 16; e += nthr) tw1[e] = tw[e];
-    } else {
-        T.load(tw, threadIdx.x % TPL);
-    }
-    if constexpr (B % 2 == 0) {
+    if constexpr (!FUSED) T.load(tw, threadIdx.x % TPL);
+    if constexpr (B % 2 == 0 && (L <= 1024 || FUSED)) {
         // two neighbouring columns per lane (16-byte accesses); npx is a
-        // power of two >= B, so every column of the group exists
+        // power of two >= B, so every column of the group exists.  All 8
+        // loads of a lane are issued before any is used: range-checked
+        // buffer loads, rows outside [oy, oy + ny) at an out-of-range offset
+        // read zeros (the grid is at most 8192^2 x 8 bytes, offsets fit).
+        // (2048 and 8192 keep the loop below: their twiddles leave no room
+        // for 8 loads in flight)
+        constexpr int PER = (L * B / 2) / (TPL * B);
+        const auto rs = oph_buf_rsrc(work, (unsigned)((size_t)(oy + ny) * npx * sizeof(float2)));
+        float4 ld[PER];
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int idx = threadIdx.x + j * nthr;
+            const int pr = idx % (B / 2), i = idx / (B / 2);
+            const bool in = i >= oy && i < oy + ny;
+            ld[j] = oph_buf_load_f4(rs, in ? (unsigned)(((size_t)i * npx + cbase + 2 * pr) * sizeof(float2)) : OPH_BUF_OOB);
+        }
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int idx = threadIdx.x + j * nthr;
+            const int pr = idx % (B / 2), i = idx / (B / 2);
+            lines[(2 * pr) * PL + fft::pad16(i)] = make_float2(ld[j].x, ld[j].y);
+            lines[(2 * pr + 1) * PL + fft::pad16(i)] = make_float2(ld[j].z, ld[j].w);
+        }
+    } else if constexpr (B % 2 == 0) {
         for (int idx = threadIdx.x; idx < L * B / 2; idx += nthr) {
             const int pr = idx % (B / 2), i = idx / (B / 2);
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -199,6 +218,10 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
             lines[b * PL + fft::pad16(i)] =
                 (in_rows && c < npx) ? work[(size_t)i * npx + c] : make_float2(0.f, 0.f);
         }
+    }
+    if constexpr (FUSED) {
+        // (after the column loads, so that its loads do not hold them up)
+        for (int e = threadIdx.x; e < 15 * 16; e += nthr) tw1[e] = tw[e];
     }
     __syncthreads();
     const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;

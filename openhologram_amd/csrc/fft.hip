@@ -159,31 +159,71 @@ __global__ __launch_bounds__(L / 16) void fft_rows_f(float2* data, int rows, int
     }
 }
 
+// wide: every column of the workgroup exists, the stride is even and the
+// L x stride array is below 2 GiB; then each lane moves two neighbouring
+// columns of a row (16 bytes) with range-checked buffer accesses, and its 8
+// loads are all issued before the first is used (a load loop with a bounds
+// branch completes each load before issuing the next).
 template <int L, int SIGN, int B>
 __global__ __launch_bounds__(1024) void fft_cols_kernel(float2* data, int ncols, int stride, float scale,
-                                                        const float2* __restrict__ tw) {
+                                                        const float2* __restrict__ tw, int wide) {
     constexpr int TPL = L / 16;
     constexpr int PL = fft::padded_len(L);
     extern __shared__ float2 lds[];
     const int nthr = TPL * B;
     const int g = xcd_remap(blockIdx.x, gridDim.x);
     const int c0 = g * B;
+    constexpr int PER = (L * B / 2) / (TPL * B);  // 16-byte pieces per lane
+    const bool w2 = B % 2 == 0 && wide && c0 + B <= ncols;
+    const auto rs = oph_buf_rsrc(data, (unsigned)((size_t)L * stride * sizeof(float2)));
+    if constexpr (B % 2 == 0 && L <= 4096) {
+        if (w2) {
+            float4 ld[PER];
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int idx = threadIdx.x + j * nthr;
+                const int pr = idx % (B / 2), i = idx / (B / 2);
+                ld[j] = oph_buf_load_f4(rs, (unsigned)(((size_t)i * stride + c0 + 2 * pr) * sizeof(float2)));
+            }
+#pragma unroll
+            for (int j = 0; j < PER; ++j) {
+                const int idx = threadIdx.x + j * nthr;
+                const int pr = idx % (B / 2), i = idx / (B / 2);
+                lds[(2 * pr) * PL + fft::pad16(i)] = make_float2(ld[j].x, ld[j].y);
+                lds[(2 * pr + 1) * PL + fft::pad16(i)] = make_float2(ld[j].z, ld[j].w);
+            }
+        }
+    }
+    if (!w2 || !(B % 2 == 0 && L <= 4096)) {
+        for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
+            const int b = idx % B, i = idx / B;
+            const int c = c0 + b;
+            lds[b * PL + fft::pad16(i)] = c < ncols ? data[(size_t)i * stride + c] : make_float2(0.f, 0.f);
+        }
+    }
+    // (twiddles after the column data: their loads do not hold up the data's)
     fft::Twiddles<L> T;
     T.load(tw, threadIdx.x % TPL);
-    for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
-        const int b = idx % B, i = idx / B;
-        const int c = c0 + b;
-        lds[b * PL + fft::pad16(i)] = c < ncols ? data[(size_t)i * stride + c] : make_float2(0.f, 0.f);
-    }
     __syncthreads();
     const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
     fft::line<L, SIGN>(lds + b * PL, T, t);
-    for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
-        const int bb = idx % B, i = idx / B;
-        const int c = c0 + bb;
-        if (c < ncols) {
-            const float2 v = lds[bb * PL + fft::pad16(i)];
-            data[(size_t)i * stride + c] = make_float2(v.x * scale, v.y * scale);
+    if (w2) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int idx = threadIdx.x + j * nthr;
+            const int pr = idx % (B / 2), i = idx / (B / 2);
+            const float2 u = lds[(2 * pr) * PL + fft::pad16(i)], v = lds[(2 * pr + 1) * PL + fft::pad16(i)];
+            oph_buf_store_f4(make_float4(u.x * scale, u.y * scale, v.x * scale, v.y * scale), rs,
+                             (unsigned)(((size_t)i * stride + c0 + 2 * pr) * sizeof(float2)));
+        }
+    } else {
+        for (int idx = threadIdx.x; idx < L * B; idx += nthr) {
+            const int bb = idx % B, i = idx / B;
+            const int c = c0 + bb;
+            if (c < ncols) {
+                const float2 v = lds[bb * PL + fft::pad16(i)];
+                data[(size_t)i * stride + c] = make_float2(v.x * scale, v.y * scale);
+            }
         }
     }
 }
@@ -219,11 +259,8 @@ int launch_rows(oph_ctx* ctx, float2* data, int rows, int stride, float scale) {
     return OPH_OK;
 }
 
-template <int L, int SIGN>
-int launch_cols(oph_ctx* ctx, float2* data, int ncols, int stride, float scale) {
-    const float2* tw = oph_twiddles(ctx, L);
-    if (!tw) return OPH_ENOMEM;
-    constexpr int B = fftk::col_lines(L);
+template <int L, int SIGN, int B>
+int launch_cols_b(oph_ctx* ctx, float2* data, int ncols, int stride, float scale, const float2* tw) {
     const size_t shm = (size_t)B * fft::padded_len(L) * sizeof(float2);
     static bool attr = false;
     if (!attr) {
@@ -233,9 +270,36 @@ int launch_cols(oph_ctx* ctx, float2* data, int ncols, int stride, float scale) 
         attr = true;
     }
     const int grid = (ncols + B - 1) / B;
+    const int wide = stride % 2 == 0 && (size_t)L * stride * sizeof(float2) < (1u << 31);
     OPH_LAUNCH(ctx, "fft_cols", (fft_cols_kernel<L, SIGN, B>), dim3(grid), dim3(B * (L / 16)), shm, data,
-               ncols, stride, scale, tw);
+               ncols, stride, scale, tw, wide);
     return OPH_OK;
+}
+
+template <int L, int SIGN>
+int launch_cols(oph_ctx* ctx, float2* data, int ncols, int stride, float scale) {
+    const float2* tw = oph_twiddles(ctx, L);
+    if (!tw) return OPH_ENOMEM;
+    constexpr int BMAX = fftk::col_lines(L);
+    static int b = 0;
+    if (!b) {
+        const char* e = getenv("OPH_FFT_COLUMNS");
+        b = e ? atoi(e) : BMAX;
+        if (b > BMAX || b < 1 || (b & (b - 1))) b = BMAX;
+    }
+    switch (b) {
+        case 1: return launch_cols_b<L, SIGN, 1>(ctx, data, ncols, stride, scale, tw);
+        case 2:
+            if constexpr (BMAX >= 2) return launch_cols_b<L, SIGN, 2>(ctx, data, ncols, stride, scale, tw);
+            break;
+        case 4:
+            if constexpr (BMAX >= 4) return launch_cols_b<L, SIGN, 4>(ctx, data, ncols, stride, scale, tw);
+            break;
+        case 8:
+            if constexpr (BMAX >= 8) return launch_cols_b<L, SIGN, 8>(ctx, data, ncols, stride, scale, tw);
+            break;
+    }
+    return launch_cols_b<L, SIGN, BMAX>(ctx, data, ncols, stride, scale, tw);
 }
 
 template <int SIGN>

@@ -146,6 +146,16 @@ __device__ __forceinline__ float2 oph_buf_load_f2(__amdgpu_buffer_rsrc_t r, unsi
     const oph_f32x2 v = __builtin_bit_cast(oph_f32x2, __builtin_amdgcn_raw_buffer_load_b64(r, off, soff, 0));
     return make_float2(v.x, v.y);
 }
+typedef float oph_f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned oph_u32x4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 oph_buf_load_f4(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    const oph_f32x4 v = __builtin_bit_cast(oph_f32x4, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ void oph_buf_store_f4(float4 x, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    const oph_f32x4 v = {x.x, x.y, x.z, x.w};
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(oph_u32x4, v), r, off, 0, 0);
+}
 __device__ __forceinline__ void oph_buf_store_f2(float2 x, __amdgpu_buffer_rsrc_t r, unsigned off) {
     const oph_f32x2 v = {x.x, x.y};
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(oph_u32x2, v), r, off, 0, 0);
