@@ -27,7 +27,7 @@ VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9      # VALU issue peak, unpacked FP32 lane-
 # Wave-level VALU instructions per launch at the bench sizes, from rocprofv3
 # SQ_INSTS_VALU (profiles/valu_counters_rs_wrp.md); valid for the kernels as
 # built here, re-measure after changing them.
-PROFILED_VALU_INSTS = {"rs_accumulate": 181585920, "asm_cols": 28180480, "wrp_gather": 33586086}
+PROFILED_VALU_INSTS = {"rs_accumulate": 181585920, "asm_cols": 26714112, "wrp_gather": 33586086}
 
 
 def rel_l2(a, b):
@@ -137,8 +137,8 @@ WRP_KERNELS = ["wrp_bin_count", "wrp_bin_colscan", "wrp_bin_fill", "wrp_gather",
 # FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); summary in
 # profiles/hbm_traffic_wrp_depthmap.md.  Valid for the kernels as built here;
 # re-measure after changing them.
-PROFILED_TRAFFIC = {"asm_cols": (65.6 + 74.8) * 2**20, "asm_rows_fwd": (32.4 + 64.2) * 2**20,
-                    "asm_rows_inv": (64.4 + 32.0) * 2**20}
+PROFILED_TRAFFIC = {"asm_cols": (66.5 + 119.6) * 2**20, "asm_rows_fwd": (32.4 + 64.2) * 2**20,
+                    "asm_rows_inv": (64.2 + 32.0) * 2**20}
 
 
 def wrp_bytes(n_side, pad):
@@ -466,12 +466,16 @@ def bench_fft2d(ctx, args):
     wall, per = _time_steps(ctx, lambda: ctx.fft2d(d, device=True, asynchronous=True), args,
                             ["fft_rows", "fft_cols"])
     moved = 2 * n * n * 8
+    # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE runs of
+    # `bench.py --workload fft2d --only --steps 2 --warmup 1 --no-cpu` (same
+    # calibration as PROFILED_TRAFFIC)
+    traffic = {"fft_rows": (128.2 + 128.0) * 2**20, "fft_cols": (128.8 + 136.0) * 2**20}
     roof = {}
     for k in ("fft_rows", "fft_cols"):
         if k in per:
             gbs = moved / (per[k]["avg_us"] * 1e-6) / 1e9
             roof[k] = {"kernel": k, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                       "frac": gbs / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes": moved}
+                       "frac": gbs / HBM_PEAK_GBS, "traffic": traffic[k], "algorithmic_bytes": moved}
     # accuracy against numpy's FFT (double precision) on the same array
     f = ctx.fft2d(x)
     t0 = time.perf_counter()
