@@ -214,21 +214,41 @@ __global__ __launch_bounds__(256) void wrp_bin_colscan(WrpArgs a) {
 // pass 3: per chunk, point indices into the tile lists (slot = tile offset +
 // chunk prefix + rank within the chunk from an LDS counter).  Every
 // workgroup forms the tile offsets (exclusive prefix of the tile totals) in
-// LDS itself; chunk 0 also stores them for the gather.  Dynamic LDS: 2 ntiles
-// words.
+// LDS itself and adds its own chunk's prefixes, so the point loop reads only
+// LDS; chunk 0 also stores the offsets for the gather.  Dynamic LDS: 2
+// ntiles words.
 __global__ __launch_bounds__(BIN_THREADS) void wrp_bin_fill(WrpArgs a) {
     extern __shared__ unsigned fill_lds[];
     __shared__ unsigned part[BIN_THREADS];
     unsigned* cur = fill_lds;
-    unsigned* offs = fill_lds + a.ntiles;
+    unsigned* base = fill_lds + a.ntiles;  // tile offset + this chunk's prefix
     const int n = a.ntiles, t = threadIdx.x;
     const int per = (n + BIN_THREADS - 1) / BIN_THREADS;
     const int b = min(n, t * per), e = min(n, b + per);
+    const int chunk = xcd_remap(blockIdx.x, gridDim.x);
+    // The thread's tile totals and this chunk's prefixes in those tiles, read
+    // once; up to FAST tiles per thread the loads are all issued before any
+    // is used (range-checked buffer loads, out-of-range entries read 0)
+    constexpr int FAST = 4;
+    unsigned tv[FAST], cv[FAST];
     unsigned sum = 0;
-    for (int i = b; i < e; ++i) {
-        cur[i] = 0u;
-        sum += a.totals[i];
+    if (per <= FAST) {
+        const auto rt = oph_buf_rsrc(a.totals, (unsigned)(n * sizeof(unsigned)));
+        const auto rc = oph_buf_rsrc(a.chunk_counts, (unsigned)((size_t)n * a.nchunks * sizeof(unsigned)));
+#pragma unroll
+        for (int j = 0; j < FAST; ++j) {
+            const int i = b + j;
+            const bool in = j < per && i < e;
+            tv[j] = __builtin_amdgcn_raw_buffer_load_b32(rt, in ? (unsigned)(i * sizeof(unsigned)) : OPH_BUF_OOB, 0, 0);
+            cv[j] = __builtin_amdgcn_raw_buffer_load_b32(
+                rc, in ? (unsigned)(((size_t)i * a.nchunks + chunk) * sizeof(unsigned)) : OPH_BUF_OOB, 0, 0);
+        }
+#pragma unroll
+        for (int j = 0; j < FAST; ++j) sum += tv[j];
+    } else {
+        for (int i = b; i < e; ++i) sum += a.totals[i];
     }
+    for (int i = b; i < e; ++i) cur[i] = 0u;
     part[t] = sum;
     __syncthreads();
     for (int off = 1; off < BIN_THREADS; off <<= 1) {
@@ -237,28 +257,45 @@ __global__ __launch_bounds__(BIN_THREADS) void wrp_bin_fill(WrpArgs a) {
         part[t] += v;
         __syncthreads();
     }
-    const int chunk = xcd_remap(blockIdx.x, gridDim.x);
     unsigned run = part[t] - sum;  // exclusive prefix of this thread's range
-    for (int i = b; i < e; ++i) {
-        offs[i] = run;
-        if (chunk == 0) a.offsets[i] = run;
-        run += a.totals[i];
+    if (per <= FAST) {
+#pragma unroll
+        for (int j = 0; j < FAST; ++j) {
+            const int i = b + j;
+            if (j < per && i < e) {
+                base[i] = run + cv[j];
+                if (chunk == 0) a.offsets[i] = run;
+                run += tv[j];
+            }
+        }
+    } else {
+        for (int i = b; i < e; ++i) {
+            base[i] = run + a.chunk_counts[(size_t)i * a.nchunks + chunk];
+            if (chunk == 0) a.offsets[i] = run;
+            run += a.totals[i];
+        }
     }
     if (chunk == 0 && t == BIN_THREADS - 1) a.offsets[n] = part[BIN_THREADS - 1];
     __syncthreads();
-    const int64_t base = (int64_t)chunk * a.chunk;
+    const int64_t pbase = (int64_t)chunk * a.chunk;
+    // the chunk's records; a record's first 16 bytes (tx, ty, w, exact) in
+    // one load ahead of the test on w
+    const auto rp = oph_buf_rsrc(a.pts + pbase, (unsigned)((size_t)a.chunk * sizeof(WrpPoint)));
     for (int k = threadIdx.x; k < a.chunk; k += BIN_THREADS) {
-        const int64_t i = base + k;
+        const int64_t i = pbase + k;
         if (i >= a.n) break;
-        const WrpPoint q = a.pts[i];
+        const float4 hf = oph_buf_load_f4(rp, (unsigned)(k * sizeof(WrpPoint)));
+        WrpPoint q;
+        q.tx = __float_as_int(hf.x);
+        q.ty = __float_as_int(hf.y);
+        q.w = __float_as_int(hf.z);
         if (q.w <= 0) continue;
         int x0, x1, y0, y1;
         tile_range(q, a, x0, x1, y0, y1);
         for (int ty = y0; ty <= y1; ++ty)
             for (int tx = x0; tx <= x1; ++tx) {
                 const int tile = ty * a.ntx + tx;
-                const unsigned slot =
-                    offs[tile] + a.chunk_counts[(size_t)tile * a.nchunks + chunk] + atomicAdd(&cur[tile], 1u);
+                const unsigned slot = base[tile] + atomicAdd(&cur[tile], 1u);
                 if (slot < a.capacity)
                     a.list[slot] = (unsigned)i;
                 else
