@@ -155,7 +155,12 @@ __global__ __launch_bounds__(512) void asm_rows_fwd(const float2* __restrict__ i
 template <int L>
 __host__ __device__ constexpr bool asm_cols_fused() { return fft::regs_fusable<L>() && fft::n16(L) == 3; }
 
-template <int L, int B, bool EXACT>
+// MID: the data rows are exactly the middle half, [L/4, 3L/4) (twofold
+// padding; 4096-point columns, two per workgroup): the first forward
+// butterfly's inputs t + m L/16 with m outside [4, 12) are known zeros and
+// only the last inverse butterfly's outputs with m in [4, 12) are kept, so
+// neither is read or written in LDS.
+template <int L, int B, bool EXACT, bool MID = false>
 __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, int oy,
                                                  const float4* __restrict__ tabx,
                                                  const float4* __restrict__ taby,
@@ -176,7 +181,24 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
     float2* tw1 = lds;  // first, so every read of it is one address plus an immediate
     float2* lines = lds + (FUSED ? 256 : 0);
     if constexpr (!FUSED) T.load(tw, threadIdx.x % TPL);
-    if constexpr (B % 2 == 0 && (L <= 1024 || FUSED)) {
+    static_assert(!MID || (FUSED && B == 2), "middle-half rows: 4096-point columns, two per workgroup");
+    if constexpr (MID) {
+        // lane rows threadIdx.x + j nthr, data for j in [J0, J1): no bounds tests
+        constexpr int J0 = L / 4 / (TPL * B), J1 = 3 * L / 4 / (TPL * B);
+        const auto rs = oph_buf_rsrc(work, (unsigned)((size_t)(3 * L / 4) * npx * sizeof(float2)));
+        float4 ld[J1 - J0];
+#pragma unroll
+        for (int j = J0; j < J1; ++j) {
+            const int i = threadIdx.x + j * nthr;
+            ld[j - J0] = oph_buf_load_f4(rs, (unsigned)(((size_t)i * npx + cbase) * sizeof(float2)));
+        }
+#pragma unroll
+        for (int j = J0; j < J1; ++j) {
+            const int i = threadIdx.x + j * nthr;
+            lines[fft::pad16(i)] = make_float2(ld[j - J0].x, ld[j - J0].y);
+            lines[PL + fft::pad16(i)] = make_float2(ld[j - J0].z, ld[j - J0].w);
+        }
+    } else if constexpr (B % 2 == 0 && (L <= 1024 || FUSED)) {
         // two neighbouring columns per lane (16-byte accesses); npx is a
         // power of two >= B, so every column of the group exists.  All 8
         // loads of a lane are issued before any is used: range-checked
@@ -233,7 +255,17 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
         // in registers, which are the inverse's first butterfly's inputs:
         // the transfer is applied in between, without a pass through LDS
         float2 w[1][16];
-        fft::stage<L, 16, -1, TPL>(line, w, 1, t);
+        if constexpr (MID) {
+            float2 u[16];
+#pragma unroll
+            for (int m = 0; m < 16; ++m)
+                u[m] = (m >= 4 && m < 12) ? line[fft::lane_pad<L>(t, m)] : make_float2(0.f, 0.f);
+            __syncthreads();  // every lane has read its inputs
+            fft::stage0_from_regs<L, -1>(u, line, t);
+            __syncthreads();
+        } else {
+            fft::stage<L, 16, -1, TPL>(line, w, 1, t);
+        }
 #pragma unroll
         for (int r = 1; r < 16; ++r) w[0][r] = tw1[(r - 1) * 16 + (t & 15)];
         fft::stage<L, 16, -1, TPL>(line, w, 16, t);
@@ -253,7 +285,16 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
         fft::stage<L, 16, +1, TPL>(line, w, 16, t);
 #pragma unroll
         for (int r = 1; r < 16; ++r) w[0][r] = oph_buf_load_f2(twl, t * sizeof(float2), (r - 1) * TPL * sizeof(float2));
-        fft::stage<L, 16, +1, TPL>(line, w, 16 * 16, t);
+        if constexpr (MID) {
+            // the last butterfly reads and writes only the lane's own
+            // elements t + m L/16, so no barrier separates the two
+            fft::last_stage_to_regs_w<L, +1>(line, w[0], t, v);
+#pragma unroll
+            for (int m = 4; m < 12; ++m) line[fft::lane_pad<L>(t, m)] = v[m];
+            __syncthreads();
+        } else {
+            fft::stage<L, 16, +1, TPL>(line, w, 16 * 16, t);
+        }
     } else {
         fft::line<L, -1>(line, T, t);
         // (a few rows at a time: every stage's twiddles stay in registers)
@@ -583,20 +624,20 @@ int launch_rows_inv(oph_ctx* ctx, const float2* work, int nx, int ny, int ox, in
     return OPH_OK;
 }
 
-template <int LY, int B, bool EXACT>
+template <int LY, int B, bool EXACT, bool MID = false>
 int launch_cols_b(oph_ctx* ctx, float2* work, int npx, int ny, int oy, const float4* tx, const float4* ty,
                   const AsmConst& k, const float2* tw) {
     const size_t shm = ((size_t)B * fft::padded_len(LY) + (asm_cols_fused<LY>() ? 256 : 0)) * sizeof(float2);
     static bool attr = false;
     if (!attr) {
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&asm_cols<LY, B, EXACT>),
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&asm_cols<LY, B, EXACT, MID>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         hipGetLastError();
         attr = true;
     }
     // the compact y table follows the two float4 tables (ty has LY entries)
     const float2* ty2 = reinterpret_cast<const float2*>(ty + LY);
-    OPH_LAUNCH(ctx, "asm_cols", (asm_cols<LY, B, EXACT>), dim3((npx + B - 1) / B), dim3(B * (LY / 16)), shm,
+    OPH_LAUNCH(ctx, "asm_cols", (asm_cols<LY, B, EXACT, MID>), dim3((npx + B - 1) / B), dim3(B * (LY / 16)), shm,
                work, npx, ny, oy, tx, ty, ty2, k, tw);
     return OPH_OK;
 }
@@ -611,7 +652,13 @@ int launch_cols_ex(oph_ctx* ctx, float2* work, int npx, int ny, int oy, const fl
     switch (b) {
         case 1: return launch_cols_b<LY, 1, EXACT>(ctx, work, npx, ny, oy, tx, ty, k, tw);
         case 2:
-            if constexpr (BMAX >= 2) return launch_cols_b<LY, 2, EXACT>(ctx, work, npx, ny, oy, tx, ty, k, tw);
+            if constexpr (BMAX >= 2) {
+                if constexpr (asm_cols_fused<LY>()) {
+                    if (oy == LY / 4 && ny == LY / 2)
+                        return launch_cols_b<LY, 2, EXACT, true>(ctx, work, npx, ny, oy, tx, ty, k, tw);
+                }
+                return launch_cols_b<LY, 2, EXACT>(ctx, work, npx, ny, oy, tx, ty, k, tw);
+            }
             break;
         case 4:
             if constexpr (BMAX >= 4) return launch_cols_b<LY, 4, EXACT>(ctx, work, npx, ny, oy, tx, ty, k, tw);
