@@ -27,7 +27,7 @@ VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9      # VALU issue peak, unpacked FP32 lane-
 # Wave-level VALU instructions per launch at the bench sizes, from rocprofv3
 # SQ_INSTS_VALU (profiles/valu_counters_rs_wrp.md); valid for the kernels as
 # built here, re-measure after changing them.
-PROFILED_VALU_INSTS = {"rs_accumulate": 181585920, "asm_cols": 26714112, "wrp_gather": 33586086}
+PROFILED_VALU_INSTS = {"rs_accumulate": 181585920, "asm_cols": 25419776, "wrp_gather": 33586086}
 
 
 def rel_l2(a, b):
@@ -137,7 +137,7 @@ WRP_KERNELS = ["wrp_bin_count", "wrp_bin_colscan", "wrp_bin_fill", "wrp_gather",
 # FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); summary in
 # profiles/hbm_traffic_wrp_depthmap.md.  Valid for the kernels as built here;
 # re-measure after changing them.
-PROFILED_TRAFFIC = {"asm_cols": (66.4 + 117.8) * 2**20, "asm_rows_fwd": (32.4 + 64.2) * 2**20,
+PROFILED_TRAFFIC = {"asm_cols": (65.9 + 101.7) * 2**20, "asm_rows_fwd": (32.4 + 64.2) * 2**20,
                     "asm_rows_inv": (64.2 + 32.0) * 2**20}
 
 
