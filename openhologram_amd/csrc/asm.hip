@@ -177,7 +177,7 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
     // twiddles (k = t) read from the table just before each use; other
     // lengths hold every stage's twiddles in registers
     constexpr bool FUSED = asm_cols_fused<L>();
-    fft::Twiddles<FUSED ? 16 : L> T;
+    fft::Twiddles<FUSED ? 16 : L> T;  // (a placeholder when FUSED: not loaded or used)
     float2* tw1 = lds;  // first, so every read of it is one address plus an immediate
     float2* lines = lds + (FUSED ? 256 : 0);
     if constexpr (!FUSED) T.load(tw, threadIdx.x % TPL);
