@@ -95,6 +95,21 @@ def test_asm_4096_columns(ctx, nx, pitch, lam, z, band):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("band", [False, True])
+def test_asm_4096_columns_full_height(ctx, band):
+    """4096-row field without padding: 4096-point columns whose data fill the
+    whole column (the general register-fused column path, not the
+    middle-half one of twofold padding)."""
+    print(f"case: asm 4096x16 pad 1 band={band}", flush=True)
+    rng = np.random.default_rng(40 + int(band))
+    u = (rng.standard_normal((4096, 16)) + 1j * rng.standard_normal((4096, 16))).astype(np.complex64)
+    out = ctx.asm(u, 8e-6, 532e-9, 0.05, pad=1, band_limit=band)
+    ref = O.asm(u.astype(np.complex128), 8e-6, 8e-6, 532e-9, 0.05, pad=1, band_limit=band)
+    err = rel_l2(out, ref)
+    print("rel-L2", err)
+    assert err < 1e-5
+
+
 def test_asm_many_rows_per_workgroup(ctx):
     """2048^2 field on the 4096^2 grid: the persistent row passes take
     several rows per workgroup (next row prefetched across the transform)
