@@ -159,6 +159,20 @@ def test_wrp_vs_oracle(ctx, mode):
     assert np.array_equal(out, out2)
 
 
+def test_wrp_large_plane(ctx):
+    """4096 x 2048 plane (2048 tiles, more than four per binning thread):
+    the tile-offset scan's general path, against the oracle, deterministic."""
+    print("case: wrp 4096x2048 2000 points", flush=True)
+    x, y, z, a, zw = scenes.wrp_points(2000, seed=12)
+    y = y * 0.5
+    out = ctx.wrp(4096, 2048, 8e-6, 532e-9, zw, x, y, z, a)
+    ref = O.wrp(4096, 2048, 8e-6, 532e-9, zw, x, y, z, a)
+    err = rel_l2(out, ref)
+    print("rel-L2", err)
+    assert err < 1e-5
+    assert np.array_equal(out, ctx.wrp(4096, 2048, 8e-6, 532e-9, zw, x, y, z, a))
+
+
 def test_wrp_edge_cases(ctx):
     # empty set, a point outside the plane, unit amplitude, accumulate, ragged size
     e = np.zeros(0)
