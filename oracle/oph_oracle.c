@@ -13,19 +13,31 @@
  * output element is produced by one thread in the reference's summation order
  * (the reference's own OpenMP loops race on the output; this one does not).
  *
- * How each function is pinned (the tests that hold it):
+ * How each function is pinned (the tests that hold it; tests/test_oracle_*.py):
  *   orc_keti_fringe_f32   bit-exact vs the compiled reference source (oracle/_ref)
  *   orc_rs                single-point closed form, linearity, translation
  *                         equivariance, Re() vs the FP32 reference fixtures
- *   orc_fresnel           brute-force per-pair sum in Python on tiny cases,
- *                         rank-1 separable closed form of one plane
+ *   orc_fresnel           brute-force long-double per-pair sum on tiny cases,
+ *                         rank-1 separable closed form of one plane, and the
+ *                         paraxial limit of orc_rs (same points, pixel origins
+ *                         aligned: phase sign, carrier and grid conventions)
  *   orc_fft               numpy.fft on random inputs, O(N^2) DFT, Parseval
- *   orc_asm               delta spectrum -> exact transfer value, forward/back
- *                         identity, Fresnel limit of a point source
+ *   orc_asm_transfer      values against exp(i 2 pi z sqrt(..)), band limit on
+ *                         both axes against Matsushima's closed-form cutoff
+ *                         |f| <= 1/(lambda sqrt((2 df z)^2 + 1))
+ *   orc_asm               plane-wave eigenfunction (with and without the band
+ *                         limit), forward/back identity, evanescent removal,
+ *                         spherical wave carried from z1 to z1 + dz, paraxial
+ *                         limit against the Fresnel transfer function, padded
+ *                         result = crop(IFFT(orc_asm_transfer . FFT(embed)))
  *   orc_wrp               single-point window vs the closed-form spherical wave,
- *                         window size and clipping
- *   orc_depthmap          one-pixel scene vs closed form, layer linearity,
- *                         hand-computed 8x8 encode, FFT-shift cancellation
+ *                         window size and clipping, linearity
+ *   orc_dm_encode         explicit inverse-DFT sum over the kept spectrum rows
+ *                         with the shift phase (bins on both sides of both crop
+ *                         edges, complex amplitudes, padded and unpadded grids)
+ *   orc_depthmap          one-pixel scene: spectrum and fringe against closed
+ *                         forms (with and without the field lens), layer
+ *                         linearity, depth-index modes, level drift
  *   orc_normalize_u8      bit-exact vs the compiled reference ophNormalize
  *   orc_scale_u8          hand-computed values (min, max, interior, constant)
  *   orc_dm_reconstruct    closed form: with the eye focused at distance 0 the
@@ -383,16 +395,51 @@ void orc_fft2d(c128* a, int nx, int ny, int sign, int normalize, int nthreads) {
 /* ------------------------------------------------------------------------ */
 
 /*
+ * Transfer value of the angular-spectrum method at FFT-order bin (r, c) of an
+ * Npy x Npx grid (fx = kx/(Npx px), kx in [-Npx/2, Npx/2), fy likewise):
+ *   H = exp(i 2 pi z sqrt(1/lambda^2 - fx^2 - fy^2)), 0 if evanescent
+ * (CWO_ANGULAR, CWO/cwo_lib.h:66-101; fn_FresnelPropagation_as.m:24).
+ * band_limit zeroes H where |fx z / sqrt(..)| > 1/(2 dfx) or
+ * |fy z / sqrt(..)| > 1/(2 dfy), dfx = 1/(Npx px) the grid's frequency step
+ * (fn_FresnelPropagation_as.m:33-38 with sx = sy = 0: Matsushima's band limit).
+ */
+static c128 asm_h(int r, int c, int Npx, int Npy, double px, double py, double lambda,
+                  double z, int band_limit) {
+    const c128 zero = {0.0, 0.0};
+    const double dfx = 1.0 / (Npx * px), dfy = 1.0 / (Npy * py);
+    const int ky = r < Npy / 2 ? r : r - Npy;
+    const int kx = c < Npx / 2 ? c : c - Npx;
+    const double fx = kx * dfx, fy = ky * dfy;
+    const double s = 1.0 / (lambda * lambda) - fx * fx - fy * fy;
+    if (s <= 0.0) return zero;
+    const double sq = sqrt(s);
+    if (band_limit) {
+        const double fla = fabs(fx * z / sq), flb = fabs(fy * z / sq);
+        if (fla > 1.0 / (2.0 * dfx) || flb > 1.0 / (2.0 * dfy)) return zero;
+    }
+    double t = z * sq;  /* turns */
+    t -= floor(t);
+    return cexpi(TWO_PI * t);
+}
+
+/* The transfer of orc_asm on the whole Npy x Npx grid, FFT order. */
+void orc_asm_transfer(int Npx, int Npy, double px, double py, double lambda, double z,
+                      int band_limit, c128* H) {
+    for (int r = 0; r < Npy; ++r)
+        for (int c = 0; c < Npx; ++c)
+            H[(size_t)r * Npx + c] = asm_h(r, c, Npx, Npy, px, py, lambda, z, band_limit);
+}
+
+/*
  * ASM propagation of an ny x nx field over distance z, FP64.
  * Plumbing of CWO::Diffract (CWO/cwo.cpp:2198-2278): zero-pad into a
  * (pad*ny) x (pad*nx) grid at offset (Npx/2 - nx/2, Npy/2 - ny/2)
  * (:2235-2239), transform, crop the same window back (:2258-2263).
- * Transfer as CWO_ANGULAR (CWO/cwo_lib.h:66-101 and the open statement
- * OpenHolo_LFCGH/fn_FresnelPropagation_as.m:24):
- *   H(fx,fy) = exp(i 2 pi z sqrt(1/lambda^2 - fx^2 - fy^2)), 0 if evanescent
- * on FFT-order grids fx = kx/(Npx px), kx in [-Npx/2, Npx/2).  band_limit
- * applies fn_FresnelPropagation_as.m:33-38 (Matsushima) with sx = sy = 0.
+ * Transfer asm_h above on the padded grid (band_limit as there).
  * The inverse transform is normalised by 1/(Npx Npy) (cwo.cpp:2930).
+ * (fn_FresnelPropagation_as.m evaluates its transfer on the grid
+ * ((1:N) - N/2) dal, which after its fftshift sits one bin above each
+ * sample's frequency; the FFT-order grid here is CWO's and the exact one.)
  */
 void orc_asm(c128* field, int nx, int ny, double px, double py, double lambda,
              double z, int pad, int band_limit, int nthreads) {
@@ -402,30 +449,11 @@ void orc_asm(c128* field, int nx, int ny, double px, double py, double lambda,
     for (int r = 0; r < ny; ++r)
         memcpy(g + (size_t)(r + oy) * Npx + ox, field + (size_t)r * nx, sizeof(c128) * nx);
     orc_fft2d(g, Npx, Npy, -1, 0, nthreads);
-    const double inv_l2 = 1.0 / (lambda * lambda);
-    const double dfx = 1.0 / (Npx * px), dfy = 1.0 / (Npy * py);
-    for (int r = 0; r < Npy; ++r) {
-        const int ky = r < Npy / 2 ? r : r - Npy;
-        const double fy = ky * dfy;
+    for (int r = 0; r < Npy; ++r)
         for (int c = 0; c < Npx; ++c) {
-            const int kx = c < Npx / 2 ? c : c - Npx;
-            const double fx = kx * dfx;
-            const double s = inv_l2 - fx * fx - fy * fy;
             c128* e = &g[(size_t)r * Npx + c];
-            if (s <= 0.0) { e->re = e->im = 0.0; continue; }
-            const double sq = sqrt(s);
-            if (band_limit) {
-                const double fla = fabs(fx * z / sq), flb = fabs(fy * z / sq);
-                if (fla > 1.0 / (2.0 * dfx) || flb > 1.0 / (2.0 * dfy)) {
-                    e->re = e->im = 0.0;
-                    continue;
-                }
-            }
-            double t = z * sq;  /* turns */
-            t -= floor(t);
-            *e = cmul(*e, cexpi(TWO_PI * t));
+            *e = cmul(*e, asm_h(r, c, Npx, Npy, px, py, lambda, z, band_limit));
         }
-    }
     orc_fft2d(g, Npx, Npy, +1, 1, nthreads);
     for (int r = 0; r < ny; ++r)
         memcpy(field + (size_t)r * nx, g + (size_t)(r + oy) * Npx + ox, sizeof(c128) * nx);
@@ -501,6 +529,37 @@ void orc_depth_index(int n, const unsigned char* dep8, int quant_mode, int Q,
         }
     }
     free(lev);
+}
+
+/*
+ * Symmetrisation encode of a centred-order spectrum U (Ny x Nx, overwritten),
+ * Encoding_Symmetrization(ivec2(0,1)) of HologramGenerator.cpp:481-522 and
+ * encoding_CPU / get_shift_phase_value of HologramGenerator_CPU.cpp:231-270,
+ * 367-408:
+ *   c = floor(Ny/2), keep 1-based spectrum rows c - floor(c/2) .. that + c - 1
+ *   (all columns; :494-496, then -1 to 0-based :511-514), zero the rest;
+ *   inverse fftwShift (FFTW_BACKWARD) normalised by 1/(Nx Ny);
+ *   fringe[r][c] = Re(v[r + oy][c + ox] exp(i 2 pi yy / (4 py))),
+ *   yy = Sy/2 - py r - py (CPU.cpp:376-390), Sy = h py.
+ * The scene window (h x w) sits at offset ((Nx - w)/2, (Ny - h)/2) of the
+ * grid; Nx = w, Ny = h is the reference's unpadded case.
+ */
+void orc_dm_encode(c128* U, int Nx, int Ny, int w, int h, double py, double* fringe,
+                   int nthreads) {
+    const int ox = (Nx - w) / 2, oy = (Ny - h) / 2;
+    const int cropy = Ny / 2;
+    const int cropy1 = cropy - cropy / 2 - 1;
+    const int cropy2 = cropy1 + cropy - 1;
+    for (int y = 0; y < Ny; ++y)
+        if (y < cropy1 || y > cropy2) memset(U + (size_t)y * Nx, 0, sizeof(c128) * Nx);
+    ref_fftwshift(U, Nx, Ny, -1, 1, nthreads);
+    const double Sys = py * h;
+    for (int r = 0; r < h; ++r) {
+        const double yy = (Sys / 2.0) - py * r - py;
+        const c128 sp = cexpi(2 * M_PI * (yy / (4 * py)));
+        for (int c = 0; c < w; ++c)
+            fringe[(size_t)r * w + c] = cmul(U[(size_t)(r + oy) * Nx + (c + ox)], sp).re;
+    }
 }
 
 /*
@@ -597,22 +656,7 @@ void orc_depthmap(int w, int h, const unsigned char* img8, const unsigned char* 
             for (int c = 0; c < w; ++c) field[(size_t)r * w + c] = V[(size_t)(r + oy) * Nx + (c + ox)];
         free(V);
     }
-    if (fringe) {
-        /* Encoding_Symmetrization(ivec2(0,1)), HologramGenerator.cpp:487-514 */
-        const int cropy = Ny / 2;
-        const int cropy1 = cropy - cropy / 2 - 1;
-        const int cropy2 = cropy1 + cropy - 1;
-        for (int y = 0; y < Ny; ++y)
-            if (y < cropy1 || y > cropy2) memset(U + (size_t)y * Nx, 0, sizeof(c128) * Nx);
-        ref_fftwshift(U, Nx, Ny, -1, 1, nthreads);
-        const double Sys = py * h;
-        for (int r = 0; r < h; ++r) {
-            const double yy = (Sys / 2.0) - py * r - py;
-            const c128 sp = cexpi(2 * M_PI * (yy / (4 * py)));
-            for (int c = 0; c < w; ++c)
-                fringe[(size_t)r * w + c] = cmul(U[(size_t)(r + oy) * Nx + (c + ox)], sp).re;
-        }
-    }
+    if (fringe) orc_dm_encode(U, Nx, Ny, w, h, py, fringe, nthreads);
     free(u);
     free(U);
     free(lev);

@@ -58,6 +58,10 @@ def lib():
         _lib.orc_fft2d.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         _lib.orc_asm.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                  C.c_double, C.c_int, C.c_int, C.c_int]
+        _lib.orc_asm_transfer.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double,
+                                          C.c_double, C.c_double, C.c_int, P]
+        _lib.orc_dm_encode.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, P,
+                                       C.c_int]
         _lib.orc_depth_levels.argtypes = [C.c_double, C.c_double, C.c_int, P]
         _lib.orc_depth_index.argtypes = [C.c_int, P, C.c_int, C.c_int, C.c_double,
                                          C.c_double, C.c_int, P]
@@ -183,6 +187,23 @@ def asm(field, px, py, lam, z, pad=2, band_limit=False, nthreads=0):
     _pow2(f.shape[0] * pad, f.shape[1] * pad)
     lib().orc_asm(_p(f), f.shape[1], f.shape[0], px, py, lam, z, pad, int(band_limit), nthreads)
     return f
+
+
+def asm_transfer(Npx, Npy, px, py, lam, z, band_limit=False):
+    """Angular-spectrum transfer on an Npy x Npx grid in FFT order."""
+    H = np.zeros((Npy, Npx), np.complex128)
+    lib().orc_asm_transfer(Npx, Npy, px, py, lam, z, int(band_limit), _p(H))
+    return H
+
+
+def dm_encode(spectrum, w, h, py, nthreads=0):
+    """Symmetrisation encode of a centred-order spectrum (Ny x Nx) to an
+    h x w real fringe (scene window centred in the grid)."""
+    U = np.array(spectrum, np.complex128, order="C", copy=True)
+    _pow2(*U.shape)
+    fr = np.zeros((h, w), np.float64)
+    lib().orc_dm_encode(_p(U), U.shape[1], U.shape[0], w, h, py, _p(fr), nthreads)
+    return fr
 
 
 def depth_levels(near, far, L):

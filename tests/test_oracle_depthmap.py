@@ -91,3 +91,73 @@ def test_depthmap_layer_linearity_and_field():
     # a zero-intensity scene gives nothing (alpha = [img > 0])
     z, fz = O.depthmap(np.zeros_like(img), dep, *args, quant_mode=1)
     assert np.all(fz == 0)
+
+
+def _kept_rows(Ny):
+    """Spectrum rows the symmetrisation encode keeps, 0-based: 1-based rows
+    c - floor(c/2) .. that + c - 1 with c = floor(Ny/2)
+    (HologramGenerator.cpp:494-496, shifted by -1 at :511-514)."""
+    c = Ny // 2
+    y1 = c - c // 2
+    return y1 - 1, y1 + c - 2
+
+
+def _encode_closed_form(S, w, h, py):
+    """Fringe of a centred-order spectrum S as an explicit inverse-DFT sum over
+    the kept rows (no FFT): the normalised backward transform between the two
+    fftshifts takes bin (y0, x0) to exp(+i 2 pi ((x0 - Nx/2)(c + ox + Nx/2)/Nx
+    + (y0 - Ny/2)(r + oy + Ny/2)/Ny)) / (Nx Ny) at scene pixel (r, c), then
+    the shift phase exp(i 2 pi yy/(4 py)), yy = h py/2 - py r - py
+    (HologramGenerator_CPU.cpp:376-390), and Re."""
+    Ny, Nx = S.shape
+    oy, ox = (Ny - h) // 2, (Nx - w) // 2
+    y1, y2 = _kept_rows(Ny)
+    y0 = np.arange(y1, y2 + 1)
+    x0 = np.arange(Nx)
+    r = np.arange(h)
+    c = np.arange(w)
+    Ey = np.exp(2j * np.pi * np.outer(r + oy + Ny // 2, y0 - Ny // 2) / Ny)
+    Ex = np.exp(2j * np.pi * np.outer(x0 - Nx // 2, c + ox + Nx // 2) / Nx)
+    v = Ey @ S[y1:y2 + 1, :] @ Ex / (Nx * Ny)
+    yy = h * py / 2 - py * r - py
+    return (v * np.exp(2j * np.pi * yy / (4 * py))[:, None]).real
+
+
+@pytest.mark.parametrize("Ny,Nx,h,w", [(16, 8, 16, 8), (16, 32, 10, 20), (32, 16, 24, 12)])
+def test_encode_edges_closed_form(Ny, Nx, h, w):
+    """Encode of a hand-built spectrum: complex bins on both sides of both crop
+    edges (the outer ones must vanish), at random columns."""
+    rng = np.random.default_rng(Ny + Nx + h)
+    py = 8e-6
+    y1, y2 = _kept_rows(Ny)
+    S = np.zeros((Ny, Nx), np.complex128)
+    for row in (y1 - 1, y1, (y1 + y2) // 2, y2, y2 + 1):
+        for col in rng.choice(Nx, 2, replace=False):
+            S[row, col] = rng.normal() + 1j * rng.normal()
+    fr = O.dm_encode(S, w, h, py)
+    ref = _encode_closed_form(S, w, h, py)
+    assert np.abs(fr - ref).max() < 1e-12 * np.abs(ref).max()
+    # the outer bins alone encode to zero
+    S_out = np.zeros_like(S)
+    S_out[[y1 - 1, y2 + 1]] = S[[y1 - 1, y2 + 1]]
+    assert np.abs(O.dm_encode(S_out, w, h, py)).max() < 1e-15
+
+
+@pytest.mark.parametrize("field_lens", [0.0, 0.4])
+@pytest.mark.parametrize("pad", [False, True])
+def test_depthmap_single_pixel_fringe(pad, field_lens):
+    """Whole pipeline of a one-pixel scene, spectrum and encode both in closed
+    form; with a field lens the layer sits at z = -f d/(d - f)
+    (HologramGenerator.cpp:432)."""
+    h, w = (8, 16) if not pad else (6, 10)
+    Ny, Nx = (h, w) if not pad else (16, 32)
+    img = np.zeros((h, w), np.uint8)
+    dep = np.zeros((h, w), np.uint8)
+    img[3, 5], dep[3, 5] = 180, 247   # mode 0: layer 256 - 247 = 9
+    px, py, lam, near, far = 8e-6, 8e-6, 473e-9, 0.09, 0.11
+    _, fr = O.depthmap(img, dep, Nx, Ny, px, py, lam, near, far, 256, field_lens=field_lens)
+    d = O.depth_levels(near, far, 256)[8]
+    zl = -field_lens * d / (d - field_lens) if field_lens > 0 else d
+    S = _single_pixel_spectrum(Nx, Ny, h, w, 3, 5, 180 / 255, px, py, lam, zl)
+    ref = _encode_closed_form(S, w, h, py)
+    assert np.abs(fr - ref).max() < 1e-9 * np.abs(ref).max()

@@ -118,3 +118,90 @@ def test_wrp_linearity():
     g = O.wrp(64, 64, 8e-6, 532e-9, -0.1, x[:11], y[:11], z[:11], a[:11]) + \
         O.wrp(64, 64, 8e-6, 532e-9, -0.1, x[11:], y[11:], z[11:], a[11:])
     assert rel_l2(f, g) < 1e-13
+
+
+def _bl_cutoff(lam, df, z):
+    """Matsushima's band limit on one frequency axis with the other at zero
+    (K. Matsushima, T. Shimobaba, Opt. Express 17, 19662 (2009), eq. 13):
+    |f| <= 1/(lam sqrt((2 df z)^2 + 1)), df the grid's frequency step."""
+    return 1.0 / (lam * np.sqrt((2 * df * z) ** 2 + 1))
+
+
+@pytest.mark.parametrize("Npx,Npy,px,py,z", [(64, 128, 8e-6, 6e-6, 0.016),
+                                              (256, 128, 8e-6, 8e-6, 0.1)])
+def test_asm_band_limit_axis_cutoff(Npx, Npy, px, py, z):
+    lam = 532e-9
+    H = O.asm_transfer(Npx, Npy, px, py, lam, z, band_limit=True)
+    H0 = O.asm_transfer(Npx, Npy, px, py, lam, z, band_limit=False)
+    for n, p, line, line0 in ((Npx, px, H[0, :], H0[0, :]), (Npy, py, H[:, 0], H0[:, 0])):
+        df = 1.0 / (n * p)
+        f = np.fft.fftfreq(n, p)
+        cut = _bl_cutoff(lam, df, z)
+        inside = np.abs(f) <= cut
+        # the cutoff falls between bins (no rounding question) and mid-band,
+        # where a threshold off by 2x would keep a different set of bins
+        assert np.min(np.abs(np.abs(f) - cut)) > 0.05 * df
+        assert 4 <= inside.sum() <= n - 4
+        for scale in (0.5, 2.0):
+            cut2 = 1.0 / (lam * np.sqrt((df * z / scale) ** 2 * 4 + 1))
+            assert not np.array_equal(np.abs(f) <= cut2, inside)
+        assert np.array_equal(line != 0, inside)
+        assert np.array_equal(line[inside], line0[inside])
+        ref = np.exp(2j * np.pi * z * np.sqrt(1 / lam ** 2 - f ** 2))
+        assert np.abs(line0 - ref).max() < 1e-9
+
+
+def test_asm_band_limited_plane_waves():
+    """Through orc_asm itself (no padding): a plane wave just inside the
+    band-limit cutoff passes with H, one just outside is removed."""
+    n, p, lam, z = 64, 8e-6, 532e-9, 0.016
+    df = 1.0 / (n * p)
+    kc = int(_bl_cutoff(lam, df, z) / df)
+    yy, xx = np.mgrid[0:n, 0:n]
+    for kx0, ky0, passes in ((kc, 0, True), (kc + 1, 0, False), (0, -kc, True),
+                             (0, -kc - 1, False)):
+        u = np.exp(2j * np.pi * (kx0 * xx + ky0 * yy) / n)
+        out = O.asm(u, p, p, lam, z, pad=1, band_limit=True)
+        if passes:
+            fx, fy = kx0 * df, ky0 * df
+            Hv = np.exp(2j * np.pi * z * np.sqrt(1 / lam ** 2 - fx ** 2 - fy ** 2))
+            assert np.abs(out - Hv * u).max() < 1e-9
+        else:
+            assert np.abs(out).max() < 1e-12
+
+
+@pytest.mark.parametrize("band_limit", [False, True])
+def test_asm_padded_composition(band_limit):
+    """Padded ASM = crop(IFFT(H . FFT(embed))) with the embedding offset
+    (Npx/2 - nx/2, Npy/2 - ny/2) of CWO::Diffract (cwo.cpp:2235-2239), built
+    here with numpy's FFT and the transfer alone."""
+    rng = np.random.default_rng(7)
+    ny, nx, p, lam, z = 16, 32, 8e-6, 633e-9, 0.02
+    u = rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))
+    Npy, Npx = 2 * ny, 2 * nx
+    g = np.zeros((Npy, Npx), np.complex128)
+    oy, ox = Npy // 2 - ny // 2, Npx // 2 - nx // 2
+    g[oy:oy + ny, ox:ox + nx] = u
+    H = O.asm_transfer(Npx, Npy, p, p, lam, z, band_limit)
+    ref = np.fft.ifft2(H * np.fft.fft2(g))[oy:oy + ny, ox:ox + nx]
+    assert rel_l2(O.asm(u, p, p, lam, z, pad=2, band_limit=band_limit), ref) < 1e-12
+
+
+def test_asm_paraxial_limit_fresnel_tf():
+    """A beam of low spatial frequencies: the angular spectrum agrees with the
+    Fresnel transfer function H = exp(ikz) exp(-i pi lam z (fx^2 + fy^2))
+    (WM/FresnelPropagation.m:7-14; Goodman, Fourier Optics, eq. 4-21), the
+    difference being the phase pi z lam^3 f^4 / 4 ~ 1e-7 rad here."""
+    n, p, lam, z = 128, 8e-6, 532e-9, 0.02
+    c = (np.arange(n) - n / 2) * p
+    X, Y = np.meshgrid(c, c)
+    u = np.exp(-(X ** 2 + Y ** 2) / (20 * p) ** 2) * np.exp(2j * np.pi * 3 * X / (n * p))
+    out = O.asm(u, p, p, lam, z, pad=1)
+    f = np.fft.fftfreq(n, p)
+    FX, FY = np.meshgrid(f, f)
+    HF = np.exp(2j * np.pi * ((z / lam) % 1)) * np.exp(-1j * np.pi * lam * z * (FX ** 2 + FY ** 2))
+    ref = np.fft.ifft2(np.fft.fft2(u) * HF)
+    assert rel_l2(out, ref) < 1e-5
+    assert rel_l2(out, u * np.exp(2j * np.pi * ((z / lam) % 1))) > 0.1  # it did diffract
+    bad = O.asm(u, p, p, lam, -z, pad=1)
+    assert rel_l2(bad, ref) > 0.5
