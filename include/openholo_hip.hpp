@@ -500,9 +500,14 @@ class OPHWRP {
         p.amp = nullptr;
         p.phase = nullptr;
         p.n = (int64_t)n;
-        oph_wrp(detail::context(), n_, n_, pitch_, lambda_, z, 1, &p, reinterpret_cast<oph_c64*>(field_.data()),
-                OPH_ACCUMULATE);
+        // the reference adds into the plane (AddPixel); a failed call leaves
+        // the plane as it was and is reported on stderr and by status()
+        status_ = oph_wrp(detail::context(), n_, n_, pitch_, lambda_, z, 1, &p,
+                          reinterpret_cast<oph_c64*>(field_.data()), OPH_ACCUMULATE);
+        if (status_ != OPH_OK) std::fprintf(stderr, "SingleWRP: %s\n", oph_last_error(detail::context()));
     }
+    // OPH_OK, or the error code of the last SingleWRP call
+    int status() const { return status_; }
     int Fresnelpropagation(float z) {
         const int m = detail::next_pow2(n_);
         if (m == n_) {
@@ -535,6 +540,7 @@ class OPHWRP {
    private:
     double lambda_ = 0, pitch_ = 0;
     int n_ = 0;
+    int status_ = OPH_OK;
     std::vector<std::complex<float>> field_;
 };
 

@@ -129,6 +129,18 @@ int oph_create(int device, oph_ctx** out) {
         delete c;
         return OPH_EDEVICE;
     }
+    // the context's asynchronous-WRP overflow word, cleared once here
+    void* flag = nullptr;
+    if (hipMalloc(&flag, sizeof(unsigned)) != hipSuccess || hipMemset(flag, 0, sizeof(unsigned)) != hipSuccess) {
+        hipGetLastError();
+        if (flag) hipFree(flag);
+        hipEventDestroy(c->ev_begin);
+        hipEventDestroy(c->ev_end);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return OPH_ENOMEM;
+    }
+    c->wrp_flag = static_cast<unsigned*>(flag);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
     *out = c;
@@ -145,6 +157,7 @@ void oph_destroy(oph_ctx* ctx) {
         if (t) hipFree(t);
     for (void* t : ctx->tw8)
         if (t) hipFree(t);
+    if (ctx->wrp_flag) hipFree(ctx->wrp_flag);
     for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
     if (ctx->ev_begin) hipEventDestroy(ctx->ev_begin);
     if (ctx->ev_end) hipEventDestroy(ctx->ev_end);
@@ -158,12 +171,15 @@ int oph_sync(oph_ctx* ctx) {
     int rc = oph_enter(ctx);
     if (rc) return rc;
     OPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->pending_overflow) {
+    if (ctx->wrp_flag) {
         unsigned f = 0;
-        OPH_HIP_CHECK(ctx, hipMemcpy(&f, ctx->pending_overflow, sizeof f, hipMemcpyDeviceToHost));
-        ctx->pending_overflow = nullptr;
-        if (f) return oph_fail(ctx, OPH_ENOMEM, "an asynchronous WRP call overflowed its tile lists; "
-                                                "repeat it synchronously");
+        OPH_HIP_CHECK(ctx, hipMemcpy(&f, ctx->wrp_flag, sizeof f, hipMemcpyDeviceToHost));
+        if (f) {
+            const unsigned zero = 0;
+            OPH_HIP_CHECK(ctx, hipMemcpy(ctx->wrp_flag, &zero, sizeof zero, hipMemcpyHostToDevice));
+            return oph_fail(ctx, OPH_ENOMEM, "an asynchronous WRP call overflowed its tile lists and left its "
+                                             "output unwritten; repeat it synchronously");
+        }
     }
     return OPH_OK;
 }

@@ -51,7 +51,7 @@ struct oph_ctx {
     int cu_count = 256;
     void* tw[24] = {};  // FFT twiddle tables by log2(length), radix-16 stages
     void* tw8[24] = {}; // the same for radix-8 stages
-    unsigned* pending_overflow = nullptr;  // device flag left by an async WRP call
+    unsigned* wrp_flag = nullptr;          // device word set by an async WRP call whose lists overflowed
     std::vector<double> host_scratch;      // per-call host tables uploaded to the device
 };
 

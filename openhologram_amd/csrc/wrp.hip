@@ -78,7 +78,8 @@ struct WrpArgs {
     float* gmax;             // [1]
     unsigned* list;
     unsigned capacity;
-    unsigned* overflow;
+    unsigned* overflow;      // this call's flag (cleared by wrp_bin_colscan)
+    unsigned* sticky;        // the context's flag for asynchronous calls, or null
 };
 
 constexpr int BIN_THREADS = 256;
@@ -105,13 +106,19 @@ __device__ __forceinline__ void tile_range(const WrpPoint& q, const WrpArgs& a, 
 __device__ WrpPoint make_point(const WrpArgs& a, int64_t i) {
     const double dz = a.z_wrp - a.z[i];
     WrpPoint q;
-    q.tx = (int)(a.x[i] / a.pitch) + (a.nx >> 1);
-    q.ty = (int)(a.y[i] / a.pitch) + (a.ny >> 1);
+    // centres and half-widths are clamped so that tx +- w stays inside int:
+    // |tx|, |ty| <= 2^29 and w <= 2^30 (a window that large covers the whole
+    // plane of any allowed size from any clamped centre that can reach it)
+    const double cx = fmin(fmax(trunc(a.x[i] / a.pitch), -536870912.0), 536870912.0);
+    const double cy = fmin(fmax(trunc(a.y[i] / a.pitch), -536870912.0), 536870912.0);
+    q.tx = (int)cx + (a.nx >> 1);
+    q.ty = (int)cy + (a.ny >> 1);
+    double wd;
     if (a.mode == 0)
-        q.w = (int)(fabs(dz) * a.grating / a.pitch);
+        wd = trunc(fabs(dz) * a.grating / a.pitch);
     else
-        q.w = (int)fabs(a.lambda * dz / a.pitch / a.pitch / 2 + 0.5) * 2 - 1;
-    if (q.w < 0) q.w = 0;
+        wd = trunc(fabs(a.lambda * dz / a.pitch / a.pitch / 2 + 0.5)) * 2 - 1;
+    q.w = (int)fmin(fmax(wd, 0.0), 1073741824.0);
     const double sgn = dz > 0.0 ? 1.0 : -1.0;
     const double adz = fabs(dz);
     const double am = a.amp ? a.amp[i] : 1.0;
@@ -296,10 +303,12 @@ __global__ __launch_bounds__(BIN_THREADS) void wrp_bin_fill(WrpArgs a) {
             for (int tx = x0; tx <= x1; ++tx) {
                 const int tile = ty * a.ntx + tx;
                 const unsigned slot = base[tile] + atomicAdd(&cur[tile], 1u);
-                if (slot < a.capacity)
+                if (slot < a.capacity) {
                     a.list[slot] = (unsigned)i;
-                else
+                } else {
                     *a.overflow = 1u;
+                    if (a.sticky) *a.sticky = 1u;
+                }
             }
     }
 }
@@ -308,7 +317,10 @@ struct GatherArgs {
     const WrpPoint* pts;
     const unsigned* offsets;
     const unsigned* list;
+    const unsigned* chunk_pre;  // [ntiles][nchunks] exclusive prefix over chunks
+    const unsigned* overflow;
     const float* gmax;
+    int nchunks;
     unsigned capacity;
     int nx, ny, ntx;
     double pitch, inv_lambda;
@@ -330,9 +342,9 @@ __device__ __forceinline__ void abs_range(int w0, int w1, int& lo, int& hi) {
     }
 }
 
-// Point record as the gather stages it in LDS (32 bytes).
+// Point record as the gather stages it in LDS (40 bytes).
 struct GRec {
-    short tx, ty, w, exact;
+    int tx, ty, w, exact;
     int idx;
     float f0, ks, p2dz2, ampd, cd;
 };
@@ -349,12 +361,50 @@ constexpr int REC_BATCH = 128;
 // summed in segments of at most SEGMENT points (one segment for all but
 // dense tiles), each with its own scale, so the fixed-point unit stays below
 // 2^-19 of the largest contribution; segment sums are added to the tile's
-// output in segment order (deterministic).
+// output in segment order.  Segments end on binning-chunk boundaries (the
+// largest boundary within SEGMENT entries, or the next one when a single
+// chunk holds more): the order of entries inside a chunk's run comes from an
+// LDS atomic in wrp_bin_fill, but which points form each segment then does
+// not depend on it, so the result is deterministic.
 constexpr unsigned SEGMENT = 2048;
+
+// End (relative to the tile's list start) of the segment that starts at the
+// chunk boundary r0: pre[c] is the exclusive prefix of chunk c's entries in
+// this tile (c < nchunks), and total the tile's entry count.
+__device__ unsigned segment_end(const unsigned* pre, int nchunks, unsigned total, unsigned r0) {
+    const unsigned target = r0 + SEGMENT;
+    if (target >= total) return total;
+    // largest c with pre[c] <= target (pre[0] = 0)
+    int lo = 0, hi = nchunks - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= target)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    if (pre[lo] > r0) return pre[lo];
+    // the chunk starting at r0 alone exceeds SEGMENT: the next boundary above r0
+    lo = 0;
+    hi = nchunks;  // boundary nchunks is total > r0
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        const unsigned v = mid < nchunks ? pre[mid] : total;
+        if (v > r0)
+            hi = mid;
+        else
+            lo = mid + 1;
+    }
+    return lo < nchunks ? pre[lo] : total;
+}
 
 __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
     __shared__ unsigned long long acc[TILE * TILE];
     __shared__ GRec recs[REC_BATCH];
+    __shared__ unsigned seg_hi;
+    // an overflowed binning leaves the output untouched (the call is redone
+    // at the exact size, or reported)
+    if (*g.overflow) return;
     const int tile = blockIdx.x;
     const int tyi = tile / g.ntx, txi = tile % g.ntx;
     const int X0 = txi * TILE, Y0 = tyi * TILE;
@@ -364,12 +414,15 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
     const float mx = *g.gmax;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     constexpr int NW = GATHER_THREADS / 64;
-    for (unsigned sb = b;; sb += SEGMENT) {
-        const unsigned se = min(e, sb + SEGMENT);
+    const unsigned* pre = g.chunk_pre + (size_t)tile * g.nchunks;
+    for (unsigned sb = b;;) {
+        __syncthreads();  // the previous segment's output reads are done
+        if (threadIdx.x == 0) seg_hi = e - b <= SEGMENT ? e : b + segment_end(pre, g.nchunks, e - b, sb - b);
+        __syncthreads();
+        const unsigned se = seg_hi;
         const float bound = mx * (float)max(se - sb, 1u);
         const int S = bound > 0.f ? max(-60, min(60, 30 - (int)ceilf(log2f(bound)))) : 30;
         const float scale = ldexpf(1.f, S);
-        __syncthreads();  // the previous segment's output reads are done
         for (int i = threadIdx.x; i < TILE * TILE; i += GATHER_THREADS) acc[i] = 0ull;
         for (unsigned bs = sb; bs < se; bs += REC_BATCH) {
             const int cnt_b = (int)min((unsigned)REC_BATCH, se - bs);
@@ -378,10 +431,10 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                 const unsigned idx = g.list[bs + threadIdx.x];
                 const WrpPoint q = g.pts[idx];
                 GRec r;
-                r.tx = (short)q.tx;
-                r.ty = (short)q.ty;
-                r.w = (short)q.w;
-                r.exact = (short)q.exact;
+                r.tx = q.tx;
+                r.ty = q.ty;
+                r.w = q.w;
+                r.exact = q.exact;
                 r.idx = (int)idx;
                 r.f0 = q.f0;
                 r.ks = q.ks;
@@ -492,6 +545,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
             }
         }
         if (se >= e) break;
+        sb = se;
     }
 }
 
@@ -539,6 +593,7 @@ int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, do
     a.overflow = a.offsets + ntiles + 1;
     a.chunk_max = reinterpret_cast<float*>(a.overflow + 1);
     a.gmax = a.chunk_max + nchunks;
+    a.sticky = check ? nullptr : ctx->wrp_flag;
     size_t cap = ctx->slot_size[SLOT_TABLE2] / sizeof(unsigned);
     const size_t want = (size_t)24 * np + 65536;
     if (cap < want) cap = want;
@@ -561,6 +616,9 @@ int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, do
         g.pts = a.pts;
         g.offsets = a.offsets;
         g.list = a.list;
+        g.chunk_pre = a.chunk_counts;
+        g.overflow = a.overflow;
+        g.nchunks = nchunks;
         g.gmax = a.gmax;
         g.capacity = a.capacity;
         g.nx = nx;
@@ -572,22 +630,18 @@ int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, do
         g.accumulate = accumulate;
         OPH_LAUNCH(ctx, "wrp_gather", wrp_gather, dim3(ntiles), dim3(GATHER_THREADS), 0, g);
         // the list size is only known on the device; a capacity overflow is
-        // detected after the fact and the pass is redone with the exact size.
-        // Asynchronous calls skip the read-back (the capacity is 24 entries
-        // per point, far above what grating-angle windows need) and leave the
-        // flag for oph_sync() to report.
-        if (!check) {
-            ctx->pending_overflow = a.overflow;
-            break;
-        }
+        // detected after the fact (the gather then writes nothing, so an
+        // accumulating call is still intact) and the pass is redone with the
+        // exact size.  Asynchronous calls skip the read-back (the capacity is
+        // 24 entries per point, far above what grating-angle windows need):
+        // their overflow sets the context's flag, which oph_sync() reports.
+        if (!check) break;
         unsigned hdr[2] = {0, 0};
         OPH_HIP_CHECK(ctx, hipMemcpyAsync(hdr, a.offsets + ntiles, 2 * sizeof(unsigned), hipMemcpyDeviceToHost,
                                           ctx->stream));
         OPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
-        if (!hdr[1] || accumulate) {
-            if (hdr[1]) return oph_fail(ctx, OPH_ENOMEM, "WRP tile lists overflowed with accumulate set");
-            break;
-        }
+        if (!hdr[1]) break;
+        if (attempt == 1) return oph_fail(ctx, OPH_ENOMEM, "WRP tile lists overflowed at the exact size");
         cap = (size_t)hdr[0] + 65536;
     }
     return OPH_OK;
@@ -602,6 +656,8 @@ extern "C" int oph_wrp(oph_ctx* ctx, int nx, int ny, double pitch, double lambda
     if (!(pitch > 0) || !(lambda > 0) || lambda >= 2 * pitch)
         return oph_fail(ctx, OPH_EINVAL, "pitch and wavelength must be positive with lambda < 2 pitch");
     if (mode != 0 && mode != 1) return oph_fail(ctx, OPH_EINVAL, "mode must be 0 or 1");
+    if ((flags & OPH_ASYNC) && !(flags & OPH_DEVICE_PTR))
+        return oph_fail(ctx, OPH_EINVAL, "OPH_ASYNC needs device pointers");
     if (pts->n < 0 || (pts->n > 0 && (!pts->x || !pts->y || !pts->z)))
         return oph_fail(ctx, OPH_EINVAL, "bad point arrays");
     const int64_t n = pts->n;

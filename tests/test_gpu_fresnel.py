@@ -33,9 +33,15 @@ def test_fresnel_edge_cases(ctx):
     assert np.allclose(np.abs(U), 0.7, atol=1e-4)
 
 
+# config-4 bars: the plain path at the north-star 1e-5; the MFMA forms at the
+# 1e-3 contraction target (FP16) or their factor-rounding level (the split
+# forms measure ~6e-5..8e-5 over 1e6 points; BF16 unsplit ~1.9e-3)
+CONFIG4_BOUND = {"valu": 1e-5, "bf16": 6e-3, "bf16_split": 2e-4, "fp16": 1e-3, "fp16_split": 2e-4}
+
+
 def test_fresnel_config4_sampled(ctx):
-    """1,000,000 points in 64 planes, 4096^2 at 8 um: every MFMA precision
-    against the FP64 oracle on 4096 sampled pixels."""
+    """1,000,000 points in 64 planes, 4096^2 at 8 um: the VALU kernel and
+    every MFMA precision against the FP64 oracle on 4096 sampled pixels."""
     print("case: fresnel config4 sampled", flush=True)
     x, y, p, z, a, ph = scenes.fresnel_points(1_000_000, planes=64, seed=4)
     rng = np.random.default_rng(9)
@@ -43,9 +49,9 @@ def test_fresnel_config4_sampled(ctx):
     ref = O.fresnel(4096, 4096, 8e-6, 532e-9, x, y, z, a, ph, rows=rows, cols=cols)
     d = [ctx.to_device(v) for v in (x, y, z, a, ph)]
     out = ctx.empty((4096, 4096), np.complex64)
-    for m in ("bf16", "bf16_split", "fp16", "fp16_split"):
+    for m in ("valu", "bf16", "bf16_split", "fp16", "fp16_split"):
         ctx.fresnel(4096, 4096, 8e-6, 532e-9, *d, method=m, out=out, device=True)
         U = out.numpy()[rows, cols]
         err = rel_l2(U, ref)
-        print(m, "config4 sampled rel-L2", err)
-        assert err < max(BOUND[m], 1e-3 if m != "bf16" else BOUND[m])
+        print(m, "config4 sampled rel-L2", err, flush=True)
+        assert err < CONFIG4_BOUND[m]

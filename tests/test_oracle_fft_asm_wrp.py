@@ -205,3 +205,25 @@ def test_asm_paraxial_limit_fresnel_tf():
     assert rel_l2(out, u * np.exp(2j * np.pi * ((z / lam) % 1))) > 0.1  # it did diffract
     bad = O.asm(u, p, p, lam, -z, pad=1)
     assert rel_l2(bad, ref) > 0.5
+
+
+def test_wrp_mode1_vs_reference_output():
+    """The reference example's own output (W/openwrp/openwrp.cpp:15-18):
+    SingleWRP(0.5 mm) of object.txt on a 515^2 plane at 8 um, 532 nm, saved
+    as an arg image (tests/golden/make_wrp_golden.py).  The reference sums in
+    FP32 (cosf of a float wave number times r, ~1e-3 rad per term) with a
+    racing OpenMP AddPixel, and CWO's arg writer is closed; its quantisation
+    was read off the file (zero pixels are 127: floor).  Measured: 99.19 %
+    of pixels equal, 99.989 % within one level."""
+    import os
+    from conftest import GOLDEN, wrp_table_agreement
+    g = np.load(os.path.join(GOLDEN, "wrp_table.npz"), allow_pickle=False)
+    xyz = g["xyz"].astype(np.float64)
+    lam, p, n, zw = (float(v) for v in g["params"])
+    U = O.wrp(int(n), int(n), p, lam, zw, xyz[:, 0], xyz[:, 1], xyz[:, 2], mode=1)
+    exact, near = wrp_table_agreement(U, g["gray"])
+    assert exact >= 0.99 and near >= 0.9995, (exact, near)
+    # mode 0 (the grating-angle window) or a flipped plane is far off
+    U0 = O.wrp(int(n), int(n), p, lam, zw, xyz[:, 0], xyz[:, 1], xyz[:, 2], mode=0)
+    assert wrp_table_agreement(U0, g["gray"])[0] < 0.8
+    assert wrp_table_agreement(U[::-1], g["gray"])[0] < 0.8   # 70 % of pixels are empty in both
