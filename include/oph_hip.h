@@ -301,6 +301,35 @@ int oph_normalize_u8(oph_ctx* ctx, const float* src, unsigned char* dst, int nx,
  * flags: OPH_DEVICE_PTR. */
 int oph_scale_u8(oph_ctx* ctx, const double* src, unsigned char* dst, int64_t n, int flags);
 
+/* Encodings of a complex field (nx x ny complex64, row-major) to a float
+ * array -- replaces the MATLAB toolbox's fn_encoding
+ * (OpenHolo_Encoding/fn_encoding.m:1-98, with fn_myfft2.m / fn_myifft2.m):
+ *   OPH_ENC_SSB         single sideband (:29-47): centred spectrum with the
+ *                       1-based block floor(N/2)..N (left/top) or 1..floor(N/2)
+ *                       (right/bottom) zeroed, inverse, (Re - min)/(max - min);
+ *                       nx, ny powers of two in 16..8192
+ *   OPH_ENC_NI          |U + max|U||^2 / its maximum (:49-54)
+ *   OPH_ENC_P2A         angle(U) + 0.5, the reference's angle + pi/(2 pi) (:56-58)
+ *   OPH_ENC_REAL        Re(U) (:20-21)
+ *   OPH_ENC_BURCKHARDT  A1, A2, A3 interleaved, out is ny x 3nx (:60-83), with
+ *                       the reference's branch terms as written
+ *   OPH_ENC_TWO_PHASE   angle + pi +- acos(|U|/max|U|) interleaved, out is
+ *                       ny x 2nx (:85-98)
+ * passband (SSB only): OPH_PASS_LEFT, _RIGHT, _TOP, _BOTTOM.
+ * flags: OPH_DEVICE_PTR, OPH_ASYNC (with device pointers). */
+#define OPH_ENC_SSB        0
+#define OPH_ENC_NI         1
+#define OPH_ENC_P2A        2
+#define OPH_ENC_REAL       3
+#define OPH_ENC_BURCKHARDT 4
+#define OPH_ENC_TWO_PHASE  5
+#define OPH_PASS_LEFT      0
+#define OPH_PASS_RIGHT     1
+#define OPH_PASS_TOP       2
+#define OPH_PASS_BOTTOM    3
+int oph_encode_field(oph_ctx* ctx, const oph_c64* field, int nx, int ny, int method, int passband,
+                     float* out, int flags);
+
 /* 8-bit grayscale BMP with a 256-entry gray palette, byte-for-byte the file
  * ophSaveFileBmp writes (K/oph_common/oph_common.cpp:49-90): 1078-byte
  * header, rows stored as given with no row padding.  `path` is used as is

@@ -129,6 +129,7 @@ def load():
         "oph_cnn_save": (I, [P, P, C.c_char_p]),
         "oph_cnn_load": (I, [P, P, C.c_char_p]),
         "oph_normalize_u8": (I, [P, P, P, I, I, I]),
+        "oph_encode_field": (I, [P, P, I, I, I, I, P, I]),
         "oph_scale_u8": (I, [P, P, P, I64, I]),
         "oph_save_bmp8": (I, [P, I, I, C.c_char_p]),
         "oph_save_bmp24": (I, [P, I, I, C.c_char_p]),
@@ -239,6 +240,22 @@ class Context:
             dst = np.zeros((ny, nx), np.uint8)
         self.check(self.lib.oph_normalize_u8(self.h, ptr(src), ptr(dst), nx, ny, DEVICE_PTR if device else 0))
         return dst
+
+    def encode_field(self, field, method, passband="left", device=False, out=None, asynchronous=False):
+        """MATLAB-toolbox encoding of a complex field (oph_encode_field).
+        method: ssb | ni | p2a | real | burckhardt | two_phase; the output is
+        (ny, nx) float32, (ny, 3nx) for burckhardt, (ny, 2nx) for two_phase."""
+        m = ENCODINGS[method]
+        ny, nx = field.shape
+        k = 3 if method == "burckhardt" else 2 if method == "two_phase" else 1
+        if not device:
+            field = np.ascontiguousarray(field, np.complex64)
+        if out is None:
+            out = self.empty((ny, k * nx), np.float32) if device else np.zeros((ny, k * nx), np.float32)
+        flags = (DEVICE_PTR if device else 0) | (ASYNC if asynchronous else 0)
+        self.check(self.lib.oph_encode_field(self.h, ptr(field), nx, ny, m, PASSBANDS[passband], ptr(out),
+                                             flags))
+        return out
 
     def scale_u8(self, src, device=False, dst=None):
         """Write_Result_image scaling of a double array -> u8 (same shape)."""
@@ -465,6 +482,10 @@ class DeviceArray:
         except Exception:
             pass
         self.ptr = None
+
+
+ENCODINGS = {"ssb": 0, "ni": 1, "p2a": 2, "real": 3, "burckhardt": 4, "two_phase": 5}
+PASSBANDS = {"left": 0, "right": 1, "top": 2, "bottom": 3}
 
 
 def make_spec(nx, ny, pitch, lambda_, scale=(1.0, 1.0, 1.0), offset_z=0.0, tilt_deg=(0.0, 0.0)):

@@ -241,7 +241,10 @@ def test_wrp_overflow_retry_with_accumulate(ctx):
     a = rng.uniform(0.2, 1.0, n)
     assert _mode1_tiles(1024, 1024, 8e-6, 532e-9, 0.0, x, y, z) > 24 * n + 65536
     base = (rng.standard_normal((1024, 1024)) + 1j * rng.standard_normal((1024, 1024))).astype(np.complex64)
-    out = ctx.wrp(1024, 1024, 8e-6, 532e-9, 0.0, x, y, z, a, mode=1, out=base.copy(), accumulate=True)
+    # a fresh context: its list buffer starts at the first capacity
+    from openhologram_amd import Context
+    with Context(0) as c2:
+        out = c2.wrp(1024, 1024, 8e-6, 532e-9, 0.0, x, y, z, a, mode=1, out=base.copy(), accumulate=True)
     ref = O.wrp(1024, 1024, 8e-6, 532e-9, 0.0, x, y, z, a, mode=1)
     err = rel_l2(out - base, ref)
     print("rel-L2", err)
@@ -266,20 +269,25 @@ def test_wrp_async_flags(ctx):
     ctx.normalize_u8(fr, device=True, dst=u8)
     ctx.sync()
     assert rel_l2(out.numpy(), O.wrp(64, 64, 8e-6, 532e-9, 0.0, x, y, z)) < 1e-5
-    # overflow: the windows of the retry test, asynchronously
+    # overflow: the windows of the retry test, asynchronously, in a fresh
+    # context (its list buffer starts at the first capacity)
     print("case: wrp async overflow 1024x1024 mode 1", flush=True)
+    from openhologram_amd import Context
     rng = np.random.default_rng(31)
     n = 1500
     px, py = rng.uniform(-1.5e-3, 1.5e-3, (2, n))
     pz = rng.uniform(-0.0315, -0.0305, n)
-    big = ctx.to_device(np.full((1024, 1024), 3 + 4j, np.complex64))
-    ctx.wrp(1024, 1024, 8e-6, 532e-9, 0.0, ctx.to_device(px), ctx.to_device(py), ctx.to_device(pz), mode=1,
-            out=big, device=True, asynchronous=True, accumulate=True)
-    with pytest.raises(OphError) as e:
-        ctx.sync()
-    assert "overflow" in str(e.value)
-    assert np.all(big.numpy() == np.complex64(3 + 4j))
-    ctx.sync()   # reported once
+    assert _mode1_tiles(1024, 1024, 8e-6, 532e-9, 0.0, px, py, pz) > 24 * n + 65536
+    with Context(0) as c2:
+        big = c2.to_device(np.full((1024, 1024), 3 + 4j, np.complex64))
+        c2.wrp(1024, 1024, 8e-6, 532e-9, 0.0, c2.to_device(px), c2.to_device(py), c2.to_device(pz), mode=1,
+               out=big, device=True, asynchronous=True, accumulate=True)
+        with pytest.raises(OphError) as e:
+            c2.sync()
+        assert "overflow" in str(e.value)
+        assert np.all(big.numpy() == np.complex64(3 + 4j))
+        c2.sync()   # reported once
+        del big
 
 
 def test_wrp_reference_table(ctx):
