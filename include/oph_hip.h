@@ -172,7 +172,11 @@ int oph_asm_propagate(oph_ctx* ctx, oph_c64* field, int nx, int ny, double px,
  *   r = sign(dz) sqrt((wx p)^2 + (wy p)^2 + dz^2)
  * (pixels outside the plane skipped).  mode 0: grating-angle half-width
  * w = (int)(|dz| tan(asin(lambda/(2p)))/p) (CWO/cwoPLS.cpp:613); mode 1:
- * the reference's w = (int)|lambda dz/(2 p^2) + 0.5| * 2 - 1 (WRP.cpp:111).
+ * the reference's w = (int)|lambda dz/(2 p^2) + 0.5| * 2 - 1 (WRP.cpp:111);
+ * mode 2: CWO's cwoPLS::WRP (CWO/cwoPLS.cpp:522-591), Fresnel phase
+ *   wrp[..] += a exp(i k (dz + ((wx p)^2 + (wy p)^2)/(2 dz))),
+ * half-width w = (int)(|dz| lambda/(2p)/p) (:557; the reference takes |dz|
+ * from its call argument, equal to dz for points at z = 0).
  * pts->phase is ignored (the reference points carry none).  wrp is ny x nx
  * complex64, row index = y.  OPH_ACCUMULATE adds into wrp. */
 int oph_wrp(oph_ctx* ctx, int nx, int ny, double pitch, double lambda,

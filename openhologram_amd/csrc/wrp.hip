@@ -9,7 +9,11 @@
 //   wrp[ty+wy][tx+wx] += a_n exp(i k r) / (r + 0.05)                (:131-136)
 // Half-width w: mode 0 the grating-angle radius (int)(|dz| tan(asin(lambda/
 // (2p)))/p) (CWO/cwoPLS.cpp:613), mode 1 the reference's
-// (int)|lambda dz/(2p^2) + 0.5| * 2 - 1 (WRP.cpp:111).
+// (int)|lambda dz/(2p^2) + 0.5| * 2 - 1 (WRP.cpp:111).  Mode 2 is CWO's own
+// statement cwoPLS::WRP (CWO/cwoPLS.cpp:522-591): Fresnel phase
+// r = dz + rho^2/(2 dz), amplitude a (no 1/(r + 0.05)), half-width
+// (int)(|dz| lambda/(2p)/p) (:557; the reference takes |dz| from its call
+// argument, which is dz when the points sit at z = 0).
 //
 // The reference scatters with a racing AddPixel; here the plane is gathered
 // per 64x64 tile, with no global atomics:
@@ -47,8 +51,11 @@ constexpr int TILE = 64;
 // so 8 waves per SIMD hide the LDS-atomic and transcendental latencies)
 constexpr int GATHER_THREADS = 512;
 
+// evaluation path of a point (uniform over its window)
+constexpr int PATH_SERIES = 0, PATH_EXACT = 1, PATH_FRESNEL = 2, PATH_FRESNEL_EXACT = 3;
+
 struct WrpPoint {
-    int tx, ty, w, exact;
+    int tx, ty, w, exact;  // exact: one of PATH_*
     float f0;     // frac(sign |dz| / lambda + phase/2pi), turns
     float ks;     // sign |dz| / lambda
     float p2dz2;  // p^2 / dz^2
@@ -116,8 +123,10 @@ __device__ WrpPoint make_point(const WrpArgs& a, int64_t i) {
     double wd;
     if (a.mode == 0)
         wd = trunc(fabs(dz) * a.grating / a.pitch);
-    else
+    else if (a.mode == 1)
         wd = trunc(fabs(a.lambda * dz / a.pitch / a.pitch / 2 + 0.5)) * 2 - 1;
+    else
+        wd = trunc(fabs(dz) * (a.lambda / (2.0 * a.pitch)) / a.pitch);
     q.w = (int)fmin(fmax(wd, 0.0), 1073741824.0);
     const double sgn = dz > 0.0 ? 1.0 : -1.0;
     const double adz = fabs(dz);
@@ -133,13 +142,23 @@ __device__ WrpPoint make_point(const WrpArgs& a, int64_t i) {
     q.pad0 = q.pad1 = q.pad2 = 0.f;
     q.dz = dz;
     q.pad3 = 0.0;
+    if (a.mode == 2) {
+        // Fresnel phase (dz + rho^2/(2 dz))/lambda = f0 + ks u/2 with
+        // u = rho^2/dz^2, unit amplitude factor: FP32 while the quadratic
+        // part stays within 16 turns at the window corner, else FP64
+        const double umax = 2.0 * (double)q.w * q.w * (adz > 0.0 ? a.pitch * a.pitch / (dz * dz) : 0.0);
+        q.ampd = (float)am;
+        q.cd = 0.f;
+        q.exact = (adz * a.inv_lambda * 0.5 * umax > 16.0 || adz == 0.0) ? PATH_FRESNEL_EXACT : PATH_FRESNEL;
+        return q;
+    }
     // series error at the window corner: (|dz|/lambda) 7/256 u^5 turns, and
     // the amplitude's dropped second-order term (cd P)^2
     const double umax = 2.0 * (double)q.w * q.w * (adz > 0.0 ? a.pitch * a.pitch / (dz * dz) : 0.0);
     const double u5 = umax * umax * umax * umax * umax;
     q.exact = (umax > 0.05 || adz * a.inv_lambda * (7.0 / 256.0) * u5 > 1e-7 || fabs(q.cd) * 0.5 * umax > 1e-3)
-                  ? 1
-                  : 0;
+                  ? PATH_EXACT
+                  : PATH_SERIES;
     return q;
 }
 
@@ -467,12 +486,26 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                 // the point's amplitude with the segment's fixed-point scale folded in
                 const float amps = q.ampd * scale;
                 // one loop per evaluation path (the path is uniform over the point)
-                auto walk = [&](auto exact_tag) {
+                auto walk = [&](auto exact_tag, auto fresnel_tag) {
                     constexpr bool EXACT = decltype(exact_tag)::value;
+                    constexpr bool FRESNEL = decltype(fresnel_tag)::value;
                     while (iy < nay) {
                         const int ax = ax0 + ix, ay = ay0 + iy;
                         float re, im;
-                        if constexpr (!EXACT) {
+                        if constexpr (FRESNEL && !EXACT) {
+                            const float fx = (float)ax, fy = (float)ay;
+                            const float u = fmaf(fx, fx, fy * fy) * q.p2dz2;
+                            const float ph = fmaf(q.ks, 0.5f * u, q.f0);
+                            re = amps * cos_turns(ph);
+                            im = amps * sin_turns(ph);
+                        } else if constexpr (FRESNEL) {
+                            const double dz = g.pts[q.idx].dz;
+                            const double dx = ax * g.pitch, dy = ay * g.pitch;
+                            double t = (dz + (dx * dx + dy * dy) / (2.0 * dz)) * g.inv_lambda;
+                            t -= rint(t);
+                            re = amps * cos_turns((float)t);
+                            im = amps * sin_turns((float)t);
+                        } else if constexpr (!EXACT) {
                             const float fx = (float)ax, fy = (float)ay;
                             const float u = fmaf(fx, fx, fy * fy) * q.p2dz2;
                             const float p = u * fmaf(u, fmaf(u, fmaf(u, -5.f / 128.f, 1.f / 16.f), -0.125f), 0.5f);
@@ -517,10 +550,12 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                         }
                     }
                 };
-                if (q.exact)
-                    walk(std::true_type{});
-                else
-                    walk(std::false_type{});
+                switch (q.exact) {
+                    case PATH_SERIES: walk(std::false_type{}, std::false_type{}); break;
+                    case PATH_EXACT: walk(std::true_type{}, std::false_type{}); break;
+                    case PATH_FRESNEL: walk(std::false_type{}, std::true_type{}); break;
+                    default: walk(std::true_type{}, std::true_type{}); break;
+                }
             }
         }
         __syncthreads();
@@ -655,7 +690,7 @@ extern "C" int oph_wrp(oph_ctx* ctx, int nx, int ny, double pitch, double lambda
     if (nx <= 0 || ny <= 0 || nx > 65536 || ny > 65536) return oph_fail(ctx, OPH_EINVAL, "bad WRP size");
     if (!(pitch > 0) || !(lambda > 0) || lambda >= 2 * pitch)
         return oph_fail(ctx, OPH_EINVAL, "pitch and wavelength must be positive with lambda < 2 pitch");
-    if (mode != 0 && mode != 1) return oph_fail(ctx, OPH_EINVAL, "mode must be 0 or 1");
+    if (mode < 0 || mode > 2) return oph_fail(ctx, OPH_EINVAL, "mode must be 0, 1 or 2");
     if ((flags & OPH_ASYNC) && !(flags & OPH_DEVICE_PTR))
         return oph_fail(ctx, OPH_EINVAL, "OPH_ASYNC needs device pointers");
     if (pts->n < 0 || (pts->n > 0 && (!pts->x || !pts->y || !pts->z)))
@@ -691,6 +726,7 @@ extern "C" int oph_wrp_hologram(oph_ctx* ctx, int n_side, double pitch, double l
     int rc = oph_enter(ctx);
     if (rc) return rc;
     if (pad != 1 && pad != 2) return oph_fail(ctx, OPH_EINVAL, "pad must be 1 or 2");
+    if (mode < 0 || mode > 2) return oph_fail(ctx, OPH_EINVAL, "mode must be 0, 1 or 2");
     const int np = n_side * pad;
     if (n_side <= 0 || (np & (np - 1)) || np < 16 || np > 8192)
         return oph_fail(ctx, OPH_ENOTSUP, "padded grid %d must be a power of two in 16..8192", np);

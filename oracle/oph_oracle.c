@@ -274,6 +274,9 @@ void orc_fresnel(int nx, int ny, double pitch, double lambda, int64_t n,
  * Half-width w: mode 0 = grating-angle radius of CWO/cwoPLS.cpp:613
  *   w = (int)(|dz| tan(asin(lambda/(2p))) / p);
  * mode 1 = W/openwrp/WRP.cpp:111, w = (int)|lambda dz/(2p^2) + 0.5|*2 - 1.
+ * mode 2 = cwoPLS::WRP (CWO/cwoPLS.cpp:522-591): w = (int)(|dz| lambda/(2p)/p)
+ *   (:557), Fresnel phase r = dz + (dx^2 + dy^2)/(2 dz) (:573), amplitude a
+ *   (the reference's unit amplitude when amp is NULL; no 1/(r + 0.05)).
  * amp may be NULL (unit amplitude, as the reference's cwoObjPoint).
  * The reference adds points in parallel with a racing AddPixel; here each
  * WRP pixel sums its contributions in point order n = 0..P-1.
@@ -293,8 +296,10 @@ void orc_wrp(int nx, int ny, double pitch, double lambda, double z_wrp, int mode
         ty[m] = (int)(y[m] / pitch) + nyh;
         if (mode == 0)
             tw[m] = (int)(fabs(dz) * ga / pitch);
-        else
+        else if (mode == 1)
             tw[m] = (int)fabs(lambda * dz / pitch / pitch / 2 + 0.5) * 2 - 1;
+        else
+            tw[m] = (int)(fabs(dz) * (lambda / (2.0 * pitch)) / pitch);
     }
     memset(wrp, 0, sizeof(c128) * (size_t)nx * ny);
     int row;
@@ -311,10 +316,11 @@ void orc_wrp(int nx, int ny, double pitch, double lambda, double z_wrp, int mode
                 const int col = tx[m] + wx;
                 if (col < 0 || col >= nx) continue;
                 const double dx = wx * pitch, dy = wy * pitch;
-                const double r = sgn * sqrt(dx * dx + dy * dy + dz * dz);
+                const double r = mode == 2 ? dz + (dx * dx + dy * dy) / (2.0 * dz)
+                                           : sgn * sqrt(dx * dx + dy * dy + dz * dz);
                 double t = r / lambda;
                 t -= floor(t);
-                const double inv = a / (r + 0.05);
+                const double inv = mode == 2 ? a : a / (r + 0.05);
                 c128* o = &wrp[(size_t)row * nx + col];
                 o->re += cos(TWO_PI * t) * inv;
                 o->im += sin(TWO_PI * t) * inv;

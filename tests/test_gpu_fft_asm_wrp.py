@@ -147,7 +147,7 @@ def test_wrp_many_points_and_linearity(ctx):
     assert np.array_equal(out, ctx.wrp(128, 128, 8e-6, 532e-9, zw, x, y, z, a))
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_wrp_vs_oracle(ctx, mode):
     print(f"case: wrp 256x256 mode {mode}", flush=True)
     x, y, z, a, zw = scenes.wrp_points(3000, seed=5)
@@ -307,3 +307,19 @@ def test_wrp_reference_table(ctx):
     assert exact >= 0.99 and near >= 0.9995
     ref = O.wrp(int(n), int(n), p, lam, zw, xyz[:, 0], xyz[:, 1], xyz[:, 2], mode=1)
     assert rel_l2(U, ref) < 1e-5
+
+
+def test_wrp_mode2_fp64_path(ctx):
+    """Mode 2 (cwoPLS::WRP) with windows whose quadratic phase passes 16
+    turns (|dz| ~ 40 mm): the per-pixel FP64 path, against the oracle."""
+    print("case: wrp mode 2 large dz 512x512", flush=True)
+    rng = np.random.default_rng(41)
+    n = 200
+    x, y = rng.uniform(-1e-3, 1e-3, (2, n))
+    z = rng.uniform(-0.045, -0.035, n)
+    a = rng.uniform(0.2, 1.0, n)
+    out = ctx.wrp(512, 512, 8e-6, 532e-9, 0.0, x, y, z, a, mode=2)
+    ref = O.wrp(512, 512, 8e-6, 532e-9, 0.0, x, y, z, a, mode=2)
+    err = rel_l2(out, ref)
+    print("rel-L2", err)
+    assert err < 1e-5

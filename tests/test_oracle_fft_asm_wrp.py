@@ -227,3 +227,31 @@ def test_wrp_mode1_vs_reference_output():
     U0 = O.wrp(int(n), int(n), p, lam, zw, xyz[:, 0], xyz[:, 1], xyz[:, 2], mode=0)
     assert wrp_table_agreement(U0, g["gray"])[0] < 0.8
     assert wrp_table_agreement(U[::-1], g["gray"])[0] < 0.8   # 70 % of pixels are empty in both
+
+
+def test_wrp_mode2_is_windowed_fresnel():
+    """cwoPLS::WRP (mode 2) is the Fresnel point field restricted to each
+    point's window: for points on the pixel grid (pitch 2^-17 m, so x/p is
+    exact) the plane equals the sum of orc_fresnel's single-point fields at
+    distance dz = z_wrp - z_n, each masked to [-w, w)^2 around its pixel with
+    w = (int)(|dz| lambda/(2p)/p) (cwoPLS.cpp:557)."""
+    n, p, lam, zw = 64, 2.0 ** -17, 532e-9, 0.0
+    pts = [(3, -5, -0.004, 0.7), (-10, 8, -0.0061, 1.0), (20, 20, -0.0035, 0.4)]
+    x = np.array([q[0] * p for q in pts])
+    y = np.array([q[1] * p for q in pts])
+    z = np.array([q[2] for q in pts])
+    a = np.array([q[3] for q in pts])
+    out = O.wrp(n, n, p, lam, zw, x, y, z, a, mode=2)
+    ref = np.zeros((n, n), np.complex128)
+    for k in range(len(pts)):
+        d = zw - z[k]
+        w = int(abs(d) * (lam / (2 * p)) / p)
+        assert 3 <= w < 40
+        f = O.fresnel(n, n, p, lam, x[k:k + 1], y[k:k + 1], [d], a[k:k + 1])
+        tx, ty = pts[k][0] + n // 2, pts[k][1] + n // 2
+        m = np.zeros((n, n), bool)
+        m[max(ty - w, 0):ty + w, max(tx - w, 0):tx + w] = True
+        ref += np.where(m, f, 0)
+    assert np.abs(out - ref).max() < 1e-9
+    # no 1/(r + 0.05) and a different phase from mode 0
+    assert rel_l2(O.wrp(n, n, p, lam, zw, x, y, z, a, mode=0), out) > 0.5
