@@ -111,7 +111,10 @@ __device__ __forceinline__ void tile_range(const WrpPoint& q, const WrpArgs& a, 
 }
 
 __device__ WrpPoint make_point(const WrpArgs& a, int64_t i) {
+    // every mode's quantities are formed and the mode's selected at the end
+    // (no branch on the mode: one straight-line path for all three)
     const double dz = a.z_wrp - a.z[i];
+    const bool fres = a.mode == 2;
     WrpPoint q;
     // centres and half-widths are clamped so that tx +- w stays inside int:
     // |tx|, |ty| <= 2^29 and w <= 2^30 (a window that large covers the whole
@@ -120,13 +123,10 @@ __device__ WrpPoint make_point(const WrpArgs& a, int64_t i) {
     const double cy = fmin(fmax(trunc(a.y[i] / a.pitch), -536870912.0), 536870912.0);
     q.tx = (int)cx + (a.nx >> 1);
     q.ty = (int)cy + (a.ny >> 1);
-    double wd;
-    if (a.mode == 0)
-        wd = trunc(fabs(dz) * a.grating / a.pitch);
-    else if (a.mode == 1)
-        wd = trunc(fabs(a.lambda * dz / a.pitch / a.pitch / 2 + 0.5)) * 2 - 1;
-    else
-        wd = trunc(fabs(dz) * (a.lambda / (2.0 * a.pitch)) / a.pitch);
+    const double w0 = trunc(fabs(dz) * a.grating / a.pitch);
+    const double w1 = trunc(fabs(a.lambda * dz / a.pitch / a.pitch / 2 + 0.5)) * 2 - 1;
+    const double w2 = trunc(fabs(dz) * (a.lambda / (2.0 * a.pitch)) / a.pitch);
+    const double wd = a.mode == 0 ? w0 : a.mode == 1 ? w1 : w2;
     q.w = (int)fmin(fmax(wd, 0.0), 1073741824.0);
     const double sgn = dz > 0.0 ? 1.0 : -1.0;
     const double adz = fabs(dz);
@@ -135,30 +135,25 @@ __device__ WrpPoint make_point(const WrpArgs& a, int64_t i) {
     t -= rint(t);
     q.f0 = (float)t;
     q.ks = (float)(sgn * adz * a.inv_lambda);
-    q.p2dz2 = adz > 0.0 ? (float)(a.pitch * a.pitch / (dz * dz)) : 0.f;
+    const double p2 = adz > 0.0 ? a.pitch * a.pitch / (dz * dz) : 0.0;
+    q.p2dz2 = (float)p2;
     const double D = sgn * adz + 0.05;
-    q.ampd = (float)(am / D);
-    q.cd = (float)(-sgn * adz / D);
+    // spherical modes: amplitude a/D (1 + cd P); Fresnel (mode 2): a
+    q.ampd = fres ? (float)am : (float)(am / D);
+    q.cd = fres ? 0.f : (float)(-sgn * adz / D);
     q.pad0 = q.pad1 = q.pad2 = 0.f;
     q.dz = dz;
     q.pad3 = 0.0;
-    if (a.mode == 2) {
-        // Fresnel phase (dz + rho^2/(2 dz))/lambda = f0 + ks u/2 with
-        // u = rho^2/dz^2, unit amplitude factor: FP32 while the quadratic
-        // part stays within 16 turns at the window corner, else FP64
-        const double umax = 2.0 * (double)q.w * q.w * (adz > 0.0 ? a.pitch * a.pitch / (dz * dz) : 0.0);
-        q.ampd = (float)am;
-        q.cd = 0.f;
-        q.exact = (adz * a.inv_lambda * 0.5 * umax > 16.0 || adz == 0.0) ? PATH_FRESNEL_EXACT : PATH_FRESNEL;
-        return q;
-    }
-    // series error at the window corner: (|dz|/lambda) 7/256 u^5 turns, and
-    // the amplitude's dropped second-order term (cd P)^2
-    const double umax = 2.0 * (double)q.w * q.w * (adz > 0.0 ? a.pitch * a.pitch / (dz * dz) : 0.0);
+    const double umax = 2.0 * (double)q.w * q.w * p2;
+    // spherical: series error at the window corner, (|dz|/lambda) 7/256 u^5
+    // turns, and the amplitude's dropped second-order term (cd P)^2
     const double u5 = umax * umax * umax * umax * umax;
-    q.exact = (umax > 0.05 || adz * a.inv_lambda * (7.0 / 256.0) * u5 > 1e-7 || fabs(q.cd) * 0.5 * umax > 1e-3)
-                  ? PATH_EXACT
-                  : PATH_SERIES;
+    const bool sph_exact =
+        umax > 0.05 || adz * a.inv_lambda * (7.0 / 256.0) * u5 > 1e-7 || fabs(-sgn * adz / D) * 0.5 * umax > 1e-3;
+    // Fresnel: phase (dz + rho^2/(2 dz))/lambda = f0 + ks u/2, u = rho^2/dz^2,
+    // in FP32 while the quadratic part stays within 16 turns at the corner
+    const bool fr_exact = adz * a.inv_lambda * 0.5 * umax > 16.0 || adz == 0.0;
+    q.exact = fres ? (fr_exact ? PATH_FRESNEL_EXACT : PATH_FRESNEL) : (sph_exact ? PATH_EXACT : PATH_SERIES);
     return q;
 }
 
