@@ -182,6 +182,24 @@ int oph_asm_propagate(oph_ctx* ctx, oph_c64* field, int nx, int ny, double px,
 int oph_wrp(oph_ctx* ctx, int nx, int ny, double pitch, double lambda,
             double z_wrp, int mode, const oph_points* pts, oph_c64* wrp, int flags);
 
+/* ---- allocation-free WRP ----
+ * No counterpart in the reference: oph_wrp for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, hosts
+ * that forbid hipMalloc).  Same arguments, kernels and result as oph_wrp
+ * (bit-identical output), but every array is a device pointer (flags must
+ * contain OPH_DEVICE_PTR; OPH_ACCUMULATE and OPH_ASYNC as in oph_wrp) and
+ * the binning state lives in a static device array of the library (under
+ * 24 MiB per device, held until the process ends) instead of the context's
+ * scratch, so the call allocates nothing.  Limits: at most 131072 points,
+ * planes of at most 1024 tiles of 64x64 (e.g. 2048 x 2048), and at most
+ * 3,211,264 (point, tile) list entries (24 per point plus 65536); larger
+ * calls return OPH_EINVAL, and a list overflow returns OPH_ENOMEM (reported
+ * by oph_sync for OPH_ASYNC) with the plane left unwritten.  All calls on
+ * one device share the array: calls that overlap on a device (from two
+ * contexts at once) race. */
+int oph_wrp_noalloc(oph_ctx* ctx, int nx, int ny, double pitch, double lambda,
+                    double z_wrp, int mode, const oph_points* pts, oph_c64* wrp, int flags);
+
 /* ---- WRP hologram: oph_wrp on an n_side^2 plane, then oph_asm_propagate
  * over `distance` with zero padding `pad` (replaces SingleWRP followed by
  * OPHWRP::Fresnelpropagation, W/openwrp/WRP.cpp:65-69, whose CWO Fresnel

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "oph_internal.h"
+#include "pipelines.h"
 
 int oph_fail(oph_ctx* ctx, int code, const char* fmt, ...) {
     if (ctx) {
@@ -141,6 +142,15 @@ int oph_create(int device, oph_ctx** out) {
         return OPH_ENOMEM;
     }
     c->wrp_flag = static_cast<unsigned*>(flag);
+    // the allocation-free WRP's static scratch: its address on this device
+    if (oph_wrp_noalloc_setup(c, device) != OPH_OK) {
+        hipFree(flag);
+        hipEventDestroy(c->ev_begin);
+        hipEventDestroy(c->ev_end);
+        hipStreamDestroy(c->stream);
+        delete c;
+        return OPH_EDEVICE;
+    }
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->cu_count = prop.multiProcessorCount;
     *out = c;

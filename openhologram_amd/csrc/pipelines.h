@@ -13,3 +13,7 @@ int oph_asm_device(oph_ctx* ctx, float2* field, int nx, int ny, int npx, int npy
 int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z_wrp, int mode,
                    const double* x, const double* y, const double* z, const double* amp, int64_t n,
                    float2* wrp, int accumulate, int check);
+
+// Records the device address of the allocation-free WRP's static scratch on
+// `device` (current); called once per context creation.
+int oph_wrp_noalloc_setup(oph_ctx* ctx, int device);

@@ -39,6 +39,7 @@
 // so no square root or division per pixel; points whose window makes the
 // truncated series worse than 1e-7 turns use an exact FP64 evaluation.
 #include <cmath>
+#include <mutex>
 #include <type_traits>
 
 #include "oph_internal.h"
@@ -92,6 +93,19 @@ struct WrpArgs {
 constexpr int BIN_THREADS = 256;
 constexpr int MAX_CHUNKS = 1024;     // chunks per call (bounds the [tile][chunk] count matrix)
 constexpr int MAX_TILES = 16384;     // LDS counters of one binning workgroup
+
+// points per binning workgroup for n points: one per thread while that keeps
+// the chunk count at most MAX_CHUNKS
+__host__ __device__ constexpr int64_t wrp_chunk(int64_t np) {
+    return ((np + MAX_CHUNKS - 1) / MAX_CHUNKS + BIN_THREADS - 1) / BIN_THREADS * BIN_THREADS;
+}
+// words of binning bookkeeping: chunk counts, totals, offsets (ntiles + 1),
+// overflow, chunk maxima, global max
+__host__ __device__ constexpr size_t wrp_book_words(int64_t nchunks, int64_t ntiles) {
+    return (size_t)nchunks * ntiles + 2 * (size_t)ntiles + 2 + nchunks + 1;
+}
+// first tile-list capacity (entries) for n points
+__host__ __device__ constexpr size_t wrp_list_entries(int64_t np) { return (size_t)24 * np + 65536; }
 
 __device__ __forceinline__ void tile_range(const WrpPoint& q, const WrpArgs& a, int& x0, int& x1, int& y0,
                                            int& y1) {
@@ -581,9 +595,22 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
 
 }  // namespace
 
-int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z_wrp, int mode,
-                   const double* x, const double* y, const double* z, const double* amp, int64_t n,
-                   float2* wrp, int accumulate, int check) {
+namespace {
+
+// Where one WRP call keeps its binning state.  From the context's grow-only
+// slots (oph_wrp_device) or from the static array of the allocation-free
+// entry point (fixed: a list overflow is then an error, not a retry).
+struct WrpScratch {
+    bool fixed;          // true: pts/book/list given, capacity list_cap
+    WrpPoint* pts;
+    unsigned* book;
+    unsigned* list;
+    size_t list_cap;     // entries
+};
+
+int wrp_core(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z_wrp, int mode,
+             const double* x, const double* y, const double* z, const double* amp, int64_t n, float2* wrp,
+             int accumulate, int check, const WrpScratch& ws) {
     WrpArgs a{};
     a.x = x;
     a.y = y;
@@ -606,16 +633,21 @@ int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, do
     const int ntiles = a.ntiles;
     const int64_t np = n > 0 ? n : 1;
     // one point per thread while that keeps the chunk count bounded
-    int64_t chunk = (np + MAX_CHUNKS - 1) / MAX_CHUNKS;
-    chunk = (chunk + BIN_THREADS - 1) / BIN_THREADS * BIN_THREADS;
+    const int64_t chunk = wrp_chunk(np);
     a.chunk = (int)chunk;
     const int nchunks = (int)((np + chunk - 1) / chunk);
     a.nchunks = nchunks;
-    a.pts = static_cast<WrpPoint*>(oph_scratch(ctx, SLOT_AUX0, sizeof(WrpPoint) * np));
     // bookkeeping: chunk counts, totals, offsets (ntiles + 1), overflow, chunk
     // maxima, global max
-    const size_t book = (size_t)nchunks * ntiles + 2 * (size_t)ntiles + 2 + nchunks + 1;
-    unsigned* bk = static_cast<unsigned*>(oph_scratch(ctx, SLOT_AUX1, sizeof(unsigned) * book));
+    const size_t book = wrp_book_words(nchunks, ntiles);
+    unsigned* bk;
+    if (ws.fixed) {
+        a.pts = ws.pts;
+        bk = ws.book;
+    } else {
+        a.pts = static_cast<WrpPoint*>(oph_scratch(ctx, SLOT_AUX0, sizeof(WrpPoint) * np));
+        bk = static_cast<unsigned*>(oph_scratch(ctx, SLOT_AUX1, sizeof(unsigned) * book));
+    }
     if (!a.pts || !bk) return OPH_ENOMEM;
     a.chunk_counts = bk;
     a.totals = bk + (size_t)nchunks * ntiles;
@@ -624,11 +656,11 @@ int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, do
     a.chunk_max = reinterpret_cast<float*>(a.overflow + 1);
     a.gmax = a.chunk_max + nchunks;
     a.sticky = check ? nullptr : ctx->wrp_flag;
-    size_t cap = ctx->slot_size[SLOT_TABLE2] / sizeof(unsigned);
-    const size_t want = (size_t)24 * np + 65536;
-    if (cap < want) cap = want;
+    size_t cap = ws.fixed ? ws.list_cap : ctx->slot_size[SLOT_TABLE2] / sizeof(unsigned);
+    const size_t want = wrp_list_entries(np);
+    if (!ws.fixed && cap < want) cap = want;
     for (int attempt = 0; attempt < 2; ++attempt) {
-        a.list = static_cast<unsigned*>(oph_scratch(ctx, SLOT_TABLE2, cap * sizeof(unsigned)));
+        a.list = ws.fixed ? ws.list : static_cast<unsigned*>(oph_scratch(ctx, SLOT_TABLE2, cap * sizeof(unsigned)));
         if (!a.list) return OPH_ENOMEM;
         a.capacity = (unsigned)(cap < 0xFFFFFFFFull ? cap : 0xFFFFFFFFull);
         OPH_LAUNCH(ctx, "wrp_bin_count", wrp_bin_count, dim3(nchunks), dim3(BIN_THREADS), 0, a);
@@ -671,10 +703,22 @@ int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, do
                                           ctx->stream));
         OPH_HIP_CHECK(ctx, hipStreamSynchronize(ctx->stream));
         if (!hdr[1]) break;
+        if (ws.fixed)
+            return oph_fail(ctx, OPH_ENOMEM, "WRP tile lists need %u entries, above the static capacity %zu; "
+                                             "the plane was left unwritten", hdr[0], cap);
         if (attempt == 1) return oph_fail(ctx, OPH_ENOMEM, "WRP tile lists overflowed at the exact size");
         cap = (size_t)hdr[0] + 65536;
     }
     return OPH_OK;
+}
+
+}  // namespace
+
+int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z_wrp, int mode,
+                   const double* x, const double* y, const double* z, const double* amp, int64_t n,
+                   float2* wrp, int accumulate, int check) {
+    return wrp_core(ctx, nx, ny, pitch, lambda, z_wrp, mode, x, y, z, amp, n, wrp, accumulate, check,
+                    WrpScratch{false, nullptr, nullptr, nullptr, 0});
 }
 
 extern "C" int oph_wrp(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z_wrp, int mode,
@@ -758,6 +802,92 @@ extern "C" int oph_wrp_hologram(oph_ctx* ctx, int n_side, double pitch, double l
                              pts->n, d, 0, !(flags & OPH_ASYNC))))
         return rc;
     if ((rc = oph_asm_device(ctx, d, n_side, n_side, np, np, pitch, pitch, lambda, distance, band_limit, work)))
+        return rc;
+    return oph_finish(ctx, flags);
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free WRP (oph_wrp_noalloc): the binning state lives in one
+// file-scope device array, `scratch`, sized for at most NOALLOC_MAX_POINTS
+// points on planes of at most NOALLOC_MAX_TILES 64x64 tiles (2048 x 2048),
+// with a fixed tile-list capacity.  Each device holds its own copy for the
+// life of the process.  In host code `scratch` names no device address: its
+// address on a device comes from hipGetSymbolAddress, taken once per device
+// (with that device current) when a context is created there, and only that
+// address is passed to kernels.  Every call on a device shares its copy and
+// overwrites everything it reads (records, counts, offsets, flags and lists
+// are all written by the binning passes before use), so nothing needs
+// clearing; calls that overlap on one device (two contexts' streams at once)
+// race on it.
+namespace {
+
+constexpr int64_t NOALLOC_MAX_POINTS = 131072;
+constexpr int NOALLOC_MAX_TILES = 1024;
+constexpr size_t NOALLOC_LIST = wrp_list_entries(NOALLOC_MAX_POINTS);
+constexpr size_t align256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NOALLOC_PTS_BYTES = align256(sizeof(WrpPoint) * NOALLOC_MAX_POINTS);
+constexpr size_t NOALLOC_BOOK_BYTES = align256(
+    sizeof(unsigned) * wrp_book_words((NOALLOC_MAX_POINTS + wrp_chunk(NOALLOC_MAX_POINTS) - 1) /
+                                          wrp_chunk(NOALLOC_MAX_POINTS),
+                                      NOALLOC_MAX_TILES));
+constexpr size_t NOALLOC_LIST_BYTES = align256(sizeof(unsigned) * NOALLOC_LIST);
+constexpr size_t NOALLOC_BYTES = NOALLOC_PTS_BYTES + NOALLOC_BOOK_BYTES + NOALLOC_LIST_BYTES;
+static_assert(NOALLOC_BYTES < (size_t)24 << 20, "the static WRP scratch stays below 24 MiB");
+
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex noalloc_mutex;
+void* noalloc_ptr[MAX_DEVICES] = {};
+
+}  // namespace
+
+// Called by oph_create with `device` current: records the device address of
+// `scratch` on that device (once per device).
+int oph_wrp_noalloc_setup(oph_ctx* ctx, int device) {
+    if (device < 0 || device >= MAX_DEVICES)
+        return oph_fail(ctx, OPH_ENOTSUP, "device %d: the allocation-free WRP supports devices 0..%d", device,
+                        MAX_DEVICES - 1);
+    std::lock_guard<std::mutex> lock(noalloc_mutex);
+    if (noalloc_ptr[device]) return OPH_OK;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        hipGetLastError();
+        return oph_fail(ctx, OPH_EDEVICE, "hipGetSymbolAddress of the static WRP scratch failed: %s",
+                        hipGetErrorString(e));
+    }
+    noalloc_ptr[device] = p;
+    return OPH_OK;
+}
+
+extern "C" int oph_wrp_noalloc(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z_wrp, int mode,
+                               const oph_points* pts, oph_c64* wrp, int flags) {
+    int rc = oph_enter(ctx);
+    if (rc) return rc;
+    if (!pts || !wrp) return oph_fail(ctx, OPH_EINVAL, "null argument");
+    if (!(flags & OPH_DEVICE_PTR))
+        return oph_fail(ctx, OPH_EINVAL, "oph_wrp_noalloc takes device pointers only (OPH_DEVICE_PTR)");
+    if (nx <= 0 || ny <= 0) return oph_fail(ctx, OPH_EINVAL, "bad WRP size");
+    const int64_t ntiles = (int64_t)((nx + TILE - 1) / TILE) * ((ny + TILE - 1) / TILE);
+    if (ntiles > NOALLOC_MAX_TILES)
+        return oph_fail(ctx, OPH_EINVAL, "WRP plane %dx%d has %lld tiles of 64x64; the allocation-free entry point "
+                                         "takes at most %d (2048 x 2048)",
+                        nx, ny, (long long)ntiles, NOALLOC_MAX_TILES);
+    if (pts->n < 0 || pts->n > NOALLOC_MAX_POINTS)
+        return oph_fail(ctx, OPH_EINVAL, "%lld points; the allocation-free entry point takes at most %lld",
+                        (long long)pts->n, (long long)NOALLOC_MAX_POINTS);
+    if (!(pitch > 0) || !(lambda > 0) || lambda >= 2 * pitch)
+        return oph_fail(ctx, OPH_EINVAL, "pitch and wavelength must be positive with lambda < 2 pitch");
+    if (mode < 0 || mode > 2) return oph_fail(ctx, OPH_EINVAL, "mode must be 0, 1 or 2");
+    if (pts->n > 0 && (!pts->x || !pts->y || !pts->z)) return oph_fail(ctx, OPH_EINVAL, "bad point arrays");
+    void* base = ctx->device < MAX_DEVICES ? noalloc_ptr[ctx->device] : nullptr;
+    if (!base) return oph_fail(ctx, OPH_EDEVICE, "the static WRP scratch has no address on device %d", ctx->device);
+    char* b = static_cast<char*>(base);
+    const WrpScratch ws{true, reinterpret_cast<WrpPoint*>(b), reinterpret_cast<unsigned*>(b + NOALLOC_PTS_BYTES),
+                        reinterpret_cast<unsigned*>(b + NOALLOC_PTS_BYTES + NOALLOC_BOOK_BYTES), NOALLOC_LIST};
+    if ((rc = wrp_core(ctx, nx, ny, pitch, lambda, z_wrp, mode, pts->x, pts->y, pts->z, pts->amp, pts->n,
+                       reinterpret_cast<float2*>(wrp), (flags & OPH_ACCUMULATE) ? 1 : 0, !(flags & OPH_ASYNC), ws)))
         return rc;
     return oph_finish(ctx, flags);
 }

@@ -115,6 +115,7 @@ def load():
         "oph_fft2d": (I, [P, P, I, I, I, I, I]),
         "oph_asm_propagate": (I, [P, P, I, I, D, D, D, D, I, I, I]),
         "oph_wrp": (I, [P, I, I, D, D, D, I, C.POINTER(Points), P, I]),
+        "oph_wrp_noalloc": (I, [P, I, I, D, D, D, I, C.POINTER(Points), P, I]),
         "oph_wrp_hologram": (I, [P, I, D, D, D, D, I, I, I, C.POINTER(Points), P, I]),
         "oph_depthmap": (I, [P, C.POINTER(DmSpec), I, P, P, P, P, I]),
         "oph_pc_fresnel": (I, [P, I, I, D, D, C.POINTER(Points), I, P, I]),
