@@ -209,6 +209,18 @@ int oph_wrp_hologram(oph_ctx* ctx, int n_side, double pitch, double lambda,
                      double z_wrp, double distance, int pad, int band_limit,
                      int mode, const oph_points* pts, oph_c64* out, int flags);
 
+/* ---- Huygens (Rayleigh-Sommerfeld) point sum on CWO's grid ----
+ * cwoPLS::Huygens (CWO/cwoPLS.cpp:316-368) with the point amplitude and
+ * initial phase:
+ *   U[i][j] = sum_n a_n exp(i (k r + ph + phi_n)) / r,
+ *   r = sqrt((X_j - x_n)^2 + (Y_i - y_n)^2 + (z_n + z)^2),
+ *   X_j = (j - nx/2) pitch, Y_i = (i - ny/2) pitch (row index i is y).
+ * The reference computes r in double and sums in float; here the phase is
+ * reduced in FP64 and summed as oph_pc_rs does.  flags: OPH_OUT_C128 or
+ * OPH_OUT_C64, OPH_DEVICE_PTR, OPH_ACCUMULATE, OPH_ASYNC. */
+int oph_pc_huygens(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z,
+                   double ph, const oph_points* pts, void* out, int flags);
+
 /* ---- Fresnel point-cloud hologram ----
  * The direct Fresnel sum of cwoPLS::Fresnel (CWO/cwoPLS.cpp:370-422) with the
  * point amplitude, initial phase and plane carrier:

@@ -21,6 +21,12 @@
 //   * large point sets and small grids split the points over grid.z chunks;
 //     a finalize kernel sums the chunks in a fixed order (deterministic),
 //     applies the per-pixel tilt term in FP64 and writes the output format.
+// The same kernel, with the amplitude divided by r per pixel and CWO's pixel
+// grid, is cwoPLS::Huygens (CWO/cwoPLS.cpp:316-368):
+//   U[i][j] = sum_n a_n exp(i (k r + ph + phi_n)) / r,
+//   X_j = (j - nx/2) p, Y_i = (i - ny/2) p, r = |(X_j - x_n, Y_i - y_n, z_n + z)|;
+// there 1/r = (1/r0)(1 + q/r0^2)^(-1/2) ~ (1/r0)(1 - q/(2 r0^2)) per pixel
+// (relative error 3/8 (q/r0^2)^2 < 1e-8 wherever the phase expansion holds).
 #include "oph_internal.h"
 
 namespace {
@@ -41,11 +47,14 @@ struct RsArgs {
     double sx, sy, sz, oz;  // scale, offset
     int nx, ny;
     double px, py;
-    double half_lx, half_ly;
+    // pixel centre X(col) = col px + x_off, Y(row) = ydir row py + y_off
+    double x_off, y_off, ydir;
+    double ph0;             // global phase, turns
     double inv_lambda;
     float2* partial;        // [chunks][ny][nx]
 };
 
+template <bool HUYGENS>
 __global__ __launch_bounds__(256) void rs_accumulate(RsArgs a) {
     __shared__ double s_x[PBATCH], s_y[PBATCH], s_zz[PBATCH], s_ph[PBATCH];
     __shared__ float s_a[PBATCH];
@@ -54,14 +63,14 @@ __global__ __launch_bounds__(256) void rs_accumulate(RsArgs a) {
     const int tid = ty * TPB_X + tx;
     const int c0 = blockIdx.x * TILE_X + tx * BX;
     const int r0 = blockIdx.y * TILE_Y + ty * BY;
-    // block centre: pixel X(col) = (col + 1/2) px - Lx/2, Y(row) = Ly/2 - (row + 1/2) py
-    const double X0 = (c0 + 0.5 * BX) * a.px - a.half_lx;
-    const double Y0 = a.half_ly - (r0 + 0.5 * BY) * a.py;
+    // block centre (KETI: X = (col + 1/2) px - Lx/2, Y = Ly/2 - (row + 1/2) py)
+    const double X0 = (c0 + 0.5 * (BX - 1)) * a.px + a.x_off;
+    const double Y0 = a.ydir * (r0 + 0.5 * (BY - 1)) * a.py + a.y_off;
     float ex[BX], ey[BY];
 #pragma unroll
     for (int c = 0; c < BX; ++c) ex[c] = (float)((c - 0.5 * (BX - 1)) * a.px);
 #pragma unroll
-    for (int r = 0; r < BY; ++r) ey[r] = (float)(-(r - 0.5 * (BY - 1)) * a.py);
+    for (int r = 0; r < BY; ++r) ey[r] = (float)(a.ydir * (r - 0.5 * (BY - 1)) * a.py);
     // largest |offset| in the block (for the expansion bound)
     const double hmax = 0.5 * sqrt((double)(BX - 1) * (BX - 1) * a.px * a.px +
                                    (double)(BY - 1) * (BY - 1) * a.py * a.py);
@@ -90,7 +99,7 @@ __global__ __launch_bounds__(256) void rs_accumulate(RsArgs a) {
                 const double zs = a.z[j] * a.sz + a.oz;
                 zz = zs * zs;
                 am = a.amp ? (float)a.amp[j] : 1.f;
-                ph = a.phase ? a.phase[j] * 0.15915494309189535 : 0.0;  // rad -> turns
+                ph = (a.phase ? a.phase[j] * 0.15915494309189535 : 0.0) + a.ph0;  // rad -> turns
             }
             s_x[tid] = xs;
             s_y[tid] = ys;
@@ -117,6 +126,9 @@ __global__ __launch_bounds__(256) void rs_accumulate(RsArgs a) {
             // expansion bound: u = q/r0^2 <= (2 h rho + h^2)/r0^2; the
             // third-order term r0 u^3/(16 lambda) and the first-order size
             // c1 q (FP32 resolution) decide the path
+            // Huygens amplitude: am/r0 (1 - q/(2 r0^2)) per pixel
+            const float amr = HUYGENS ? am * yrf : am;
+            const float hq = HUYGENS ? -0.5f * yrf * yrf : 0.f;
             const float rhof = __builtin_sqrtf(fmaf(dxf, dxf, dyf * dyf));
             const float qmax = fmaf(2.f * hmaxf, rhof, hmaxf * hmaxf);
             const float u = qmax * yrf * yrf;
@@ -133,8 +145,9 @@ __global__ __launch_bounds__(256) void rs_accumulate(RsArgs a) {
                     for (int c = 0; c < BX; ++c) {
                         const float q = A[c] + B[r];
                         const float ph = fmaf(q, fmaf(-c2, q, c1), f0);
-                        re[r][c] = fmaf(am, cos_turns(ph), re[r][c]);
-                        im[r][c] = fmaf(am, sin_turns(ph), im[r][c]);
+                        const float ap = HUYGENS ? amr * fmaf(hq, q, 1.f) : am;
+                        re[r][c] = fmaf(ap, cos_turns(ph), re[r][c]);
+                        im[r][c] = fmaf(ap, sin_turns(ph), im[r][c]);
                     }
             } else {
 #pragma unroll
@@ -142,12 +155,14 @@ __global__ __launch_bounds__(256) void rs_accumulate(RsArgs a) {
 #pragma unroll
                     for (int c = 0; c < BX; ++c) {
                         const double dx = dx0 + (c - 0.5 * (BX - 1)) * a.px;
-                        const double dy = dy0 - (r - 0.5 * (BY - 1)) * a.py;
+                        const double dy = dy0 + a.ydir * (r - 0.5 * (BY - 1)) * a.py;
                         const double s = fma(dx, dx, fma(dy, dy, s_zz[j]));
-                        const double t = fma(s * rsqrt_f64(s), a.inv_lambda, s_ph[j]);
+                        const double ys = rsqrt_f64(s);
+                        const double t = fma(s * ys, a.inv_lambda, s_ph[j]);
                         const float ph = (float)(t - rint(t));
-                        re[r][c] = fmaf(am, cos_turns(ph), re[r][c]);
-                        im[r][c] = fmaf(am, sin_turns(ph), im[r][c]);
+                        const float ap = HUYGENS ? (float)(am * ys) : am;
+                        re[r][c] = fmaf(ap, cos_turns(ph), re[r][c]);
+                        im[r][c] = fmaf(ap, sin_turns(ph), im[r][c]);
                     }
             }
         }
@@ -218,14 +233,25 @@ __global__ __launch_bounds__(256) void rs_finalize(FinArgs f) {
 
 }  // namespace
 
-extern "C" int oph_pc_rs(oph_ctx* ctx, const oph_spec* spec, const oph_points* pts, void* out,
-                         int flags) {
-    int rc = oph_enter(ctx);
-    if (rc) return rc;
-    if (!spec || !pts || !out) return oph_fail(ctx, OPH_EINVAL, "null argument");
-    if (spec->nx <= 0 || spec->ny <= 0 || spec->nx > 65536 || spec->ny > 65536)
-        return oph_fail(ctx, OPH_EINVAL, "image size %dx%d out of range", spec->nx, spec->ny);
-    if (!(spec->pitch_x > 0) || !(spec->pitch_y > 0) || !(spec->lambda > 0))
+namespace {
+
+// Shared host side of oph_pc_rs and oph_pc_huygens: argument checks, point
+// staging, the accumulate launch (KETI or CWO pixel grid, with or without
+// 1/r) and the finalize launch.
+struct RsJob {
+    int nx, ny;
+    double px, py, lambda;
+    double scale[3], offset_z;
+    double tilt_deg[2];
+    bool cwo_grid, huygens;
+    double ph0;  // global phase (radians)
+};
+
+int rs_run(oph_ctx* ctx, const RsJob& J, const oph_points* pts, void* out, int flags) {
+    if (!pts || !out) return oph_fail(ctx, OPH_EINVAL, "null argument");
+    if (J.nx <= 0 || J.ny <= 0 || J.nx > 65536 || J.ny > 65536)
+        return oph_fail(ctx, OPH_EINVAL, "image size %dx%d out of range", J.nx, J.ny);
+    if (!(J.px > 0) || !(J.py > 0) || !(J.lambda > 0))
         return oph_fail(ctx, OPH_EINVAL, "pitch and wavelength must be positive");
     if (pts->n < 0 || (pts->n > 0 && (!pts->x || !pts->y || !pts->z)))
         return oph_fail(ctx, OPH_EINVAL, "bad point arrays");
@@ -235,7 +261,7 @@ extern "C" int oph_pc_rs(oph_ctx* ctx, const oph_spec* spec, const oph_points* p
     if ((flags & OPH_ASYNC) && !(flags & OPH_DEVICE_PTR))
         return oph_fail(ctx, OPH_EINVAL, "OPH_ASYNC needs device pointers");
 
-    const int nx = spec->nx, ny = spec->ny;
+    const int nx = J.nx, ny = J.ny;
     const int64_t n = pts->n;
     const size_t npix = (size_t)nx * ny;
     const size_t pbytes = (size_t)(n > 0 ? n : 1) * sizeof(double);
@@ -252,17 +278,28 @@ extern "C" int oph_pc_rs(oph_ctx* ctx, const oph_spec* spec, const oph_points* p
     if (n > 0 && (!a.x || !a.y || !a.z)) return OPH_ENOMEM;
     if (n > 0 && ((pts->amp && !a.amp) || (pts->phase && !a.phase))) return OPH_ENOMEM;
 
-    a.sx = spec->scale[0];
-    a.sy = spec->scale[1];
-    a.sz = spec->scale[2];
-    a.oz = spec->offset_z;
+    a.sx = J.scale[0];
+    a.sy = J.scale[1];
+    a.sz = J.scale[2];
+    a.oz = J.offset_z;
     a.nx = nx;
     a.ny = ny;
-    a.px = spec->pitch_x;
-    a.py = spec->pitch_y;
-    a.half_lx = spec->pitch_x * nx / 2.0;
-    a.half_ly = spec->pitch_y * ny / 2.0;
-    a.inv_lambda = 1.0 / spec->lambda;
+    a.px = J.px;
+    a.py = J.py;
+    const double half_lx = J.px * nx / 2.0, half_ly = J.py * ny / 2.0;
+    if (J.cwo_grid) {
+        // cwoPLS: X = (col - nx/2) p, Y = (row - ny/2) p (CWO/cwoPLS.cpp:350-351)
+        a.x_off = -(double)(nx / 2) * J.px;
+        a.y_off = -(double)(ny / 2) * J.py;
+        a.ydir = 1.0;
+    } else {
+        // KETI: X = (col + 1/2) p - Lx/2, Y = Ly/2 - (row + 1/2) p
+        a.x_off = 0.5 * J.px - half_lx;
+        a.y_off = half_ly - 0.5 * J.py;
+        a.ydir = -1.0;
+    }
+    a.ph0 = J.ph0 * 0.15915494309189535;
+    a.inv_lambda = 1.0 / J.lambda;
 
     const int gx = (nx + TILE_X - 1) / TILE_X, gy = (ny + TILE_Y - 1) / TILE_Y;
     const int64_t tiles = (int64_t)gx * gy;
@@ -278,8 +315,12 @@ extern "C" int oph_pc_rs(oph_ctx* ctx, const oph_spec* spec, const oph_points* p
     a.partial = static_cast<float2*>(oph_scratch(ctx, SLOT_WORK0, chunks * npix * sizeof(float2)));
     if (!a.partial) return OPH_ENOMEM;
 
-    OPH_LAUNCH(ctx, "rs_accumulate", rs_accumulate, dim3(gx, gy, (unsigned)chunks),
-               dim3(TPB_X, TPB_Y), 0, a);
+    if (J.huygens)
+        OPH_LAUNCH(ctx, "huygens_accumulate", rs_accumulate<true>, dim3(gx, gy, (unsigned)chunks),
+                   dim3(TPB_X, TPB_Y), 0, a);
+    else
+        OPH_LAUNCH(ctx, "rs_accumulate", rs_accumulate<false>, dim3(gx, gy, (unsigned)chunks),
+                   dim3(TPB_X, TPB_Y), 0, a);
 
     const size_t obytes = npix * (fmt == OPH_OUT_C128 ? 16 : fmt == OPH_OUT_C64 ? 8 : 4);
     void* dout = out;
@@ -296,10 +337,10 @@ extern "C" int oph_pc_rs(oph_ctx* ctx, const oph_spec* spec, const oph_points* p
     f.ny = ny;
     f.px = a.px;
     f.py = a.py;
-    f.half_lx = a.half_lx;
-    f.half_ly = a.half_ly;
-    f.stx = sin(spec->tilt_deg[0] * (M_PI / 180.0));
-    f.sty = sin(spec->tilt_deg[1] * (M_PI / 180.0));
+    f.half_lx = half_lx;
+    f.half_ly = half_ly;
+    f.stx = sin(J.tilt_deg[0] * (M_PI / 180.0));
+    f.sty = sin(J.tilt_deg[1] * (M_PI / 180.0));
     f.inv_lambda = a.inv_lambda;
     f.tilt = (f.stx != 0.0 || f.sty != 0.0);
     f.out = dout;
@@ -309,4 +350,43 @@ extern "C" int oph_pc_rs(oph_ctx* ctx, const oph_spec* spec, const oph_points* p
     if (!(flags & OPH_DEVICE_PTR))
         OPH_HIP_CHECK(ctx, hipMemcpyAsync(out, dout, obytes, hipMemcpyDeviceToHost, ctx->stream));
     return oph_finish(ctx, flags);
+}
+
+}  // namespace
+
+extern "C" int oph_pc_rs(oph_ctx* ctx, const oph_spec* spec, const oph_points* pts, void* out,
+                         int flags) {
+    int rc = oph_enter(ctx);
+    if (rc) return rc;
+    if (!spec) return oph_fail(ctx, OPH_EINVAL, "null argument");
+    RsJob J{};
+    J.nx = spec->nx;
+    J.ny = spec->ny;
+    J.px = spec->pitch_x;
+    J.py = spec->pitch_y;
+    J.lambda = spec->lambda;
+    for (int k = 0; k < 3; ++k) J.scale[k] = spec->scale[k];
+    J.offset_z = spec->offset_z;
+    J.tilt_deg[0] = spec->tilt_deg[0];
+    J.tilt_deg[1] = spec->tilt_deg[1];
+    return rs_run(ctx, J, pts, out, flags);
+}
+
+extern "C" int oph_pc_huygens(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z, double ph,
+                              const oph_points* pts, void* out, int flags) {
+    int rc = oph_enter(ctx);
+    if (rc) return rc;
+    if ((flags & OPH_OUT_MASK) == OPH_OUT_FRINGE)
+        return oph_fail(ctx, OPH_EINVAL, "oph_pc_huygens writes complex output only");
+    RsJob J{};
+    J.nx = nx;
+    J.ny = ny;
+    J.px = J.py = pitch;
+    J.lambda = lambda;
+    J.scale[0] = J.scale[1] = J.scale[2] = 1.0;
+    J.offset_z = z;
+    J.cwo_grid = true;
+    J.huygens = true;
+    J.ph0 = ph;
+    return rs_run(ctx, J, pts, out, flags);
 }

@@ -112,6 +112,7 @@ def load():
         "oph_spec_from_keti_config": (I, [C.c_char_p, C.POINTER(Spec)]),
         "oph_load_keti_points": (I, [C.c_char_p, P, I64, C.POINTER(I64)]),
         "oph_pc_rs": (I, [P, C.POINTER(Spec), C.POINTER(Points), P, I]),
+        "oph_pc_huygens": (I, [P, I, I, D, D, D, D, C.POINTER(Points), P, I]),
         "oph_fft2d": (I, [P, P, I, I, I, I, I]),
         "oph_asm_propagate": (I, [P, P, I, I, D, D, D, D, I, I, I]),
         "oph_wrp": (I, [P, I, I, D, D, D, I, C.POINTER(Points), P, I]),
@@ -296,6 +297,19 @@ class Context:
         del keep
         return out
 
+
+    def huygens(self, nx, ny, pitch, lambda_, z, x, y, zz, amp=None, phase=None, ph=0.0, out=None,
+                fmt=OUT_C128, accumulate=False, device=False, asynchronous=False):
+        """Huygens point sum on CWO's grid (oph_pc_huygens), (ny, nx) complex."""
+        pts, keep = _points(x, y, zz, amp, phase, device)
+        dt = np.complex128 if fmt == OUT_C128 else np.complex64
+        if out is None:
+            out = self.empty((ny, nx), dt) if device else np.zeros((ny, nx), dt)
+        flags = fmt | (DEVICE_PTR if device else 0) | (ASYNC if asynchronous else 0) | \
+            (ACCUMULATE if accumulate else 0)
+        self.check(self.lib.oph_pc_huygens(self.h, nx, ny, pitch, lambda_, z, ph, C.byref(pts), ptr(out), flags))
+        del keep
+        return out
 
     def fft2d(self, data, direction=-1, normalize=False, device=False, asynchronous=False):
         """In-place complex64 2-D FFT (oph_fft2d).  Host arrays are copied."""

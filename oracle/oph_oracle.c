@@ -30,6 +30,9 @@
  *                         spherical wave carried from z1 to z1 + dz, paraxial
  *                         limit against the Fresnel transfer function, padded
  *                         result = crop(IFFT(orc_asm_transfer . FFT(embed)))
+ *   orc_huygens           long-double brute force on tiny cases; single point
+ *                         against orc_rs (|U| r = a, same phase) with the grids
+ *                         aligned (half pixel, row flip)
  *   orc_wrp               single-point window vs the closed-form spherical wave,
  *                         window size and clipping, linearity
  *   orc_dm_encode         explicit inverse-DFT sum over the kept spectrum rows
@@ -256,6 +259,40 @@ void orc_fresnel(int nx, int ny, double pitch, double lambda, int64_t n,
         }
         out[q].re = re;
         out[q].im = im;
+    }
+}
+
+/*
+ * Huygens point sum, FP64, CWO/cwoPLS.cpp:316-368 (cwoPLS::Huygens) with the
+ * point amplitude and initial phase (NULL: 1 and 0):
+ *   pixel  X = (j - Nx/2) p,  Y = (i - Ny/2) p     (:350-351, zero offsets)
+ *   dz = z_n + z                                  (:352)
+ *   U[i][j] = sum_n a_n exp(i (k r + ph + phi_n)) / r,  r = |(X - x, Y - y, dz)|
+ */
+void orc_huygens(int nx, int ny, double pitch, double lambda, double z, double ph, int64_t n,
+                 const double* x, const double* y, const double* zz, const double* amp,
+                 const double* phi, c128* out, int nthreads) {
+    set_threads(nthreads);
+    int i;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (i = 0; i < ny; ++i) {
+        const double Y = (i - ny / 2) * pitch;
+        for (int j = 0; j < nx; ++j) {
+            const double X = (j - nx / 2) * pitch;
+            double re = 0.0, im = 0.0;
+            for (int64_t m = 0; m < n; ++m) {
+                const double dx = X - x[m], dy = Y - y[m], dz = zz[m] + z;
+                const double r = sqrt(dx * dx + dy * dy + dz * dz);
+                double t = r / lambda;
+                t -= floor(t);
+                const double p = TWO_PI * t + ph + (phi ? phi[m] : 0.0);
+                const double a = (amp ? amp[m] : 1.0) / r;
+                re += a * cos(p);
+                im += a * sin(p);
+            }
+            out[(size_t)i * nx + j].re = re;
+            out[(size_t)i * nx + j].im = im;
+        }
     }
 }
 

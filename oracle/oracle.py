@@ -53,6 +53,8 @@ def lib():
         _lib.orc_dm_reconstruct.argtypes = [P, C.c_int, C.c_int] + [C.c_double] * 9 + [P, C.c_int]
         _lib.orc_fresnel.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_int64,
                                      P, P, P, P, P, P, P, C.c_int64, P, C.c_int]
+        _lib.orc_huygens.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_int64, P, P, P, P, P, P, C.c_int]
         _lib.orc_wrp.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                  C.c_int, C.c_int64, P, P, P, P, P, C.c_int]
         _lib.orc_fft2d.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
@@ -164,6 +166,15 @@ def fresnel(nx, ny, pitch, lam, x, y, z, amp, phase=None, rows=None, cols=None, 
         npix = 0
     lib().orc_fresnel(nx, ny, pitch, lam, len(x), _p(x), _p(y), _p(z), _p(amp), _p(phase),
                       _p(rows), _p(cols), npix, _p(out), nthreads)
+    return out
+
+
+def huygens(nx, ny, pitch, lam, z, x, y, zz, amp=None, phase=None, ph=0.0, nthreads=0):
+    """FP64 cwoPLS::Huygens field with amplitude and phase, (ny, nx) complex128."""
+    x, y, zz, amp, phase = map(_d, (x, y, zz, amp, phase))
+    out = np.zeros((ny, nx), np.complex128)
+    lib().orc_huygens(nx, ny, pitch, lam, z, ph, len(x), _p(x), _p(y), _p(zz), _p(amp), _p(phase), _p(out),
+                      nthreads)
     return out
 
 

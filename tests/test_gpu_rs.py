@@ -97,3 +97,23 @@ def test_rs_invalid_args(ctx):
         ctx.pc_rs(dict(nx=0, ny=64, pitch=8e-6, lambda_=532e-9), [0.0], [0.0], [0.1], [1.0])
     with pytest.raises(oph.OphError):
         ctx.pc_rs(dict(nx=64, ny=64, pitch=8e-6, lambda_=-1.0), [0.0], [0.0], [0.1], [1.0])
+
+
+@pytest.mark.parametrize("nx,ny,z", [(256, 192, 0.1), (130, 70, 0.02)])
+def test_huygens_vs_oracle(ctx, nx, ny, z):
+    """cwoPLS::Huygens (1/r amplitude, CWO grid) against the FP64 oracle,
+    including points close enough for the per-pixel FP64 path."""
+    print(f"case: huygens {nx}x{ny} z {z}", flush=True)
+    rng = np.random.default_rng(nx)
+    n = 700
+    x, y = rng.uniform(-5e-4, 5e-4, (2, n))
+    zz = rng.uniform(-0.005, 0.005, n)
+    a, ph = rng.uniform(0, 1, n), rng.uniform(0, 2 * np.pi, n)
+    U = ctx.huygens(nx, ny, 8e-6, 532e-9, z, x, y, zz, a, ph, ph=0.25)
+    ref = O.huygens(nx, ny, 8e-6, 532e-9, z, x, y, zz, a, ph, ph=0.25)
+    err = rel_l2(U, ref)
+    print("rel-L2", err)
+    assert err < 1e-5
+    V = ctx.huygens(nx, ny, 8e-6, 532e-9, z, x, y, zz, a, ph, ph=0.25, fmt=oph.OUT_C64, out=U.astype(np.complex64),
+                    accumulate=True)
+    assert rel_l2(V, 2 * ref) < 1e-5

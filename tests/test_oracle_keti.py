@@ -104,3 +104,38 @@ def test_fp64_symmetry_and_linearity():
 def test_loader_files_present():
     assert os.path.exists(os.path.join(GOLDEN, "keti_TestPointCloud.dat"))
     assert os.path.exists(os.path.join(GOLDEN, "keti_TestSpec.config"))
+
+
+def _huygens_brute(nx, ny, p, lam, z, ph, x, y, zz, a, phi):
+    L = np.longdouble
+    out = np.zeros((ny, nx), np.complex128)
+    for i in range(ny):
+        for j in range(nx):
+            dx, dy, dz = L(j - nx // 2) * L(p) - x.astype(L), L(i - ny // 2) * L(p) - y.astype(L), zz.astype(L) + L(z)
+            r = np.sqrt(dx * dx + dy * dy + dz * dz)
+            t = (r / L(lam)) % 1
+            out[i, j] = np.sum(a / r.astype(np.float64) * np.exp(1j * (2 * np.pi * t.astype(np.float64) + ph + phi)))
+    return out
+
+
+def test_huygens_brute_force_and_rs():
+    """cwoPLS::Huygens (CWO/cwoPLS.cpp:316-368) restated: against a
+    long-double per-pair sum, and tied to the reference-pinned RS oracle: for
+    one point, U r = a exp(i(k r + phi)) is orc_rs's field on the same
+    pixels (RS centres are (col + 1/2) p - Lx/2 and Ly/2 - (row + 1/2) p, so
+    the point moves by -p/2 and the rows flip)."""
+    rng = np.random.default_rng(6)
+    n, nx, ny, p, lam = 6, 9, 7, 8e-6, 633e-9
+    x, y = rng.uniform(-3e-5, 3e-5, (2, n))
+    zz = rng.uniform(0.02, 0.03, n)
+    a, phi = rng.uniform(0.2, 1, n), rng.uniform(0, 2 * np.pi, n)
+    U = O.huygens(nx, ny, p, lam, 0.07, x, y, zz, a, phi, ph=0.3)
+    assert np.abs(U - _huygens_brute(nx, ny, p, lam, 0.07, 0.3, x, y, zz, a, phi)).max() < 1e-8 * np.abs(U).max()
+    # one point against orc_rs
+    x1, y1, z1 = np.array([1.2e-5]), np.array([-2.5e-5]), np.array([0.1])
+    H = O.huygens(32, 24, p, lam, 0.0, x1 - p / 2, y1 - p / 2, z1, [0.8], [0.4])[::-1]
+    R = O.rs(dict(nx=32, ny=24, pitch=p, lambda_=lam), x1, y1, z1, [0.8], [0.4])
+    Y, X = np.mgrid[0:24, 0:32]
+    r = np.sqrt(((X + 0.5) * p - 16 * p - x1[0]) ** 2 + (12 * p - (Y + 0.5) * p - y1[0]) ** 2 + z1[0] ** 2)
+    assert np.abs(H * r - R).max() < 1e-9
+    assert np.abs(np.conj(H) * r - R).max() > 0.5
