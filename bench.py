@@ -25,9 +25,19 @@ METRIC = "holograms/sec @4096² at rel-L2≤1e-3 vs FP64; point·pixel/s; HBM GB
 HBM_PEAK_GBS = 8000.0
 VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9      # VALU issue peak, unpacked FP32 lane-ops/s (3.93e13)
 # Wave-level VALU instructions per launch at the bench sizes, from rocprofv3
-# SQ_INSTS_VALU (profiles/valu_counters_rs_wrp.md); valid for the kernels as
-# built here, re-measure after changing them.
+# SQ_INSTS_VALU runs (profiles/valu_counters_rs_wrp.md, the `rocprofv3 --pmc
+# SQ_INSTS_VALU ... -- python3 bench.py --workload <rs|wrp> --only --steps 2
+# --warmup 1 --no-cpu` commands recorded there); valid for the kernels as
+# built at that time: re-measure after changing them.
 PROFILED_VALU_INSTS = {"rs_accumulate": 181585920, "asm_cols": 25419776, "wrp_gather": 33586086}
+PROFILE_SOURCES = {"valu": "profiles/valu_counters_rs_wrp.md (rocprofv3 --pmc SQ_INSTS_VALU)",
+                   "traffic": "profiles/hbm_traffic_wrp_depthmap.md (rocprofv3 --pmc FETCH_SIZE, then "
+                              "WRITE_SIZE, calibrated on copy_kernel)"}
+# RS issue model: per point-pixel pair the kernel needs at least 2 FMAs for
+# the phase polynomial, 1 add (q), 2 FMAs of accumulation and sin + cos,
+# which issue at a quarter of the rate (4 slots each): 13 issue slots per
+# pair at the VALU issue peak.
+RS_SLOTS_PER_PAIR = 13
 
 
 def rel_l2(a, b):
@@ -65,16 +75,20 @@ def bench_rs(ctx, args):
     pairs = 1000 * spec["nx"] * spec["ny"]
     k_avg = kms / max(n, 1) / 1e3
     lane_ops = PROFILED_VALU_INSTS["rs_accumulate"] * 64
+    peak_pairs = VALU_LANE_OPS / RS_SLOTS_PER_PAIR
     line = {
         "metric": METRIC, "value": args.steps / wall, "unit": "holograms/s",
         "ms_per_step": wall / args.steps * 1e3, "dtype": "f64-phase/f32-acc",
         "config": {"workload": "rs_pointcloud_1000pts_1024x1024_8um_532nm"},
         "point_pixel_per_s": pairs / (wall / args.steps),
-        # VALU issue: counted VALU lane-ops of rs_accumulate per launch / its
-        # measured duration, against the chip's VALU issue peak
-        "roofline": {"kernel": "rs_accumulate", "bound": "valu", "achieved": lane_ops / k_avg / 1e12,
-                     "peak": VALU_LANE_OPS / 1e12, "unit": "Tlane-op/s", "frac": lane_ops / k_avg / VALU_LANE_OPS,
-                     "traffic": None, "valu_lane_ops_per_pair": lane_ops / pairs},
+        # point-pixel pairs per second of rs_accumulate against the pair rate
+        # the VALU issue peak allows at RS_SLOTS_PER_PAIR slots per pair
+        "roofline": {"kernel": "rs_accumulate", "bound": "valu", "achieved": pairs / k_avg / 1e12,
+                     "peak": peak_pairs / 1e12, "unit": "Tpair/s", "frac": pairs / k_avg / peak_pairs,
+                     "traffic": None, "slots_per_pair_model": RS_SLOTS_PER_PAIR,
+                     "valu_issue_frac_counted": lane_ops / k_avg / VALU_LANE_OPS,
+                     "valu_lane_ops_per_pair_counted": lane_ops / pairs,
+                     "counted_from": PROFILE_SOURCES["valu"]},
     }
     if not args.no_cpu:
         line["cpu_baseline"], small, ref = cpu_rs(spec)
@@ -85,20 +99,32 @@ def bench_rs(ctx, args):
 
 
 def cpu_rs(spec):
+    """Two CPU legs on the same 1000 points: the reference-faithful FP32 KETI
+    fringe loop (ophgen_pointcloud.cpp:195-249 restated, bit-exact with the
+    reference source) and the FP64 complex oracle, each on a band of rows,
+    scaled by pair count.  `value` is the reference-faithful one."""
     from oracle import oracle as O
     from openhologram_amd import scenes
     x, y, z, a, ph = scenes.rs_points(1000, seed=1)
     rows = 64
     s = dict(spec, ny=rows)
     cores = host_cores()
+    pairs = 1000 * spec["nx"] * rows
+    full = 1000 * spec["nx"] * spec["ny"]
+    spec13 = np.array([1, 1, 1, 0, spec["pitch"], spec["pitch"], spec["nx"], 256, spec["lambda_"], 0, 0, 0, 0],
+                      np.float32)
+    pts5 = np.stack([x, y, z, a, ph], axis=1).astype(np.float32)
+    t0 = time.perf_counter()
+    O.keti_fringe_f32(spec13, pts5, nthreads=cores)
+    dt32 = (time.perf_counter() - t0) * rows / 256
     t0 = time.perf_counter()
     ref = O.rs(s, x, y, z, a, ph, nthreads=cores)
     dt = time.perf_counter() - t0
-    pairs = 1000 * spec["nx"] * rows
-    full = 1000 * spec["nx"] * spec["ny"]
-    return {"value": (pairs / dt) / full, "unit": "holograms/s", "cores": cores, "kind": "port",
-            "sample": f"FP64 oracle, 1000 points x {rows} of {spec['ny']} rows "
-                      f"({pairs:.3g} point-pixel pairs, {dt:.2f} s), scaled by pair count"}, s, ref
+    return {"value": (pairs / dt32) / full, "unit": "holograms/s", "cores": cores, "kind": "port",
+            "sample": f"reference-faithful FP32 KETI fringe loop, 1000 points x 256 of {spec['ny']} rows "
+                      f"({dt32 * 256 / rows:.2f} s), scaled by pair count",
+            "fp64_oracle_value": (pairs / dt) / full,
+            "fp64_oracle_sample": f"FP64 complex oracle, 1000 points x {rows} rows ({dt:.2f} s), scaled"}, s, ref
 
 
 def _time_steps(ctx, step, args, kernels):
@@ -174,11 +200,12 @@ def bench_wrp(ctx, args):
     gbs = byt[dom] / t / 1e9
     line["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS, "traffic": PROFILED_TRAFFIC.get(dom),
-                        "algorithmic_bytes": byt[dom]}
+                        "traffic_from": PROFILE_SOURCES["traffic"], "algorithmic_bytes": byt[dom]}
     if dom in PROFILED_VALU_INSTS:
         # the column pass is limited by its transform arithmetic: its VALU
         # issue rate against the VALU peak, from counted instructions
         line["roofline"]["valu_frac"] = PROFILED_VALU_INSTS[dom] * 64 / t / VALU_LANE_OPS
+        line["roofline"]["valu_counted_from"] = PROFILE_SOURCES["valu"]
     pairs = 0
     # a plain device-to-device copy of the padded work-buffer size: the
     # achievable copy rate on this device (and the calibration kernel for the

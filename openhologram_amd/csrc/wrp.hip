@@ -352,6 +352,7 @@ struct GatherArgs {
     unsigned capacity;
     int nx, ny, ntx;
     double pitch, inv_lambda;
+    float lambda_f;
     float2* wrp;
     int accumulate;
 };
@@ -370,11 +371,13 @@ __device__ __forceinline__ void abs_range(int w0, int w1, int& lo, int& hi) {
     }
 }
 
-// Point record as the gather stages it in LDS (40 bytes).
+// Point record as the gather stages it in LDS (32 bytes): ip = index << 2 |
+// evaluation path; the amplitude slope cd = -sign|dz|/D of the spherical
+// modes is formed again from ks (sign|dz| = ks lambda, D = ks lambda + 0.05).
 struct GRec {
-    int tx, ty, w, exact;
-    int idx;
-    float f0, ks, p2dz2, ampd, cd;
+    int tx, ty, w;
+    unsigned ip;
+    float f0, ks, p2dz2, ampd;
 };
 constexpr int REC_BATCH = 128;
 
@@ -426,6 +429,10 @@ __device__ unsigned segment_end(const unsigned* pre, int nchunks, unsigned total
     return lo < nchunks ? pre[lo] : total;
 }
 
+// FRESNEL: the call's window mode is 2 (Fresnel phase, unit amplitude
+// factor); the spherical modes 0 and 1 use the other instantiation, so each
+// carries only its own two evaluation paths.
+template <bool FRESNEL_MODE>
 __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
     __shared__ unsigned long long acc[TILE * TILE];
     __shared__ GRec recs[REC_BATCH];
@@ -462,13 +469,11 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                 r.tx = q.tx;
                 r.ty = q.ty;
                 r.w = q.w;
-                r.exact = q.exact;
-                r.idx = (int)idx;
+                r.ip = idx << 2 | (unsigned)q.exact;
                 r.f0 = q.f0;
                 r.ks = q.ks;
                 r.p2dz2 = q.p2dz2;
                 r.ampd = q.ampd;
-                r.cd = q.cd;
                 recs[threadIdx.x] = r;
             }
             __syncthreads();
@@ -494,6 +499,11 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                 const int cbyte = ((ty - Y0) * TILE + (tx - X0)) * 8;
                 // the point's amplitude with the segment's fixed-point scale folded in
                 const float amps = q.ampd * scale;
+                const unsigned qidx = q.ip >> 2;
+                const int qpath = (int)(q.ip & 3u);
+                // spherical modes: amplitude slope -sign|dz|/D, sign|dz| = ks lambda
+                const float sdz = q.ks * g.lambda_f;
+                const float qcd = FRESNEL_MODE ? 0.f : -sdz / (sdz + 0.05f);
                 // one loop per evaluation path (the path is uniform over the point)
                 auto walk = [&](auto exact_tag, auto fresnel_tag) {
                     constexpr bool EXACT = decltype(exact_tag)::value;
@@ -508,7 +518,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                             re = amps * cos_turns(ph);
                             im = amps * sin_turns(ph);
                         } else if constexpr (FRESNEL) {
-                            const double dz = g.pts[q.idx].dz;
+                            const double dz = g.pts[qidx].dz;
                             const double dx = ax * g.pitch, dy = ay * g.pitch;
                             double t = (dz + (dx * dx + dy * dy) / (2.0 * dz)) * g.inv_lambda;
                             t -= rint(t);
@@ -519,11 +529,11 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                             const float u = fmaf(fx, fx, fy * fy) * q.p2dz2;
                             const float p = u * fmaf(u, fmaf(u, fmaf(u, -5.f / 128.f, 1.f / 16.f), -0.125f), 0.5f);
                             const float ph = fmaf(q.ks, p, q.f0);
-                            const float am = amps * fmaf(q.cd, p, 1.f);
+                            const float am = amps * fmaf(qcd, p, 1.f);
                             re = am * cos_turns(ph);
                             im = am * sin_turns(ph);
                         } else {
-                            const double dz = g.pts[q.idx].dz;
+                            const double dz = g.pts[qidx].dz;
                             const double dx = ax * g.pitch, dy = ay * g.pitch;
                             const double sgn = dz > 0.0 ? 1.0 : -1.0;
                             const double r = sgn * sqrt(dx * dx + dy * dy + dz * dz);
@@ -559,11 +569,16 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                         }
                     }
                 };
-                switch (q.exact) {
-                    case PATH_SERIES: walk(std::false_type{}, std::false_type{}); break;
-                    case PATH_EXACT: walk(std::true_type{}, std::false_type{}); break;
-                    case PATH_FRESNEL: walk(std::false_type{}, std::true_type{}); break;
-                    default: walk(std::true_type{}, std::true_type{}); break;
+                if constexpr (FRESNEL_MODE) {
+                    if (qpath == PATH_FRESNEL)
+                        walk(std::false_type{}, std::true_type{});
+                    else
+                        walk(std::true_type{}, std::true_type{});
+                } else {
+                    if (qpath == PATH_SERIES)
+                        walk(std::false_type{}, std::false_type{});
+                    else
+                        walk(std::true_type{}, std::false_type{});
                 }
             }
         }
@@ -631,6 +646,8 @@ int wrp_core(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z
     a.grating = tan(asin(lambda / (2.0 * pitch)));
     a.mode = mode;
     const int ntiles = a.ntiles;
+    if (n > (int64_t)1 << 30)
+        return oph_fail(ctx, OPH_EINVAL, "%lld points: a WRP call takes at most 2^30", (long long)n);
     const int64_t np = n > 0 ? n : 1;
     // one point per thread while that keeps the chunk count bounded
     const int64_t chunk = wrp_chunk(np);
@@ -688,9 +705,13 @@ int wrp_core(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z
         g.ntx = a.ntx;
         g.pitch = pitch;
         g.inv_lambda = a.inv_lambda;
+        g.lambda_f = (float)lambda;
         g.wrp = wrp;
         g.accumulate = accumulate;
-        OPH_LAUNCH(ctx, "wrp_gather", wrp_gather, dim3(ntiles), dim3(GATHER_THREADS), 0, g);
+        if (mode == 2)
+            OPH_LAUNCH(ctx, "wrp_gather", wrp_gather<true>, dim3(ntiles), dim3(GATHER_THREADS), 0, g);
+        else
+            OPH_LAUNCH(ctx, "wrp_gather", wrp_gather<false>, dim3(ntiles), dim3(GATHER_THREADS), 0, g);
         // the list size is only known on the device; a capacity overflow is
         // detected after the fact (the gather then writes nothing, so an
         // accumulating call is still intact) and the pass is redone with the
