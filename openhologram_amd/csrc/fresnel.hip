@@ -170,7 +170,8 @@ __global__ void fresnel_sum_chunks(const float2* partial, int chunks, int64_t np
 // MFMA contraction.  A workgroup of 4 waves owns a 128 x 128 output tile; for
 // each chunk of 32 points it writes the chirp factors A (128 rows of y) and
 // B (128 columns of x) to LDS as 16-bit planes [re_hi, im_hi, re_lo, im_lo]
-// laid out [row][point], then each wave contracts a 64 x 64 sub-tile with
+// laid out [row][point] (generated by a phase recurrence, below), then each
+// wave contracts a 64 x 64 sub-tile with
 // v_mfma_f32_32x32x16 (lane l supplies row l%32, points 8(l/32)..+7 of A and
 // column l%32, the same points of B; accumulator register v of lane l is
 // output row 8(v/4) + 4(l/32) + v%4, column l%32).
@@ -248,11 +249,6 @@ __global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
             cr[x][y] = (f32x16)0.f;
             ci[x][y] = (f32x16)0.f;
         }
-    // generation role of this thread: one row (A) / column (B), 16 points
-    const int gr = tid & (MT - 1), gh = tid >> 7;
-    const double Yr = (i0 + gr - a.ny / 2) * a.p;
-    const double Xc = (j0 + gr - a.nx / 2) * a.p;
-    const bool row_ok = i0 + gr < a.ny, col_ok = j0 + gr < a.nx;
 
     // the next chunk's points are loaded into registers while the current
     // chunk is generated and multiplied (the loop issues no other memory
@@ -270,46 +266,53 @@ __global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
         }
         __syncthreads();
         if (tid < KCH && k0 + KCH < a.npad) nq = a.pts[k0 + KCH + tid];
-        // KCH/2 points of this thread's row/column, written as 8-element vectors
+        // Chirp factors by a phase recurrence: thread (k, gph) generates
+        // point k's entries on rows (and columns) gph + G m, m < MT/G.  Along
+        // that run the phase c (Y - y)^2 has constant second difference
+        // delta = 2 c (G p)^2, so with R = exp(i 2 pi (phi_{m+1} - phi_m)),
+        // A_{m+1} = A_m R_m and R_{m+1} = R_m exp(i 2 pi delta): two complex
+        // multiplies per entry; the anchor phi_0, the first step and delta
+        // are reduced in FP64 once per run (phases of ~1e4 turns).
+        {
+            constexpr int G = 256 / KCH;     // row phases per point
+            constexpr int M = MT / G;        // entries per run
+            const int k = tid % KCH, gph = tid / KCH;
+            const double c = p_c[k];
+            const double sp = (double)G * a.p;
+            double td = 2.0 * c * sp * sp;
+            td -= rint(td);
+            const float2 E = make_float2(cos_turns((float)td), sin_turns((float)td));
+            // one run over rows (A, with the point weight) and one over columns (B)
 #pragma unroll
-        for (int hb = 0; hb < KCH / 16; ++hb) {
-            const int kb0 = gh * (KCH / 2) + hb * 8;
-            V va[NP], vb[NP];
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                const int kk = kb0 + e;
-                const double c = p_c[kk];
-                // A: w exp(i 2 pi c (Y - y)^2)
-                const double dy = Yr - p_y[kk];
-                double ty = c * dy * dy;
-                ty -= rint(ty);
-                const float cy = cos_turns((float)ty), sy = sin_turns((float)ty);
-                const float wr_ = p_wr[kk], wi_ = p_wi[kk];
-                const float are = row_ok ? wr_ * cy - wi_ * sy : 0.f;
-                const float aim = row_ok ? wr_ * sy + wi_ * cy : 0.f;
-                // B: exp(i 2 pi c (X - x)^2)
-                const double dx = Xc - p_x[kk];
-                double tx = c * dx * dx;
-                tx -= rint(tx);
-                const float bre = col_ok ? cos_turns((float)tx) : 0.f;
-                const float bim = col_ok ? sin_turns((float)tx) : 0.f;
-                const T arh = Mfma<T>::cvt(are), aih = Mfma<T>::cvt(aim);
-                const T brh = Mfma<T>::cvt(bre), bih = Mfma<T>::cvt(bim);
-                va[0][e] = arh;
-                va[1][e] = aih;
-                vb[0][e] = brh;
-                vb[1][e] = bih;
-                if constexpr (SPLIT) {
-                    va[2][e] = Mfma<T>::cvt(are - Mfma<T>::back(arh));
-                    va[3][e] = Mfma<T>::cvt(aim - Mfma<T>::back(aih));
-                    vb[2][e] = Mfma<T>::cvt(bre - Mfma<T>::back(brh));
-                    vb[3][e] = Mfma<T>::cvt(bim - Mfma<T>::back(bih));
+            for (int ab = 0; ab < 2; ++ab) {
+                const double d0 = ab == 0 ? (i0 + gph - a.ny / 2) * a.p - p_y[k] : (j0 + gph - a.nx / 2) * a.p - p_x[k];
+                double t0 = c * d0 * d0;
+                t0 -= rint(t0);
+                double t1 = c * (2.0 * d0 * sp + sp * sp);
+                t1 -= rint(t1);
+                float2 v = make_float2(cos_turns((float)t0), sin_turns((float)t0));
+                if (ab == 0) {
+                    const float wr_ = p_wr[k], wi_ = p_wi[k];
+                    v = make_float2(wr_ * v.x - wi_ * v.y, wr_ * v.y + wi_ * v.x);
                 }
-            }
+                float2 R = make_float2(cos_turns((float)t1), sin_turns((float)t1));
+                const int lim = ab == 0 ? a.ny - i0 : a.nx - j0;   // rows / columns inside the image
+                T (*P)[MT][KS] = ab == 0 ? As : Bs;
 #pragma unroll
-            for (int q = 0; q < NP; ++q) {
-                *reinterpret_cast<V*>(&As[q][gr][kb0]) = va[q];
-                *reinterpret_cast<V*>(&Bs[q][gr][kb0]) = vb[q];
+                for (int m = 0; m < M; ++m) {
+                    const int row = gph + G * m;
+                    const bool ok = row < lim;
+                    const float re = ok ? v.x : 0.f, im = ok ? v.y : 0.f;
+                    const T rh = Mfma<T>::cvt(re), ih = Mfma<T>::cvt(im);
+                    P[0][row][k] = rh;
+                    P[1][row][k] = ih;
+                    if constexpr (SPLIT) {
+                        P[2][row][k] = Mfma<T>::cvt(re - Mfma<T>::back(rh));
+                        P[3][row][k] = Mfma<T>::cvt(im - Mfma<T>::back(ih));
+                    }
+                    v = make_float2(fmaf(v.x, R.x, -v.y * R.y), fmaf(v.x, R.y, v.y * R.x));
+                    R = make_float2(fmaf(R.x, E.x, -R.y * E.y), fmaf(R.x, E.y, R.y * E.x));
+                }
             }
         }
         __syncthreads();
