@@ -66,6 +66,9 @@ struct DmArgs {
                                    // rows contiguous (I_l(ky, x) at group x / GROUP)
     unsigned char* flags;          // [L][ngroups]
     unsigned char* any;            // [L]
+    unsigned* fbits;               // [L][fwpl]: the flags as bits (group g: word g/32, bit g%32)
+    int fwpl;                      // words per layer of fbits
+    int fbits_lds;                 // dm_rows_h stages fbits in LDS and loads present groups only
     const double* lc;              // per layer: cx, cy, K (3 doubles)
     float ux, uy;                  // lambda^2 / Sx^2, lambda^2 / Sy^2
     int exact;                     // FP64 transfer path
@@ -194,6 +197,7 @@ __global__ __launch_bounds__(2 * (LY / 16)) void dm_cols(DmArgs a, int layers_pe
         if (threadIdx.x == 0) {
             a.flags[(size_t)(l - 1) * a.ngroups + g] = 1;
             a.any[l - 1] = 1;
+            atomicOr(&a.fbits[(size_t)(l - 1) * a.fwpl + (g >> 5)], 1u << (g & 31));
         }
         __syncthreads();
     }
@@ -390,9 +394,14 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
     float2* Sb = Sa + LX;
     float2* tws = Sb + LX;
     unsigned* amask = reinterpret_cast<unsigned*>(tws + NTW);
+    // presence bits of every (layer, column group), when they fit in LDS:
+    // then only present groups' rows of I are loaded
+    unsigned* fb = amask + MAX_LAYERS / 32;
     const int t = threadIdx.x;
     for (int i = t; i < NTW; i += TPL) tws[i] = tw8[i];
     for (int i = t; i < MAX_LAYERS / 32; i += TPL) amask[i] = 0u;
+    if (a.fbits_lds)
+        for (int i = t; i < a.L * a.fwpl; i += TPL) fb[i] = a.fbits[i];
     __syncthreads();
     for (int i = t; i < a.L; i += TPL)
         if (a.any[i]) atomicOr(&amask[i >> 5], 1u << (i & 31));
@@ -443,6 +452,30 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
     unsigned nf[8];
     auto fetch = [&](int l) {
         const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * GROUP;
+        if (a.fbits_lds) {
+            // flags from LDS first; then range-checked buffer loads whose
+            // offset is out of range for absent groups (no memory access, a
+            // zero result, no branch): the rows of absent groups were never
+            // written and are not read
+            const unsigned* fl = fb + (size_t)(l - 1) * a.fwpl;
+            const auto rs = oph_buf_rsrc(a.I + (size_t)(l - 1) * lstride, (unsigned)(lstride * sizeof(float2)));
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                const int x = t + m * TPL - a.ox;
+                const int gx = x >> 2;  // x / GROUP for x >= 0
+                const bool on = (x >= 0 && x < a.w) && ((fl[gx >> 5] >> (gx & 31)) & 1u);
+                const unsigned o = (unsigned)(((size_t)gx * gstride + (x & 3) + (size_t)kya * GROUP) * sizeof(float2));
+                nf[m] = on ? 1u : 0u;
+                if constexpr (PAIR0) {
+                    const float2 lo = oph_buf_load_f2(rs, on ? o : OPH_BUF_OOB);
+                    const float2 hi = oph_buf_load_f2(rs, on ? o + (unsigned)(nyq * sizeof(float2)) : OPH_BUF_OOB);
+                    nv[m] = make_float2(lo.x, hi.x);
+                } else {
+                    nv[m] = oph_buf_load_f2(rs, on ? o : OPH_BUF_OOB);
+                }
+            }
+            return;
+        }
         const unsigned char* fl = a.flags + (size_t)(l - 1) * a.ngroups;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
@@ -627,11 +660,15 @@ int launch_dm_rows(oph_ctx* ctx, const DmArgs& a) {
             static bool attr = false;
             if (!attr) {
                 hipFuncSetAttribute(reinterpret_cast<const void*>(&dm_rows_h<LX>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)rows_h_lds<LX>());
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
                 hipGetLastError();
                 attr = true;
             }
-            OPH_LAUNCH(ctx, "dm_rows", dm_rows_h<LX>, dim3(a.Ny / 2), dim3(LX / 8), rows_h_lds<LX>(), a, tw8);
+            DmArgs b = a;
+            const size_t fbytes = (size_t)a.L * a.fwpl * sizeof(unsigned);
+            b.fbits_lds = rows_h_lds<LX>() + fbytes <= 160 * 1024 ? 1 : 0;
+            const size_t shm = rows_h_lds<LX>() + (b.fbits_lds ? fbytes : 0);
+            OPH_LAUNCH(ctx, "dm_rows", dm_rows_h<LX>, dim3(a.Ny / 2), dim3(LX / 8), shm, b, tw8);
             return OPH_OK;
         }
     }
@@ -726,7 +763,9 @@ extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, cons
         OPH_HIP_CHECK(ctx, hipMemcpyAsync(dimg, intensity, npx * ncolor, hipMemcpyHostToDevice, ctx->stream));
         OPH_HIP_CHECK(ctx, hipMemcpyAsync(ddep, depth, npx, hipMemcpyHostToDevice, ctx->stream));
     }
-    const size_t aux_bytes = npx * 2 + (size_t)L * ngroups + L + 64 + hc.size() * sizeof(double);
+    const int fwpl = (ngroups + 31) / 32;
+    const size_t aux_bytes = npx * 2 + (size_t)L * ngroups + L + 4 + (size_t)L * fwpl * 4 + 64 +
+                             hc.size() * sizeof(double);
     unsigned char* aux = static_cast<unsigned char*>(oph_scratch(ctx, SLOT_AUX0, aux_bytes + 256));
     float2* I = static_cast<float2*>(oph_scratch(ctx, SLOT_WORK2, (size_t)L * lstride * sizeof(float2)));
     float2* T = static_cast<float2*>(oph_scratch(ctx, SLOT_WORK1, (size_t)Ny * w * sizeof(float2)));
@@ -735,6 +774,9 @@ extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, cons
     unsigned short* didx = reinterpret_cast<unsigned short*>(aux + hc.size() * sizeof(double));
     unsigned char* dflags = reinterpret_cast<unsigned char*>(didx + npx);
     unsigned char* dany = dflags + (size_t)L * ngroups;
+    // the flag bits follow, at the next 4-byte boundary (word atomics)
+    unsigned* dfbits = reinterpret_cast<unsigned*>((reinterpret_cast<uintptr_t>(dany + L) + 3) & ~(uintptr_t)3);
+    const size_t nfl_bytes = reinterpret_cast<unsigned char*>(dfbits + (size_t)L * fwpl) - dflags;
     OPH_HIP_CHECK(ctx, hipMemcpyAsync(dlc, hc.data(), hc.size() * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
 
     // outputs
@@ -784,6 +826,8 @@ extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, cons
         a.I = I;
         a.flags = dflags;
         a.any = dany;
+        a.fbits = dfbits;
+        a.fwpl = fwpl;
         a.lc = dlc + (size_t)3 * L * c;
         a.ux = (float)(lam * lam / (Sx * Sx));
         a.uy = (float)(lam * lam / (Sy * Sy));
@@ -804,7 +848,7 @@ extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, cons
         a.fringe32 = fmt == OPH_DM_OUT_FRINGE32 ? reinterpret_cast<float*>(ob) : nullptr;
         a.field = fmt == OPH_DM_OUT_FIELD ? reinterpret_cast<float2*>(ob) : nullptr;
 
-        const int nfl = L * ngroups + L;
+        const int nfl = (int)nfl_bytes;   // flags, any and the flag bits
         OPH_LAUNCH(ctx, "dm_zero", dm_zero, dim3((nfl + 255) / 256), dim3(256), 0, dflags, nfl);
         const int nl = hi - lo + 1;
         int chunks = (int)((4LL * ctx->cu_count + ngroups - 1) / ngroups);
