@@ -155,12 +155,7 @@ __global__ __launch_bounds__(512) void asm_rows_fwd(const float2* __restrict__ i
 template <int L>
 __host__ __device__ constexpr bool asm_cols_fused() { return fft::regs_fusable<L>() && fft::n16(L) == 3; }
 
-// MID: the data rows are exactly the middle half, [L/4, 3L/4) (twofold
-// padding; 4096-point columns, two per workgroup): the first forward
-// butterfly's inputs t + m L/16 with m outside [4, 12) are known zeros and
-// only the last inverse butterfly's outputs with m in [4, 12) are kept, so
-// neither is read or written in LDS.
-template <int L, int B, bool EXACT, bool MID = false>
+template <int L, int B, bool EXACT>
 __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, int oy,
                                                  const float4* __restrict__ tabx,
                                                  const float4* __restrict__ taby,
@@ -181,24 +176,7 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
     float2* tw1 = lds;  // first, so every read of it is one address plus an immediate
     float2* lines = lds + (FUSED ? 256 : 0);
     if constexpr (!FUSED) T.load(tw, threadIdx.x % TPL);
-    static_assert(!MID || (FUSED && B == 2), "middle-half rows: 4096-point columns, two per workgroup");
-    if constexpr (MID) {
-        // lane rows threadIdx.x + j nthr, data for j in [J0, J1): no bounds tests
-        constexpr int J0 = L / 4 / (TPL * B), J1 = 3 * L / 4 / (TPL * B);
-        const auto rs = oph_buf_rsrc(work, (unsigned)((size_t)(3 * L / 4) * npx * sizeof(float2)));
-        float4 ld[J1 - J0];
-#pragma unroll
-        for (int j = J0; j < J1; ++j) {
-            const int i = threadIdx.x + j * nthr;
-            ld[j - J0] = oph_buf_load_f4(rs, (unsigned)(((size_t)i * npx + cbase) * sizeof(float2)));
-        }
-#pragma unroll
-        for (int j = J0; j < J1; ++j) {
-            const int i = threadIdx.x + j * nthr;
-            lines[fft::pad16(i)] = make_float2(ld[j - J0].x, ld[j - J0].y);
-            lines[PL + fft::pad16(i)] = make_float2(ld[j - J0].z, ld[j - J0].w);
-        }
-    } else if constexpr (B % 2 == 0 && (L <= 1024 || FUSED)) {
+    if constexpr (B % 2 == 0 && (L <= 1024 || FUSED)) {
         // two neighbouring columns per lane (16-byte accesses); npx is a
         // power of two >= B, so every column of the group exists.  All 8
         // loads of a lane are issued before any is used: range-checked
@@ -255,17 +233,7 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
         // in registers, which are the inverse's first butterfly's inputs:
         // the transfer is applied in between, without a pass through LDS
         float2 w[1][16];
-        if constexpr (MID) {
-            float2 u[16];
-#pragma unroll
-            for (int m = 0; m < 16; ++m)
-                u[m] = (m >= 4 && m < 12) ? line[fft::lane_pad<L>(t, m)] : make_float2(0.f, 0.f);
-            __syncthreads();  // every lane has read its inputs
-            fft::stage0_from_regs<L, -1>(u, line, t);
-            __syncthreads();
-        } else {
-            fft::stage<L, 16, -1, TPL>(line, w, 1, t);
-        }
+        fft::stage<L, 16, -1, TPL>(line, w, 1, t);
 #pragma unroll
         for (int r = 1; r < 16; ++r) w[0][r] = tw1[(r - 1) * 16 + (t & 15)];
         fft::stage<L, 16, -1, TPL>(line, w, 16, t);
@@ -285,16 +253,7 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
         fft::stage<L, 16, +1, TPL>(line, w, 16, t);
 #pragma unroll
         for (int r = 1; r < 16; ++r) w[0][r] = oph_buf_load_f2(twl, t * sizeof(float2), (r - 1) * TPL * sizeof(float2));
-        if constexpr (MID) {
-            // the last butterfly reads and writes only the lane's own
-            // elements t + m L/16, so no barrier separates the two
-            fft::last_stage_to_regs_w<L, +1>(line, w[0], t, v);
-#pragma unroll
-            for (int m = 4; m < 12; ++m) line[fft::lane_pad<L>(t, m)] = v[m];
-            __syncthreads();
-        } else {
-            fft::stage<L, 16, +1, TPL>(line, w, 16 * 16, t);
-        }
+        fft::stage<L, 16, +1, TPL>(line, w, 16 * 16, t);
     } else {
         fft::line<L, -1>(line, T, t);
         // (a few rows at a time: every stage's twiddles stay in registers)
@@ -324,6 +283,117 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
             const int cc = cbase + bb;
             if (cc < npx) work[(size_t)i * npx + cc] = lines[bb * PL + fft::pad16(i)];
         }
+    }
+}
+
+// 4096-point columns whose data rows are exactly the middle half,
+// [1024, 3072) (twofold padding), B columns per step, persistent.  The first
+// forward butterfly's inputs t + m 256 with m outside [4, 12) are known zeros
+// and only the last inverse butterfly's outputs with m in [4, 12) are kept,
+// so neither is read or written in LDS.  Each workgroup walks groups of B
+// columns: the next group's rows are fetched into registers while the
+// current group is transformed, so the loads' HBM latency overlaps the LDS
+// work.  Groups are split by XCD (the hardware places workgroup b on XCD
+// b mod 8): XCD x owns groups [x G/8, (x + 1) G/8) and its workgroups take
+// consecutive groups at each step, so the row pieces of one 128-byte line
+// are read and written through one L2 at about the same time.  npx is a
+// multiple of 8 B and the grid a multiple of 8.
+constexpr int MID_L = 4096, MID_B = 4;
+
+template <int B, bool EXACT>
+__global__ __launch_bounds__(B * (MID_L / 16), 4) void asm_cols_mid(
+    float2* work, int npx, const float4* __restrict__ tabx, const float4* __restrict__ taby,
+    const float2* __restrict__ taby2, AsmConst k, const float2* __restrict__ tw) {
+    constexpr int L = MID_L, TPL = L / 16, PL = fft::padded_len(L), NTHR = B * TPL, H = B / 2;
+    // float4 row pieces (two columns each) per lane: H per row, rows [L/4, 3L/4)
+    constexpr int PER = (L / 2) * H / NTHR;
+    extern __shared__ float2 lds[];
+    float2* tw1 = lds;  // middle stage's 240 twiddles (k = t mod 16)
+    float2* lines = lds + 256;
+    const int per = (npx / B) / 8, steps = gridDim.x / 8;
+    const int xcd = blockIdx.x % 8;
+    const int gend = (xcd + 1) * per;
+    int g = xcd * per + blockIdx.x / 8;
+    const auto rs = oph_buf_rsrc(work, (unsigned)((size_t)(3 * L / 4) * npx * sizeof(float2)));
+    float4 ld[PER];
+    auto fetch = [&](int grp) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int idx = threadIdx.x + j * NTHR;
+            const int pr = idx % H, i = L / 4 + idx / H;
+            ld[j] = oph_buf_load_f4(rs, (unsigned)(((size_t)i * npx + B * grp + 2 * pr) * sizeof(float2)));
+        }
+    };
+    // the FP64 transfer (EXACT) needs the registers the prefetch would hold:
+    // it fetches each group at the top of its step instead
+    constexpr bool PREFETCH = !EXACT;
+    if (PREFETCH && g < gend) fetch(g);
+    for (int e = threadIdx.x; e < 15 * 16; e += NTHR) tw1[e] = tw[e];
+    const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
+    float2* line = lines + b * PL;
+    const auto twl = oph_buf_rsrc(tw + fft::last_stage_tw_offset<L>(), 15 * TPL * sizeof(float2));
+    for (; g < gend; g += steps) {
+        if constexpr (!PREFETCH) fetch(g);
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int idx = threadIdx.x + j * NTHR;
+            const int pr = idx % H, i = L / 4 + idx / H;
+            lines[(2 * pr) * PL + fft::pad16(i)] = make_float2(ld[j].x, ld[j].y);
+            lines[(2 * pr + 1) * PL + fft::pad16(i)] = make_float2(ld[j].z, ld[j].w);
+        }
+        __syncthreads();
+        // a zero the compiler cannot see through, added to the lane index of
+        // the loop-invariant table loads: hoisted out of the loop they would
+        // hold ~60 registers across it and spill
+        int zero;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+        const int tz = t + zero;
+        const int c = B * g + b;
+        const float4 ax = tabx[c];
+        // forward: stage 0 on the middle rows only (the rest are zeros)
+        float2 u[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) u[m] = (m >= 4 && m < 12) ? line[fft::lane_pad<L>(t, m)] : make_float2(0.f, 0.f);
+        __syncthreads();  // every lane has read its inputs
+        fft::stage0_from_regs<L, -1>(u, line, t);
+        __syncthreads();
+        float2 w[1][16];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) w[0][r] = tw1[(r - 1) * 16 + (t & 15)];
+        fft::stage<L, 16, -1, TPL>(line, w, 16, t);
+        float2 v[16];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) w[0][r] = oph_buf_load_f2(twl, tz * sizeof(float2), (r - 1) * TPL * sizeof(float2));
+        // the forward transform's last butterfly leaves elements t + m 256 in
+        // registers, the inverse's first butterfly's inputs: the transfer is
+        // applied in between, without a pass through LDS
+        fft::last_stage_to_regs_w<L, -1>(line, w[0], t, v);
+        if (k.band) apply_transfer<L, EXACT, true>(v, ax, taby2, taby, c, tz, 0, k);
+        else apply_transfer<L, EXACT, false>(v, ax, taby2, taby, c, tz, 0, k);
+        __syncthreads();  // every lane has read the line
+        fft::stage0_from_regs<L, +1>(v, line, t);
+        // the next group's rows: in flight through the inverse transform
+        if (PREFETCH && g + steps < gend) fetch(g + steps);
+        __syncthreads();
+#pragma unroll
+        for (int r = 1; r < 16; ++r) w[0][r] = tw1[(r - 1) * 16 + (t & 15)];
+        fft::stage<L, 16, +1, TPL>(line, w, 16, t);
+#pragma unroll
+        for (int r = 1; r < 16; ++r) w[0][r] = oph_buf_load_f2(twl, tz * sizeof(float2), (r - 1) * TPL * sizeof(float2));
+        // the last butterfly reads and writes only the lane's own elements
+        // t + m 256, so no barrier separates the two
+        fft::last_stage_to_regs_w<L, +1>(line, w[0], t, v);
+#pragma unroll
+        for (int m = 4; m < 12; ++m) line[fft::lane_pad<L>(t, m)] = v[m];
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int idx = threadIdx.x + j * NTHR;
+            const int pr = idx % H, i = L / 4 + idx / H;
+            const float2 a0 = lines[(2 * pr) * PL + fft::pad16(i)], a1 = lines[(2 * pr + 1) * PL + fft::pad16(i)];
+            *reinterpret_cast<float4*>(work + (size_t)i * npx + B * g + 2 * pr) = make_float4(a0.x, a0.y, a1.x, a1.y);
+        }
+        __syncthreads();  // the lines are read out before the next group is stored
     }
 }
 
@@ -624,20 +694,45 @@ int launch_rows_inv(oph_ctx* ctx, const float2* work, int nx, int ny, int ox, in
     return OPH_OK;
 }
 
-template <int LY, int B, bool EXACT, bool MID = false>
+template <int B, bool EXACT>
+int launch_cols_mid(oph_ctx* ctx, float2* work, int npx, const float4* tx, const float4* ty, const AsmConst& k,
+                    const float2* tw) {
+    constexpr int NTHR = B * (MID_L / 16);
+    const size_t shm = ((size_t)B * fft::padded_len(MID_L) + 256) * sizeof(float2);
+    static const int occ = [&] {
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&asm_cols_mid<B, EXACT>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipGetLastError();
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&asm_cols_mid<B, EXACT>),
+                                                         NTHR, shm) != hipSuccess || n < 1) {
+            hipGetLastError();
+            n = 1;
+        }
+        return n;
+    }();
+    // one resident wave of workgroups, a multiple of 8, at most one per group
+    const int grid = std::max(8, std::min(npx / B, occ * std::max(ctx->cu_count, 1)) / 8 * 8);
+    const float2* ty2 = reinterpret_cast<const float2*>(ty + MID_L);
+    OPH_LAUNCH(ctx, "asm_cols", (asm_cols_mid<B, EXACT>), dim3(grid), dim3(NTHR), shm, work, npx, tx, ty, ty2, k,
+               tw);
+    return OPH_OK;
+}
+
+template <int LY, int B, bool EXACT>
 int launch_cols_b(oph_ctx* ctx, float2* work, int npx, int ny, int oy, const float4* tx, const float4* ty,
                   const AsmConst& k, const float2* tw) {
     const size_t shm = ((size_t)B * fft::padded_len(LY) + (asm_cols_fused<LY>() ? 256 : 0)) * sizeof(float2);
     static bool attr = false;
     if (!attr) {
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&asm_cols<LY, B, EXACT, MID>),
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&asm_cols<LY, B, EXACT>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         hipGetLastError();
         attr = true;
     }
     // the compact y table follows the two float4 tables (ty has LY entries)
     const float2* ty2 = reinterpret_cast<const float2*>(ty + LY);
-    OPH_LAUNCH(ctx, "asm_cols", (asm_cols<LY, B, EXACT, MID>), dim3((npx + B - 1) / B), dim3(B * (LY / 16)), shm,
+    OPH_LAUNCH(ctx, "asm_cols", (asm_cols<LY, B, EXACT>), dim3((npx + B - 1) / B), dim3(B * (LY / 16)), shm,
                work, npx, ny, oy, tx, ty, ty2, k, tw);
     return OPH_OK;
 }
@@ -653,9 +748,9 @@ int launch_cols_ex(oph_ctx* ctx, float2* work, int npx, int ny, int oy, const fl
         case 1: return launch_cols_b<LY, 1, EXACT>(ctx, work, npx, ny, oy, tx, ty, k, tw);
         case 2:
             if constexpr (BMAX >= 2) {
-                if constexpr (asm_cols_fused<LY>()) {
-                    if (oy == LY / 4 && ny == LY / 2)
-                        return launch_cols_b<LY, 2, EXACT, true>(ctx, work, npx, ny, oy, tx, ty, k, tw);
+                if constexpr (LY == MID_L) {
+                    if (oy == LY / 4 && ny == LY / 2 && npx % (8 * MID_B) == 0)
+                        return launch_cols_mid<MID_B, EXACT>(ctx, work, npx, tx, ty, k, tw);
                 }
                 return launch_cols_b<LY, 2, EXACT>(ctx, work, npx, ny, oy, tx, ty, k, tw);
             }
