@@ -50,6 +50,7 @@
 
 #include "fft_engine.h"
 #include "fft_kernels.h"
+#include "fft_mixed.h"
 
 namespace {
 
@@ -629,6 +630,257 @@ __global__ __launch_bounds__(B * (LY / 16)) void dm_final(DmArgs a, const float2
     }
 }
 
+// ---- general lengths -------------------------------------------------------------
+// The same three passes for grids whose sides are 2^a 3^b 5^c but not powers
+// of two (the shipped 11520 x 2160 configuration, HD scenes transformed
+// unpadded): mixed-radix line transforms (fft_mixed.h) by NT =
+// fftm::threads(N) threads per line, thread t owning elements t + m NT
+// (m < fftm::ept(N)), index arithmetic mod N (N even: the fftshifts are
+// still the (-1)^k factors above).  No chunked recurrence: every layer's
+// transfer is evaluated (dm_transfer).
+
+#define OPH_DM_MIXED_LENGTHS(X) X(540) X(720) X(1080) X(1920) X(2160) X(3840) X(11520)
+
+bool dm_mixed_length(int n) {
+    switch (n) {
+#define OPH_DMM_CASE(v) case v:
+        OPH_DM_MIXED_LENGTHS(OPH_DMM_CASE)
+#undef OPH_DMM_CASE
+        return true;
+        default: return false;
+    }
+}
+
+template <int LY>
+__global__ __launch_bounds__(2 * fftm::threads(LY)) void dm_cols_m(DmArgs a, int layers_per_wg,
+                                                                 const float2* __restrict__ twy) {
+    constexpr int NT = fftm::threads(LY);
+    constexpr int EPT = fftm::ept(LY);
+    constexpr int PL = fftm::padded_len(LY);
+    extern __shared__ unsigned char smem[];
+    float2* lines = reinterpret_cast<float2*>(smem);                 // 2 x PL
+    unsigned* pix = reinterpret_cast<unsigned*>(lines + 2 * PL);     // h x GROUP: idx << 8 | img
+    unsigned* mask = pix + (size_t)a.h * GROUP;                      // layer bitmask
+    const int lin = xcd_remap(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int g = lin % gridDim.x;
+    const int x0 = g * GROUP;
+    const int l_begin = a.lo + lin / gridDim.x * layers_per_wg;
+    const int l_end = min(a.hi, l_begin + layers_per_wg - 1);
+    const int nwords = (a.L + 32) / 32;
+    const int p = threadIdx.x / NT, t = threadIdx.x % NT;
+    float2* line = lines + p * PL;
+    for (int i = threadIdx.x; i < nwords; i += 2 * NT) mask[i] = 0u;
+    __syncthreads();
+    for (int e = threadIdx.x; e < a.h * GROUP; e += 2 * NT) {
+        const int c = e % GROUP, r = e / GROUP;
+        const int x = x0 + c;
+        unsigned v = 0u;
+        if (x < a.w) {
+            const size_t o = (size_t)r * a.w + x;
+            const unsigned im = a.img[o];
+            const unsigned id = a.idx[o];
+            if (im > 0u && id >= (unsigned)l_begin && id <= (unsigned)l_end) {
+                v = (id << 8) | im;
+                atomicOr(&mask[id >> 5], 1u << (id & 31));
+            }
+        }
+        pix[e] = v;
+    }
+    __syncthreads();
+    const size_t gstride = (size_t)(a.Ny / 2 + 1) * GROUP;
+    const size_t lstride = gstride * a.ngroups;
+    for (int l = l_begin; l <= l_end; ++l) {
+        if (!((mask[l >> 5] >> (l & 31)) & 1u)) continue;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int i = t + m * NT;
+            if (EPT * NT == LY || i < LY) {
+                const int r = i - a.oy;
+                float2 v = make_float2(0.f, 0.f);
+                if (r >= 0 && r < a.h) {
+                    const unsigned wa = pix[r * GROUP + 2 * p], wb = pix[r * GROUP + 2 * p + 1];
+                    if ((int)(wa >> 8) == l) v.x = (float)(wa & 255u) * (1.f / 255.f);
+                    if ((int)(wb >> 8) == l) v.y = (float)(wb & 255u) * (1.f / 255.f);
+                }
+                line[fft::pad16(i)] = v;
+            }
+        }
+        __syncthreads();
+        fftm::line<LY, NT, -1>(line, twy, t);
+        float2* dst = a.I + (size_t)(l - 1) * lstride + (size_t)g * gstride;
+        for (int e = threadIdx.x; e < (LY / 2 + 1) * GROUP; e += 2 * NT) {
+            const int c = e % GROUP, ky = e / GROUP;
+            const int x = x0 + c;
+            if (x >= a.w) continue;
+            const float2* ln = lines + (c >> 1) * PL;
+            const float2 Z = ln[fft::pad16(ky)];
+            const float2 Zm = ln[fft::pad16(ky == 0 ? 0 : LY - ky)];
+            float2 v;
+            if ((c & 1) == 0)
+                v = make_float2(0.5f * (Z.x + Zm.x), 0.5f * (Z.y - Zm.y));
+            else
+                v = make_float2(0.5f * (Z.y + Zm.y), -0.5f * (Z.x - Zm.x));
+            dst[e] = v;
+        }
+        if (threadIdx.x == 0) {
+            a.flags[(size_t)(l - 1) * a.ngroups + g] = 1;
+            a.any[l - 1] = 1;
+            atomicOr(&a.fbits[(size_t)(l - 1) * a.fwpl + (g >> 5)], 1u << (g & 31));
+        }
+        __syncthreads();
+    }
+}
+
+template <int LX>
+__device__ __forceinline__ void finish_row_m(const DmArgs& a, const float2 (&acc)[fftm::ept(LX)], int ky,
+                                             float2* line, const float2* __restrict__ tw, int t) {
+    constexpr int NT = fftm::threads(LX), EPT = fftm::ept(LX);
+    const int yc = (ky + a.Ny / 2) % a.Ny;
+    if (a.spectrum) {
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int kx = t + m * NT;
+            if (EPT * NT == LX || kx < LX) {
+                const int xc = (kx + LX / 2) % LX;
+                const float sg = ((kx + ky) & 1) ? -1.f : 1.f;
+                a.spectrum[(size_t)yc * LX + xc] = make_float2(sg * acc[m].x, sg * acc[m].y);
+            }
+        }
+    }
+    const bool kept = !a.encode || (yc >= a.cy1 && yc <= a.cy2);
+    if (!kept) return;  // uniform over the workgroup
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        const int kx = t + m * NT;
+        if (EPT * NT == LX || kx < LX) line[fft::pad16(kx)] = acc[m];
+    }
+    __syncthreads();
+    fftm::line<LX, NT, +1>(line, tw, t);
+    float2* dst = a.T + (size_t)ky * a.w;
+    for (int x = t; x < a.w; x += NT) dst[x] = line[fft::pad16(x + a.ox)];
+    __syncthreads();
+}
+
+template <int LX, bool EXACT>
+__global__ __launch_bounds__(fftm::threads(LX)) void dm_rows_m(DmArgs a, const float2* __restrict__ twx) {
+    constexpr int NT = fftm::threads(LX), EPT = fftm::ept(LX);
+    extern __shared__ float2 line[];  // fftm::padded_len(LX)
+    const int t = threadIdx.x;
+    const int kya = xcd_remap(blockIdx.x, gridDim.x);  // 0 .. Ny/2 - 1
+    const bool pair0 = kya == 0;              // real rows 0 and Ny/2 packed as re/im
+    const int kyb = pair0 ? a.Ny / 2 : a.Ny - kya;
+    const int sya = sgnidx(kya, a.Ny) + 1;
+    const int syb = sgnidx(kyb, a.Ny) + 1;
+    const size_t nyq = (size_t)(a.Ny / 2) * GROUP;
+    const size_t gstride = (size_t)(a.Ny / 2 + 1) * GROUP;
+    // (the per-element x factors are recomputed per layer: the two rows'
+    // accumulators already take most of the registers of a 768-thread line)
+    float2 acca[EPT], accb[EPT];
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) acca[m] = accb[m] = make_float2(0.f, 0.f);
+    const size_t lstride = gstride * a.ngroups;
+    for (int l = a.lo; l <= a.hi; ++l) {
+        if (!a.any[l - 1]) continue;
+        // the thread index through a zero the compiler cannot see through:
+        // the transforms' LDS addresses are then formed per layer instead of
+        // being hoisted out of the loop into ~100 registers
+        int zero;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+        const int tl = t + zero;
+        const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * GROUP;
+        const unsigned char* fl = a.flags + (size_t)(l - 1) * a.ngroups;
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int kx = tl + m * NT;
+            if (EPT * NT == LX || kx < LX) {
+                const int x = kx - a.ox;
+                float2 v = make_float2(0.f, 0.f);
+                if (x >= 0 && x < a.w && fl[x / GROUP]) {
+                    const size_t o = (size_t)(x / GROUP) * gstride + x % GROUP;
+                    v = src[o];
+                    if (pair0) v.y = src[nyq + o].x;
+                }
+                line[fft::pad16(kx)] = v;
+            }
+        }
+        __syncthreads();
+        fftm::line<LX, NT, -1>(line, twx, tl);
+        const double cx = a.lc[3 * (l - 1)], cy = a.lc[3 * (l - 1) + 1], K = a.lc[3 * (l - 1) + 2];
+        const float Kf = (float)K;
+        double tya = cy * (double)sya * sya;
+        tya -= rint(tya);
+        double tyb = cy * (double)syb * syb;
+        tyb -= rint(tyb);
+#pragma unroll
+        for (int m = 0; m < EPT; ++m) {
+            const int kx = tl + m * NT;
+            if (EPT * NT == LX || kx < LX) {
+                const int sx = sgnidx(kx, LX);
+                const double sx2 = (double)sx * sx;
+                const float usx = a.ux * (float)(sx * sx);
+                double txd = cx * sx2;
+                txd -= rint(txd);
+                const float tx = (float)txd;
+                float2 gv = line[fft::pad16(kx)];
+                const float2 gm = line[fft::pad16(kx == 0 ? 0 : LX - kx)];
+                float2 gmc = make_float2(gm.x, -gm.y);
+                if (pair0) split_real_pair(gv, gmc);
+                const float2 ha = dm_transfer<EXACT>(a, tx, tya, usx, sya, K, Kf, sx2);
+                acca[m] = fft::cadd(acca[m], fft::cmul(ha, gv));
+                const float2 hb = dm_transfer<EXACT>(a, tx, tyb, usx, syb, K, Kf, sx2);
+                accb[m] = fft::cadd(accb[m], fft::cmul(hb, gmc));
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < EPT; ++m) {
+        acca[m] = fft::cmul(acca[m], a.rho);
+        accb[m] = fft::cmul(accb[m], a.rho);
+    }
+    finish_row_m<LX>(a, acca, kya, line, twx, t);
+    finish_row_m<LX>(a, accb, kyb, line, twx, t);
+}
+
+template <int LY, int B>
+__global__ __launch_bounds__(B * fftm::threads(LY)) void dm_final_m(DmArgs a, const float2* __restrict__ twy) {
+    constexpr int NT = fftm::threads(LY);
+    constexpr int PL = fftm::padded_len(LY);
+    extern __shared__ float2 lds[];
+    const int nthr = B * NT;
+    const int g = xcd_remap(blockIdx.x, gridDim.x);
+    const int c0 = g * B;
+    const int b = threadIdx.x / NT, t = threadIdx.x % NT;
+    for (int e = threadIdx.x; e < LY * B; e += nthr) {
+        const int bb = e % B, ky = e / B;
+        const int x = c0 + bb;
+        const int yc = (ky + a.Ny / 2) % a.Ny;
+        const bool kept = !a.encode || (yc >= a.cy1 && yc <= a.cy2);
+        lds[bb * PL + fft::pad16(ky)] = (x < a.w && kept) ? a.T[(size_t)ky * a.w + x] : make_float2(0.f, 0.f);
+    }
+    __syncthreads();
+    fftm::line<LY, NT, +1>(lds + b * PL, twy, t);
+    const float sc = (float)a.scale;
+    for (int e = threadIdx.x; e < a.h * B; e += nthr) {
+        const int bb = e % B, r = e / B;
+        const int x = c0 + bb;
+        if (x >= a.w) continue;
+        const float2 v0 = lds[bb * PL + fft::pad16(a.oy + r)];
+        const float2 v = make_float2(v0.x * sc, v0.y * sc);
+        const size_t o = (size_t)r * a.w + x;
+        if (a.encode) {
+            double tt = a.h / 8.0 - (r + 1) / 4.0;
+            tt -= rint(tt);
+            const float ph = (float)tt;
+            const float re = v.x * cos_turns(ph) - v.y * sin_turns(ph);
+            if (a.fringe) a.fringe[o] = re;
+            if (a.fringe32) a.fringe32[o] = re;
+        } else {
+            a.field[o] = v;
+        }
+    }
+}
+
 // ---- launchers -------------------------------------------------------------------
 
 template <int LY>
@@ -696,6 +948,77 @@ int launch_dm_final(oph_ctx* ctx, const DmArgs& a) {
     return OPH_OK;
 }
 
+template <int LY>
+int launch_dm_cols_m(oph_ctx* ctx, const DmArgs& a, int chunks, int lpw) {
+    const float2* tw = oph_twiddles_mixed(ctx, LY);
+    if (!tw) return OPH_ENOMEM;
+    if constexpr (2 * (size_t)fftm::padded_len(LY) * sizeof(float2) > 150 * 1024) {
+        (void)chunks;
+        (void)lpw;
+        return oph_fail(ctx, OPH_ENOTSUP, "grid height %d too large for the column pass", LY);
+    } else {
+        const size_t shm = 2 * (size_t)fftm::padded_len(LY) * sizeof(float2) + (size_t)a.h * GROUP * 4 +
+                           (size_t)((a.L + 32) / 32) * 4;
+        if (shm > 160 * 1024)
+            return oph_fail(ctx, OPH_ENOTSUP, "scene height %d too large for the column pass", a.h);
+        static bool attr = false;
+        if (!attr) {
+            hipFuncSetAttribute(reinterpret_cast<const void*>(&dm_cols_m<LY>),
+                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+            hipGetLastError();
+            attr = true;
+        }
+        OPH_LAUNCH(ctx, "dm_cols", dm_cols_m<LY>, dim3(a.ngroups, chunks), dim3(2 * fftm::threads(LY)), shm, a,
+                   lpw, tw);
+        return OPH_OK;
+    }
+}
+
+template <int LX>
+int launch_dm_rows_m(oph_ctx* ctx, const DmArgs& a) {
+    const float2* tw = oph_twiddles_mixed(ctx, LX);
+    if (!tw) return OPH_ENOMEM;
+    const size_t shm = (size_t)fftm::padded_len(LX) * sizeof(float2);
+    static bool attr = false;
+    if (!attr) {
+        for (const void* k : {reinterpret_cast<const void*>(&dm_rows_m<LX, true>),
+                              reinterpret_cast<const void*>(&dm_rows_m<LX, false>)})
+            hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipGetLastError();
+        attr = true;
+    }
+    if (a.exact)
+        OPH_LAUNCH(ctx, "dm_rows", (dm_rows_m<LX, true>), dim3(a.Ny / 2), dim3(fftm::threads(LX)), shm, a, tw);
+    else
+        OPH_LAUNCH(ctx, "dm_rows", (dm_rows_m<LX, false>), dim3(a.Ny / 2), dim3(fftm::threads(LX)), shm, a, tw);
+    return OPH_OK;
+}
+
+// columns per dm_final_m workgroup: up to 4, at most 1024 threads and ~140 KB
+__host__ __device__ constexpr int final_m_cols(int LY) {
+    int b = 4;
+    while (b > 1 && (b * fftm::threads(LY) > 1024 || (size_t)b * fftm::padded_len(LY) * 8 > 140 * 1024)) b /= 2;
+    return b;
+}
+
+template <int LY>
+int launch_dm_final_m(oph_ctx* ctx, const DmArgs& a) {
+    const float2* tw = oph_twiddles_mixed(ctx, LY);
+    if (!tw) return OPH_ENOMEM;
+    constexpr int B = final_m_cols(LY);
+    const size_t shm = (size_t)B * fftm::padded_len(LY) * sizeof(float2);
+    static bool attr = false;
+    if (!attr) {
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&dm_final_m<LY, B>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipGetLastError();
+        attr = true;
+    }
+    OPH_LAUNCH(ctx, "dm_final", (dm_final_m<LY, B>), dim3((a.w + B - 1) / B), dim3(B * fftm::threads(LY)), shm,
+               a, tw);
+    return OPH_OK;
+}
+
 }  // namespace
 
 extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, const double* lambdas,
@@ -706,8 +1029,12 @@ extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, cons
         return oph_fail(ctx, OPH_EINVAL, "null argument");
     const int w = s->w, h = s->h, Nx = s->Nx, Ny = s->Ny, L = s->num_depth;
     if (w <= 0 || h <= 0 || w > Nx || h > Ny) return oph_fail(ctx, OPH_EINVAL, "scene %dx%d must fit the grid %dx%d", w, h, Nx, Ny);
-    if (!fftk::supported(Nx) || !fftk::supported(Ny))
-        return oph_fail(ctx, OPH_ENOTSUP, "grid %dx%d: sides must be powers of two in 16..8192", Nx, Ny);
+    // both sides powers of two, or both among the general lengths
+    const bool mixed = !(fftk::supported(Nx) && fftk::supported(Ny));
+    if (mixed && !(dm_mixed_length(Nx) && dm_mixed_length(Ny)))
+        return oph_fail(ctx, OPH_ENOTSUP,
+                        "grid %dx%d: sides must both be powers of two in 16..8192 or both among 540, 720, 1080, "
+                        "1920, 2160, 3840, 11520", Nx, Ny);
     if (L < 1 || L > MAX_LAYERS) return oph_fail(ctx, OPH_EINVAL, "num_depth must be in 1..%d", MAX_LAYERS);
     if (!(s->pitch_x > 0) || !(s->pitch_y > 0) || !(s->far_m >= s->near_m))
         return oph_fail(ctx, OPH_EINVAL, "bad pitch or depth range");
@@ -807,7 +1134,7 @@ extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, cons
     double dev = 0.0;
     for (int l = 1; l < L; ++l) dev = fmax(dev, fabs((zs[l] - zs[l - 1]) - dz));
     const double phimax = lmax * (fxm * fxm + fym * fym) / 2.0 + 0.25 * umax * umax / lmax;  // turns per metre
-    const bool horner = !exact && L > 1 && hi > lo && dev * phimax < 1e-9;
+    const bool horner = !mixed && !exact && L > 1 && hi > lo && dev * phimax < 1e-9;
 
     for (int c = 0; c < ncolor; ++c) {
         const double lam = lambdas[c];
@@ -856,6 +1183,27 @@ extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, cons
         if (chunks < 1) chunks = 1;
         const int lpw = (nl + chunks - 1) / chunks;
         chunks = (nl + lpw - 1) / lpw;
+        if (mixed) {
+            switch (Ny) {
+#define OPH_DMC(n) case n: rc = launch_dm_cols_m<n>(ctx, a, chunks, lpw); break;
+                OPH_DM_MIXED_LENGTHS(OPH_DMC)
+#undef OPH_DMC
+            }
+            if (rc) return rc;
+            switch (Nx) {
+#define OPH_DMR(n) case n: rc = launch_dm_rows_m<n>(ctx, a); break;
+                OPH_DM_MIXED_LENGTHS(OPH_DMR)
+#undef OPH_DMR
+            }
+            if (rc) return rc;
+            switch (Ny) {
+#define OPH_DMF(n) case n: rc = launch_dm_final_m<n>(ctx, a); break;
+                OPH_DM_MIXED_LENGTHS(OPH_DMF)
+#undef OPH_DMF
+            }
+            if (rc) return rc;
+            continue;
+        }
         switch (Ny) {
 #define OPH_DMC(n) case n: rc = launch_dm_cols<n>(ctx, a, chunks, lpw); break;
             OPH_FFT_LENGTHS(OPH_DMC)

@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "fft_engine.h"
+#include "fft_mixed.h"
 #include "fft_kernels.h"
 
 const float2* oph_twiddles(oph_ctx* ctx, int L, int radix) {
@@ -56,6 +57,52 @@ const float2* oph_twiddles(oph_ctx* ctx, int L, int radix) {
         return nullptr;
     }
     cache[lg] = d;
+    return static_cast<const float2*>(d);
+}
+
+const float2* oph_twiddles_mixed(oph_ctx* ctx, int N) {
+    const fftm::Plan P = fftm::plan(N);
+    if (N < 2 || P.n <= 0) {
+        oph_fail(ctx, OPH_EINVAL, "FFT length %d is not 2^a 3^b 5^c", N);
+        return nullptr;
+    }
+    int slot = -1;
+    for (int i = 0; i < 16; ++i) {
+        if (ctx->twm_len[i] == N) return static_cast<const float2*>(ctx->twm[i]);
+        if (slot < 0 && !ctx->twm[i]) slot = i;
+    }
+    if (slot < 0) {
+        oph_fail(ctx, OPH_ENOTSUP, "too many mixed-radix FFT lengths in one context");
+        return nullptr;
+    }
+    // per-stage tables in the order the plan walks the stages
+    std::vector<float2> h;
+    int Ns = 1;
+    for (int s = 0; s < P.n; ++s) {
+        const int R = P.r[s];
+        if (Ns > 1)
+            for (int r = 1; r < R; ++r)
+                for (int k = 0; k < Ns; ++k) {
+                    const double a = -2.0 * M_PI * (double)k * r / ((double)Ns * R);
+                    h.push_back(make_float2((float)cos(a), (float)sin(a)));
+                }
+        Ns *= R;
+    }
+    if (h.empty()) h.push_back(make_float2(1.f, 0.f));
+    void* d = nullptr;
+    if (hipMalloc(&d, sizeof(float2) * h.size()) != hipSuccess) {
+        hipGetLastError();
+        oph_fail(ctx, OPH_ENOMEM, "twiddle table allocation failed");
+        return nullptr;
+    }
+    if (hipMemcpy(d, h.data(), sizeof(float2) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+        hipGetLastError();
+        hipFree(d);
+        oph_fail(ctx, OPH_EDEVICE, "twiddle table upload failed");
+        return nullptr;
+    }
+    ctx->twm[slot] = d;
+    ctx->twm_len[slot] = N;
     return static_cast<const float2*>(d);
 }
 

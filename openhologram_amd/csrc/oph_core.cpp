@@ -167,6 +167,8 @@ void oph_destroy(oph_ctx* ctx) {
         if (t) hipFree(t);
     for (void* t : ctx->tw8)
         if (t) hipFree(t);
+    for (void* t : ctx->twm)
+        if (t) hipFree(t);
     if (ctx->wrp_flag) hipFree(ctx->wrp_flag);
     for (hipEvent_t e : ctx->event_pool) hipEventDestroy(e);
     if (ctx->ev_begin) hipEventDestroy(ctx->ev_begin);

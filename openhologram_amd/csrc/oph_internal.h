@@ -51,6 +51,8 @@ struct oph_ctx {
     int cu_count = 256;
     void* tw[24] = {};  // FFT twiddle tables by log2(length), radix-16 stages
     void* tw8[24] = {}; // the same for radix-8 stages
+    void* twm[16] = {}; // mixed-radix tables (lengths 2^a 3^b 5^c) ...
+    int twm_len[16] = {};  // ... and their lengths
     unsigned* wrp_flag = nullptr;          // device word set by an async WRP call whose lists overflowed
     std::vector<double> host_scratch;      // per-call host tables uploaded to the device
 };

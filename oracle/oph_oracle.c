@@ -394,10 +394,63 @@ static void fft1d(c128* a, c128* work, int n, const c128* tw) {
     if (src != a) memcpy(a, src, sizeof(c128) * n);
 }
 
+/* In-place 1-D DFT of any length n, sign and normalisation as fft1d:
+ * Stockham autosort over the prime factors of n (smallest first), each
+ * radix-p butterfly a direct p-point DFT, with tw[m] = exp(sign i 2 pi m / n)
+ * for m < n.  Stage with radix p after radices of product ns: inputs
+ * src[j + r n/p] (r < p) times exp(sign i 2 pi r k/(ns p)), k = j mod ns,
+ * then the p-point DFT, outputs to dst[(j/ns) ns p + k + q ns] (q < p) --
+ * the radix-2 loop above is the case p = 2. */
+static void fft1d_mixed(c128* a, c128* work, int n, const c128* tw) {
+    c128* src = a;
+    c128* dst = work;
+    int ns = 1, rest = n;
+    while (rest > 1) {
+        int p = 2;
+        while (rest % p) ++p;
+        const int nb = n / p;
+        const size_t step = (size_t)n / ((size_t)ns * p);
+        c128* v = (c128*)malloc(sizeof(c128) * 2 * (size_t)p);
+        c128* o = v + p;
+        for (int j = 0; j < nb; ++j) {
+            const int k = j % ns;
+            for (int r = 0; r < p; ++r) {
+                const c128 x = src[j + (size_t)r * nb];
+                v[r] = r ? cmul(x, tw[(size_t)r * k * step]) : x;
+            }
+            for (int q = 0; q < p; ++q) {
+                c128 acc = {0.0, 0.0};
+                for (int r = 0; r < p; ++r) {
+                    const c128 t = cmul(v[r], tw[((size_t)r * q % p) * nb]);
+                    acc.re += t.re;
+                    acc.im += t.im;
+                }
+                o[q] = acc;
+            }
+            const size_t base = (size_t)(j / ns) * ns * p + k;
+            for (int q = 0; q < p; ++q) dst[base + (size_t)q * ns] = o[q];
+        }
+        free(v);
+        c128* t = src; src = dst; dst = t;
+        ns *= p;
+        rest /= p;
+    }
+    if (src != a) memcpy(a, src, sizeof(c128) * n);
+}
+
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+/* exp(sign i 2 pi m / n) for m < n/2 (power of two, fft1d) or m < n (fft1d_mixed) */
 static c128* twiddles(int n, int sign) {
-    c128* tw = (c128*)malloc(sizeof(c128) * (n / 2 + 1));
-    for (int m = 0; m < n / 2; ++m) tw[m] = cexpi(sign * TWO_PI * (double)m / (double)n);
+    const int cnt = is_pow2(n) ? n / 2 + 1 : n;
+    c128* tw = (c128*)malloc(sizeof(c128) * cnt);
+    for (int m = 0; m < cnt; ++m) tw[m] = cexpi(sign * TWO_PI * (double)m / (double)n);
     return tw;
+}
+
+static void fft1d_any(c128* a, c128* work, int n, const c128* tw) {
+    if (is_pow2(n)) fft1d(a, work, n, tw);
+    else fft1d_mixed(a, work, n, tw);
 }
 
 /* 2-D DFT of a row-major ny x nx array, in place; sign as fft1d;
@@ -410,7 +463,7 @@ void orc_fft2d(c128* a, int nx, int ny, int sign, int normalize, int nthreads) {
 #pragma omp parallel for
     for (row = 0; row < ny; ++row) {
         c128* w = (c128*)malloc(sizeof(c128) * nx);
-        fft1d(a + (size_t)row * nx, w, nx, twx);
+        fft1d_any(a + (size_t)row * nx, w, nx, twx);
         free(w);
     }
     int cb;
@@ -421,7 +474,7 @@ void orc_fft2d(c128* a, int nx, int ny, int sign, int normalize, int nthreads) {
         const double s = normalize ? 1.0 / ((double)nx * ny) : 1.0;
         for (int col = cb * B; col < nx && col < cb * B + B; ++col) {
             for (int r = 0; r < ny; ++r) line[r] = a[(size_t)r * nx + col];
-            fft1d(line, line + ny, ny, twy);
+            fft1d_any(line, line + ny, ny, twy);
             for (int r = 0; r < ny; ++r) {
                 a[(size_t)r * nx + col].re = line[r].re * s;
                 a[(size_t)r * nx + col].im = line[r].im * s;

@@ -88,10 +88,10 @@ def ref_lib():
     return _ref
 
 
-def _pow2(*ns):
+def _sizes(*ns):
     for n in ns:
-        if n < 1 or (n & (n - 1)):
-            raise ValueError(f"the oracle's FFT needs power-of-two sizes, got {n}")
+        if n < 1:
+            raise ValueError(f"transform sizes must be positive, got {n}")
 
 
 def _p(a):
@@ -147,7 +147,7 @@ def scale_u8(src):
 def dm_reconstruct(fringe, ppx, ppy, lam, field_lens, eye_length, pupil, cx, cy, focus, nthreads=0):
     """Eye-model reconstruction magnitude (ny x nx doubles) of a real fringe."""
     f = np.ascontiguousarray(fringe, np.float64)
-    _pow2(*f.shape)
+    _sizes(*f.shape)
     out = np.zeros(f.shape, np.float64)
     lib().orc_dm_reconstruct(_p(f), f.shape[1], f.shape[0], ppx, ppy, lam, field_lens, eye_length, pupil,
                              cx, cy, focus, _p(out), nthreads)
@@ -188,14 +188,14 @@ def wrp(nx, ny, pitch, lam, z_wrp, x, y, z, amp=None, mode=0, nthreads=0):
 
 def fft2d(a, sign=-1, normalize=False, nthreads=0):
     a = np.array(a, np.complex128, order="C", copy=True)
-    _pow2(*a.shape)
+    _sizes(*a.shape)
     lib().orc_fft2d(_p(a), a.shape[1], a.shape[0], sign, int(normalize), nthreads)
     return a
 
 
 def asm(field, px, py, lam, z, pad=2, band_limit=False, nthreads=0):
     f = np.array(field, np.complex128, order="C", copy=True)
-    _pow2(f.shape[0] * pad, f.shape[1] * pad)
+    _sizes(f.shape[0] * pad, f.shape[1] * pad)
     lib().orc_asm(_p(f), f.shape[1], f.shape[0], px, py, lam, z, pad, int(band_limit), nthreads)
     return f
 
@@ -211,7 +211,7 @@ def dm_encode(spectrum, w, h, py, nthreads=0):
     """Symmetrisation encode of a centred-order spectrum (Ny x Nx) to an
     h x w real fringe (scene window centred in the grid)."""
     U = np.array(spectrum, np.complex128, order="C", copy=True)
-    _pow2(*U.shape)
+    _sizes(*U.shape)
     fr = np.zeros((h, w), np.float64)
     lib().orc_dm_encode(_p(U), U.shape[1], U.shape[0], w, h, py, _p(fr), nthreads)
     return fr
@@ -238,7 +238,7 @@ def depthmap(img8, dep8, Nx, Ny, px, py, lam, near, far, L, quant_mode=0, Q=256,
     img8 = np.ascontiguousarray(img8, np.uint8)
     dep8 = np.ascontiguousarray(dep8, np.uint8)
     h, w = img8.shape
-    _pow2(Nx, Ny)
+    _sizes(Nx, Ny)
     if w > Nx or h > Ny:
         raise ValueError("scene larger than the transform grid")
     spec = np.zeros((Ny, Nx), np.complex128) if want_spectrum else None

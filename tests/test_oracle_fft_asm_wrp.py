@@ -26,6 +26,25 @@ def test_fft_vs_numpy_and_dft():
     assert abs(np.sum(abs(b) ** 2) / a.size - np.sum(abs(a) ** 2)) < 1e-9
 
 
+def test_fft_mixed_radix_vs_numpy_and_dft():
+    """Lengths with factors 3, 5 and larger primes (the shipped depth-map
+    grid is 11520 x 2160 = 2^8 3^2 5 x 2^4 3^3 5): numpy's FFT, a direct
+    O(N^2) DFT, and the full-size transform against numpy on one axis."""
+    rng = np.random.default_rng(3)
+    for ny, nx in [(6, 10), (15, 12), (45, 30), (135, 18), (7, 11), (540, 720), (2160, 8)]:
+        a = rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))
+        assert rel_l2(O.fft2d(a, -1), np.fft.fft2(a)) < 1e-13, (ny, nx)
+        assert rel_l2(O.fft2d(a, +1, normalize=True), np.fft.ifft2(a)) < 1e-13, (ny, nx)
+    ny, nx = 9, 15
+    a = rng.standard_normal((ny, nx)) + 1j * rng.standard_normal((ny, nx))
+    ky, kx = np.arange(ny)[:, None, None, None], np.arange(nx)[None, :, None, None]
+    y, x = np.arange(ny)[None, None, :, None], np.arange(nx)[None, None, None, :]
+    dft = (np.exp(-2j * np.pi * (ky * y / ny + kx * x / nx)) * a[None, None]).sum(axis=(2, 3))
+    assert rel_l2(O.fft2d(a, -1), dft) < 1e-13
+    a = rng.standard_normal((2, 11520)) + 1j * rng.standard_normal((2, 11520))
+    assert rel_l2(O.fft2d(a, -1), np.fft.fft2(a)) < 1e-13
+
+
 def test_asm_plane_wave_eigenfunction():
     """Without padding a grid-periodic plane wave is an eigenfunction: the
     output is H(fx, fy) times the input, H = exp(i 2 pi z sqrt(1/l^2 - f^2))."""
