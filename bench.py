@@ -297,6 +297,53 @@ def bench_depthmap(ctx, args):
     return line
 
 
+def bench_depthmap_shipped(ctx, args):
+    """The reference's own depth-map configuration (DM/bin/config_openholo.txt,
+    committed as tests/golden/dm_config_openholo.txt): 11520 x 2160 SLM,
+    pitches 25.4 mm/2250/3 x 25.4 mm/2250, 522 nm, field lens 1 m, depth
+    0.2..0.7 m requantised to 256 levels, transformed unpadded (mixed-radix
+    2^8 3^2 5 x 2^4 3^3 5), one colour, double fringe out; the synthetic scene
+    fills the SLM as the reference's resampled input does."""
+    from openhologram_amd import lib as L, scenes
+    c = L.dm_config_load(os.path.join(ROOT, "tests", "golden", "dm_config_openholo.txt"))
+    w, h, nl = c.pn[0], c.pn[1], c.num_of_depth
+    pitch = (c.pp[0], c.pp[1])
+    kw = dict(quant_mode=c.flag_change_depth_quantization, field_lens=c.field_lens)
+    img, dep = scenes.depth_scene(w, h, seed=22, colors=1)
+    dimg, ddep = ctx.to_device(img), ctx.to_device(dep)
+    out = ctx.empty((1, h, w), np.float64)
+    steps = 2
+
+    def step():
+        ctx.depthmap(w, h, w, h, pitch, c.near_depthmap, c.far_depthmap, nl, [c.lambda_], dimg, ddep,
+                     device=True, asynchronous=True, dst=out, **kw)
+    a2 = argparse.Namespace(**vars(args))
+    a2.steps = steps
+    a2.warmup = 1
+    wall, per = _time_steps(ctx, step, a2, DM_KERNELS)
+    from oracle import oracle as O
+    idx = O.depth_index(dep, kw["quant_mode"], 256, c.near_depthmap, c.far_depthmap, nl)
+    busy = np.bincount(idx.ravel(), minlength=nl + 1)[1:]
+    layers = int((busy > 0).sum())
+    line = {"value": steps / wall, "unit": "holograms/s (1 colour)", "ms_per_step": wall / steps * 1e3,
+            "config": {"workload": "depthmap_shipped_config_11520x2160_256levels_field_lens_522nm"},
+            "kernels": per, "layers_populated": layers}
+    if not args.no_cpu:
+        l0 = int(np.argmax(busy)) + 1
+        t0 = time.perf_counter()
+        _, ref = O.depthmap(img[0], dep, w, h, pitch[0], pitch[1], c.lambda_, c.near_depthmap, c.far_depthmap,
+                            nl, layers=(l0, l0), nthreads=host_cores(), **kw)
+        dt = time.perf_counter() - t0
+        sub = ctx.depthmap(w, h, w, h, pitch, c.near_depthmap, c.far_depthmap, nl, [c.lambda_], img, dep,
+                           layers=(l0, l0), **kw)[0]
+        line["rel_l2_vs_fp64_1_layer"] = rel_l2(sub, ref)
+        line["cpu_baseline"] = {"value": 1.0 / (dt * layers), "unit": "holograms/s (1 colour)",
+                                "cores": host_cores(), "kind": "port",
+                                "sample": f"FP64 oracle, the most populated layer at full size ({dt:.2f} s "
+                                          f"incl. one encode), scaled to {layers} layers"}
+    return line
+
+
 def bench_fresnel(ctx, args):
     """Fresnel point CGH: 1e6 points on 64 planes, 4096^2 at 8 um: MFMA
     contraction per precision and the plain FP32 VALU kernel.  `value` is
@@ -517,7 +564,8 @@ def bench_fft2d(ctx, args):
     return line
 
 
-WORKLOADS = {"rs": bench_rs, "wrp": bench_wrp, "depthmap": bench_depthmap, "fresnel": bench_fresnel,
+WORKLOADS = {"rs": bench_rs, "wrp": bench_wrp, "depthmap": bench_depthmap,
+             "depthmap_shipped": bench_depthmap_shipped, "fresnel": bench_fresnel,
              "learned": bench_learned, "fft2d": bench_fft2d}
 DEFAULT = "wrp"
 
