@@ -176,9 +176,17 @@ int oph_asm_propagate(oph_ctx* ctx, oph_c64* field, int nx, int ny, double px,
  * mode 2: CWO's cwoPLS::WRP (CWO/cwoPLS.cpp:522-591), Fresnel phase
  *   wrp[..] += a exp(i k (dz + ((wx p)^2 + (wy p)^2)/(2 dz))),
  * half-width w = (int)(|dz| lambda/(2p)/p) (:557; the reference takes |dz|
- * from its call argument, equal to dz for points at z = 0).
- * pts->phase is ignored (the reference points carry none).  wrp is ny x nx
- * complex64, row index = y.  OPH_ACCUMULATE adds into wrp. */
+ * from its call argument, equal to dz for points at z = 0); mode 3: the
+ * MATLAB statement (OpenHolo_WRP/WRP_matlab_ver 1.0/Hologram_WRP.m:19-36,
+ * fn_wrp.m): full width N = round(|lambda dz/p^2/2| + 0.5) 2 - 1, square
+ * window wx, wy in [-(N-1)/2, (N-1)/2], centre round(x/p) + (nx-1)/2 - 1
+ * (likewise y; the script's 1-based round(x/p) + (res-1)/2), and
+ *   wrp[..] += a exp(i phase) exp(i k r) / r
+ * with pts->phase the script's per-point random phase (radians; NULL: 0);
+ * points with dz = 0 add nothing.  The script's matrix holds x along rows:
+ * it is the transpose of wrp.  Modes 0-2 ignore pts->phase (the reference
+ * points carry none).  wrp is ny x nx complex64, row index = y.
+ * OPH_ACCUMULATE adds into wrp. */
 int oph_wrp(oph_ctx* ctx, int nx, int ny, double pitch, double lambda,
             double z_wrp, int mode, const oph_points* pts, oph_c64* wrp, int flags);
 

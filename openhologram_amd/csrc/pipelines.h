@@ -11,8 +11,8 @@ int oph_asm_device(oph_ctx* ctx, float2* field, int nx, int ny, int npx, int npy
 
 // WRP accumulation into a zeroed-or-overwritten nx x ny complex64 plane.
 int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z_wrp, int mode,
-                   const double* x, const double* y, const double* z, const double* amp, int64_t n,
-                   float2* wrp, int accumulate, int check);
+                   const double* x, const double* y, const double* z, const double* amp, const double* phase,
+                   int64_t n, float2* wrp, int accumulate, int check);
 
 // Records the device address of the allocation-free WRP's static scratch on
 // `device` (current); called once per context creation.

@@ -13,7 +13,11 @@
 // statement cwoPLS::WRP (CWO/cwoPLS.cpp:522-591): Fresnel phase
 // r = dz + rho^2/(2 dz), amplitude a (no 1/(r + 0.05)), half-width
 // (int)(|dz| lambda/(2p)/p) (:557; the reference takes |dz| from its call
-// argument, which is dz when the points sit at z = 0).
+// argument, which is dz when the points sit at z = 0).  Mode 3 is the
+// MATLAB statement (OpenHolo_WRP/WRP_matlab_ver 1.0/Hologram_WRP.m:19-36):
+// full width N = round(|lambda dz/p^2/2| + 0.5) 2 - 1, window
+// [-(N-1)/2, (N-1)/2], centre round(x_n/p) + (Nx-1)/2 - 1, amplitude
+// a_n/r, and the point's phase (the script's random phase) added.
 //
 // The reference scatters with a racing AddPixel; here the plane is gathered
 // per 64x64 tile, with no global atomics:
@@ -64,7 +68,7 @@ struct WrpPoint {
     float cd;     // -sign |dz| / D
     float pad0, pad1, pad2;
     double dz;    // for the exact path
-    double pad3;
+    double ph;    // initial phase (turns), mode 3; 0 otherwise
 };
 
 struct WrpArgs {
@@ -72,6 +76,7 @@ struct WrpArgs {
     const double* y;
     const double* z;
     const double* amp;
+    const double* phase;     // radians, mode 3 only (may be null)
     int64_t n;
     int nx, ny, ntx, nty, ntiles;
     double pitch, inv_lambda, z_wrp, grating, lambda;
@@ -109,13 +114,15 @@ __host__ __device__ constexpr size_t wrp_list_entries(int64_t np) { return (size
 
 __device__ __forceinline__ void tile_range(const WrpPoint& q, const WrpArgs& a, int& x0, int& x1, int& y0,
                                            int& y1) {
-    int lo = q.tx - q.w, hi = q.tx + q.w - 1;
+    // window [t - w, t + w), or [t - w + 1, t + w) for mode 3 (odd width)
+    const int lo_add = a.mode == 3 ? 1 : 0;
+    int lo = q.tx - q.w + lo_add, hi = q.tx + q.w - 1;
     if (lo < 0) lo = 0;
     if (hi > a.nx - 1) hi = a.nx - 1;
     x0 = lo / TILE;
     x1 = hi / TILE;
     if (lo > hi) x1 = x0 - 1;
-    lo = q.ty - q.w;
+    lo = q.ty - q.w + lo_add;
     hi = q.ty + q.w - 1;
     if (lo < 0) lo = 0;
     if (hi > a.ny - 1) hi = a.ny - 1;
@@ -126,38 +133,46 @@ __device__ __forceinline__ void tile_range(const WrpPoint& q, const WrpArgs& a, 
 
 __device__ WrpPoint make_point(const WrpArgs& a, int64_t i) {
     // every mode's quantities are formed and the mode's selected at the end
-    // (no branch on the mode: one straight-line path for all three)
+    // (no branch on the mode: one straight-line path for all four)
     const double dz = a.z_wrp - a.z[i];
-    const bool fres = a.mode == 2;
+    const bool fres = a.mode == 2, mat = a.mode == 3;
     WrpPoint q;
     // centres and half-widths are clamped so that tx +- w stays inside int:
     // |tx|, |ty| <= 2^29 and w <= 2^30 (a window that large covers the whole
-    // plane of any allowed size from any clamped centre that can reach it)
-    const double cx = fmin(fmax(trunc(a.x[i] / a.pitch), -536870912.0), 536870912.0);
-    const double cy = fmin(fmax(trunc(a.y[i] / a.pitch), -536870912.0), 536870912.0);
-    q.tx = (int)cx + (a.nx >> 1);
-    q.ty = (int)cy + (a.ny >> 1);
+    // plane of any allowed size from any clamped centre that can reach it).
+    // Mode 3 rounds the centre (the script's round(x/p) + (res-1)/2, 1-based)
+    const double xs = a.x[i] / a.pitch, ys = a.y[i] / a.pitch;
+    const double cx = fmin(fmax(mat ? round(xs) : trunc(xs), -536870912.0), 536870912.0);
+    const double cy = fmin(fmax(mat ? round(ys) : trunc(ys), -536870912.0), 536870912.0);
+    q.tx = (int)cx + (mat ? (a.nx - 1) / 2 - 1 : (a.nx >> 1));
+    q.ty = (int)cy + (mat ? (a.ny - 1) / 2 - 1 : (a.ny >> 1));
     const double w0 = trunc(fabs(dz) * a.grating / a.pitch);
     const double w1 = trunc(fabs(a.lambda * dz / a.pitch / a.pitch / 2 + 0.5)) * 2 - 1;
     const double w2 = trunc(fabs(dz) * (a.lambda / (2.0 * a.pitch)) / a.pitch);
-    const double wd = a.mode == 0 ? w0 : a.mode == 1 ? w1 : w2;
+    // mode 3: full width N = round(|lambda dz/p^2/2| + 0.5) 2 - 1, stored as
+    // w = (N - 1)/2 + 1 over [t - w + 1, t + w); nothing for dz = 0 (r = 0)
+    const double n3 = round(fabs(a.lambda * dz / (a.pitch * a.pitch) / 2) + 0.5) * 2 - 1;
+    const double w3 = dz == 0.0 ? 0.0 : (n3 - 1) / 2 + 1;
+    const double wd = a.mode == 0 ? w0 : a.mode == 1 ? w1 : a.mode == 2 ? w2 : w3;
     q.w = (int)fmin(fmax(wd, 0.0), 1073741824.0);
     const double sgn = dz > 0.0 ? 1.0 : -1.0;
     const double adz = fabs(dz);
     const double am = a.amp ? a.amp[i] : 1.0;
-    double t = sgn * adz * a.inv_lambda;
+    const double ph = (mat && a.phase) ? a.phase[i] * (1.0 / (2.0 * M_PI)) : 0.0;
+    double t = sgn * adz * a.inv_lambda + ph;
     t -= rint(t);
     q.f0 = (float)t;
     q.ks = (float)(sgn * adz * a.inv_lambda);
     const double p2 = adz > 0.0 ? a.pitch * a.pitch / (dz * dz) : 0.0;
     q.p2dz2 = (float)p2;
-    const double D = sgn * adz + 0.05;
-    // spherical modes: amplitude a/D (1 + cd P); Fresnel (mode 2): a
+    // spherical amplitude a/(r + off) = a/D (1 + cd P), D = sign|dz| + off:
+    // off = 0.05 (WRP.cpp) or 0 (mode 3, a/r); Fresnel (mode 2): a
+    const double D = sgn * adz + (mat ? 0.0 : 0.05);
     q.ampd = fres ? (float)am : (float)(am / D);
     q.cd = fres ? 0.f : (float)(-sgn * adz / D);
     q.pad0 = q.pad1 = q.pad2 = 0.f;
     q.dz = dz;
-    q.pad3 = 0.0;
+    q.ph = ph;
     const double umax = 2.0 * (double)q.w * q.w * p2;
     // spherical: series error at the window corner, (|dz|/lambda) 7/256 u^5
     // turns, and the amplitude's dropped second-order term (cd P)^2
@@ -353,6 +368,8 @@ struct GatherArgs {
     int nx, ny, ntx;
     double pitch, inv_lambda;
     float lambda_f;
+    float r_off;     // spherical amplitude a/(r + r_off): 0.05, or 0 for mode 3
+    int lo_add;      // 1 for mode 3 (window [t - w + 1, t + w)), else 0
     float2* wrp;
     int accumulate;
 };
@@ -373,7 +390,7 @@ __device__ __forceinline__ void abs_range(int w0, int w1, int& lo, int& hi) {
 
 // Point record as the gather stages it in LDS (32 bytes): ip = index << 2 |
 // evaluation path; the amplitude slope cd = -sign|dz|/D of the spherical
-// modes is formed again from ks (sign|dz| = ks lambda, D = ks lambda + 0.05).
+// modes is formed again from ks (sign|dz| = ks lambda, D = ks lambda + r_off).
 struct GRec {
     int tx, ty, w;
     unsigned ip;
@@ -481,8 +498,8 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                 const GRec q = recs[kk];
                 const int tx = q.tx, ty = q.ty, w = q.w;
                 // window offsets of this tile's part: wx in [wx0, wx1), wy in [wy0, wy1)
-                const int wx0 = max(tx - w, max(X0, 0)) - tx, wx1 = min(tx + w, min(X0 + TILE, g.nx)) - tx;
-                const int wy0 = max(ty - w, max(Y0, 0)) - ty, wy1 = min(ty + w, min(Y0 + TILE, g.ny)) - ty;
+                const int wx0 = max(tx - w + g.lo_add, max(X0, 0)) - tx, wx1 = min(tx + w, min(X0 + TILE, g.nx)) - tx;
+                const int wy0 = max(ty - w + g.lo_add, max(Y0, 0)) - ty, wy1 = min(ty + w, min(Y0 + TILE, g.ny)) - ty;
                 if (wx1 <= wx0 || wy1 <= wy0) continue;
                 int ax0, ax1, ay0, ay1;
                 abs_range(wx0, wx1, ax0, ax1);
@@ -503,7 +520,7 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                 const int qpath = (int)(q.ip & 3u);
                 // spherical modes: amplitude slope -sign|dz|/D, sign|dz| = ks lambda
                 const float sdz = q.ks * g.lambda_f;
-                const float qcd = FRESNEL_MODE ? 0.f : -sdz / (sdz + 0.05f);
+                const float qcd = FRESNEL_MODE ? 0.f : -sdz / (sdz + g.r_off);
                 // one loop per evaluation path (the path is uniform over the point)
                 auto walk = [&](auto exact_tag, auto fresnel_tag) {
                     constexpr bool EXACT = decltype(exact_tag)::value;
@@ -537,11 +554,12 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                             const double dx = ax * g.pitch, dy = ay * g.pitch;
                             const double sgn = dz > 0.0 ? 1.0 : -1.0;
                             const double r = sgn * sqrt(dx * dx + dy * dy + dz * dz);
-                            double t = r * g.inv_lambda;
+                            double t = r * g.inv_lambda + g.pts[qidx].ph;
                             t -= rint(t);
-                            // amplitude a/(r + 0.05) = (a/D) D/(r + 0.05), D = sign|dz| + 0.05
-                            const double D = sgn * fabs(dz) + 0.05;
-                            const float am = (float)((double)amps * D / (r + 0.05));
+                            // amplitude a/(r + off) = (a/D) D/(r + off), D = sign|dz| + off
+                            const double off = (double)g.r_off;
+                            const double D = sgn * fabs(dz) + off;
+                            const float am = (float)((double)amps * D / (r + off));
                             re = am * cos_turns((float)t);
                             im = am * sin_turns((float)t);
                         }
@@ -624,13 +642,14 @@ struct WrpScratch {
 };
 
 int wrp_core(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z_wrp, int mode,
-             const double* x, const double* y, const double* z, const double* amp, int64_t n, float2* wrp,
-             int accumulate, int check, const WrpScratch& ws) {
+             const double* x, const double* y, const double* z, const double* amp, const double* phase,
+             int64_t n, float2* wrp, int accumulate, int check, const WrpScratch& ws) {
     WrpArgs a{};
     a.x = x;
     a.y = y;
     a.z = z;
     a.amp = amp;
+    a.phase = phase;
     a.n = n;
     a.nx = nx;
     a.ny = ny;
@@ -706,6 +725,8 @@ int wrp_core(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z
         g.pitch = pitch;
         g.inv_lambda = a.inv_lambda;
         g.lambda_f = (float)lambda;
+        g.r_off = mode == 3 ? 0.f : 0.05f;
+        g.lo_add = mode == 3 ? 1 : 0;
         g.wrp = wrp;
         g.accumulate = accumulate;
         if (mode == 2)
@@ -736,9 +757,9 @@ int wrp_core(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z
 }  // namespace
 
 int oph_wrp_device(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z_wrp, int mode,
-                   const double* x, const double* y, const double* z, const double* amp, int64_t n,
-                   float2* wrp, int accumulate, int check) {
-    return wrp_core(ctx, nx, ny, pitch, lambda, z_wrp, mode, x, y, z, amp, n, wrp, accumulate, check,
+                   const double* x, const double* y, const double* z, const double* amp, const double* phase,
+                   int64_t n, float2* wrp, int accumulate, int check) {
+    return wrp_core(ctx, nx, ny, pitch, lambda, z_wrp, mode, x, y, z, amp, phase, n, wrp, accumulate, check,
                     WrpScratch{false, nullptr, nullptr, nullptr, 0});
 }
 
@@ -750,7 +771,7 @@ extern "C" int oph_wrp(oph_ctx* ctx, int nx, int ny, double pitch, double lambda
     if (nx <= 0 || ny <= 0 || nx > 65536 || ny > 65536) return oph_fail(ctx, OPH_EINVAL, "bad WRP size");
     if (!(pitch > 0) || !(lambda > 0) || lambda >= 2 * pitch)
         return oph_fail(ctx, OPH_EINVAL, "pitch and wavelength must be positive with lambda < 2 pitch");
-    if (mode < 0 || mode > 2) return oph_fail(ctx, OPH_EINVAL, "mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return oph_fail(ctx, OPH_EINVAL, "mode must be 0, 1, 2 or 3");
     if ((flags & OPH_ASYNC) && !(flags & OPH_DEVICE_PTR))
         return oph_fail(ctx, OPH_EINVAL, "OPH_ASYNC needs device pointers");
     if (pts->n < 0 || (pts->n > 0 && (!pts->x || !pts->y || !pts->z)))
@@ -758,12 +779,15 @@ extern "C" int oph_wrp(oph_ctx* ctx, int nx, int ny, double pitch, double lambda
     const int64_t n = pts->n;
     const size_t pb = (size_t)(n > 0 ? n : 1) * sizeof(double);
     const int sf = n > 0 ? flags : (flags | OPH_DEVICE_PTR);
-    if (!(flags & OPH_DEVICE_PTR) && !oph_scratch(ctx, SLOT_PTS, 4 * pb)) return OPH_ENOMEM;
+    // the phase is read by mode 3 only
+    const double* hph = mode == 3 ? pts->phase : nullptr;
+    if (!(flags & OPH_DEVICE_PTR) && !oph_scratch(ctx, SLOT_PTS, 5 * pb)) return OPH_ENOMEM;
     const double* dx = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->x, pb, sf, 0));
     const double* dy = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->y, pb, sf, pb));
     const double* dz = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->z, pb, sf, 2 * pb));
     const double* da = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->amp, pb, sf, 3 * pb));
-    if (n > 0 && (!dx || !dy || !dz || (pts->amp && !da))) return OPH_ENOMEM;
+    const double* dp = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, hph, pb, sf, 4 * pb));
+    if (n > 0 && (!dx || !dy || !dz || (pts->amp && !da) || (hph && !dp))) return OPH_ENOMEM;
     const size_t ob = (size_t)nx * ny * sizeof(float2);
     float2* out = reinterpret_cast<float2*>(wrp);
     if (!(flags & OPH_DEVICE_PTR)) {
@@ -772,7 +796,7 @@ extern "C" int oph_wrp(oph_ctx* ctx, int nx, int ny, double pitch, double lambda
         if (flags & OPH_ACCUMULATE)
             OPH_HIP_CHECK(ctx, hipMemcpyAsync(out, wrp, ob, hipMemcpyHostToDevice, ctx->stream));
     }
-    if ((rc = oph_wrp_device(ctx, nx, ny, pitch, lambda, z_wrp, mode, dx, dy, dz, da, n, out,
+    if ((rc = oph_wrp_device(ctx, nx, ny, pitch, lambda, z_wrp, mode, dx, dy, dz, da, dp, n, out,
                              (flags & OPH_ACCUMULATE) ? 1 : 0, !(flags & OPH_ASYNC))))
         return rc;
     if (!(flags & OPH_DEVICE_PTR))
@@ -786,7 +810,7 @@ extern "C" int oph_wrp_hologram(oph_ctx* ctx, int n_side, double pitch, double l
     int rc = oph_enter(ctx);
     if (rc) return rc;
     if (pad != 1 && pad != 2) return oph_fail(ctx, OPH_EINVAL, "pad must be 1 or 2");
-    if (mode < 0 || mode > 2) return oph_fail(ctx, OPH_EINVAL, "mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return oph_fail(ctx, OPH_EINVAL, "mode must be 0, 1, 2 or 3");
     const int np = n_side * pad;
     if (n_side <= 0 || (np & (np - 1)) || np < 16 || np > 8192)
         return oph_fail(ctx, OPH_ENOTSUP, "padded grid %d must be a power of two in 16..8192", np);
@@ -798,17 +822,19 @@ extern "C" int oph_wrp_hologram(oph_ctx* ctx, int n_side, double pitch, double l
         const int64_t n = pts->n;
         const size_t pb = (size_t)(n > 0 ? n : 1) * sizeof(double);
         const int sf = n > 0 ? flags : (flags | OPH_DEVICE_PTR);
-        if (!oph_scratch(ctx, SLOT_PTS, 4 * pb)) return OPH_ENOMEM;
+        const double* hph = mode == 3 ? pts->phase : nullptr;
+        if (!oph_scratch(ctx, SLOT_PTS, 5 * pb)) return OPH_ENOMEM;
         const double* dx = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->x, pb, sf, 0));
         const double* dy = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->y, pb, sf, pb));
         const double* dz = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->z, pb, sf, 2 * pb));
         const double* da = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, pts->amp, pb, sf, 3 * pb));
-        if (n > 0 && (!dx || !dy || !dz || (pts->amp && !da))) return OPH_ENOMEM;
+        const double* dp = static_cast<const double*>(oph_stage_in(ctx, SLOT_PTS, hph, pb, sf, 4 * pb));
+        if (n > 0 && (!dx || !dy || !dz || (pts->amp && !da) || (hph && !dp))) return OPH_ENOMEM;
         const size_t ob = (size_t)n_side * n_side * sizeof(float2);
         float2* d = static_cast<float2*>(oph_scratch(ctx, SLOT_OUT, ob));
         float2* work = static_cast<float2*>(oph_scratch(ctx, SLOT_WORK1, (size_t)np * np * sizeof(float2)));
         if (!d || !work) return OPH_ENOMEM;
-        if ((rc = oph_wrp_device(ctx, n_side, n_side, pitch, lambda, z_wrp, mode, dx, dy, dz, da, n, d, 0, 1)))
+        if ((rc = oph_wrp_device(ctx, n_side, n_side, pitch, lambda, z_wrp, mode, dx, dy, dz, da, dp, n, d, 0, 1)))
             return rc;
         if ((rc = oph_asm_device(ctx, d, n_side, n_side, np, np, pitch, pitch, lambda, distance, band_limit,
                                  work)))
@@ -820,7 +846,7 @@ extern "C" int oph_wrp_hologram(oph_ctx* ctx, int n_side, double pitch, double l
     float2* work = static_cast<float2*>(oph_scratch(ctx, SLOT_WORK1, (size_t)np * np * sizeof(float2)));
     if (!work) return OPH_ENOMEM;
     if ((rc = oph_wrp_device(ctx, n_side, n_side, pitch, lambda, z_wrp, mode, pts->x, pts->y, pts->z, pts->amp,
-                             pts->n, d, 0, !(flags & OPH_ASYNC))))
+                             mode == 3 ? pts->phase : nullptr, pts->n, d, 0, !(flags & OPH_ASYNC))))
         return rc;
     if ((rc = oph_asm_device(ctx, d, n_side, n_side, np, np, pitch, pitch, lambda, distance, band_limit, work)))
         return rc;
@@ -900,14 +926,15 @@ extern "C" int oph_wrp_noalloc(oph_ctx* ctx, int nx, int ny, double pitch, doubl
                         (long long)pts->n, (long long)NOALLOC_MAX_POINTS);
     if (!(pitch > 0) || !(lambda > 0) || lambda >= 2 * pitch)
         return oph_fail(ctx, OPH_EINVAL, "pitch and wavelength must be positive with lambda < 2 pitch");
-    if (mode < 0 || mode > 2) return oph_fail(ctx, OPH_EINVAL, "mode must be 0, 1 or 2");
+    if (mode < 0 || mode > 3) return oph_fail(ctx, OPH_EINVAL, "mode must be 0, 1, 2 or 3");
     if (pts->n > 0 && (!pts->x || !pts->y || !pts->z)) return oph_fail(ctx, OPH_EINVAL, "bad point arrays");
     void* base = ctx->device < MAX_DEVICES ? noalloc_ptr[ctx->device] : nullptr;
     if (!base) return oph_fail(ctx, OPH_EDEVICE, "the static WRP scratch has no address on device %d", ctx->device);
     char* b = static_cast<char*>(base);
     const WrpScratch ws{true, reinterpret_cast<WrpPoint*>(b), reinterpret_cast<unsigned*>(b + NOALLOC_PTS_BYTES),
                         reinterpret_cast<unsigned*>(b + NOALLOC_PTS_BYTES + NOALLOC_BOOK_BYTES), NOALLOC_LIST};
-    if ((rc = wrp_core(ctx, nx, ny, pitch, lambda, z_wrp, mode, pts->x, pts->y, pts->z, pts->amp, pts->n,
+    if ((rc = wrp_core(ctx, nx, ny, pitch, lambda, z_wrp, mode, pts->x, pts->y, pts->z, pts->amp,
+                       mode == 3 ? pts->phase : nullptr, pts->n,
                        reinterpret_cast<float2*>(wrp), (flags & OPH_ACCUMULATE) ? 1 : 0, !(flags & OPH_ASYNC), ws)))
         return rc;
     return oph_finish(ctx, flags);

@@ -332,9 +332,10 @@ class Context:
         return field
 
     def wrp(self, nx, ny, pitch, lambda_, z_wrp, x, y, z, amp=None, mode=0, out=None, accumulate=False,
-            device=False, asynchronous=False):
-        """WRP plane (oph_wrp), (ny, nx) complex64."""
-        pts, keep = _points(x, y, z, amp, None, device)
+            device=False, asynchronous=False, phase=None):
+        """WRP plane (oph_wrp), (ny, nx) complex64.  phase (radians) is read
+        by mode 3 (the MATLAB statement's per-point random phase) only."""
+        pts, keep = _points(x, y, z, amp, phase, device)
         if out is None:
             out = self.empty((ny, nx), np.complex64) if device else np.zeros((ny, nx), np.complex64)
         flags = (DEVICE_PTR if device else 0) | (ASYNC if asynchronous else 0) | \
@@ -345,9 +346,9 @@ class Context:
         return out
 
     def wrp_hologram(self, n_side, pitch, lambda_, z_wrp, distance, x, y, z, amp=None, pad=2,
-                     band_limit=False, mode=0, out=None, device=False, asynchronous=False):
+                     band_limit=False, mode=0, out=None, device=False, asynchronous=False, phase=None):
         """WRP + ASM hologram (oph_wrp_hologram), (n, n) complex64."""
-        pts, keep = _points(x, y, z, amp, None, device)
+        pts, keep = _points(x, y, z, amp, phase, device)
         if out is None:
             out = self.empty((n_side, n_side), np.complex64) if device else \
                 np.zeros((n_side, n_side), np.complex64)

@@ -314,18 +314,27 @@ void orc_huygens(int nx, int ny, double pitch, double lambda, double z, double p
  * mode 2 = cwoPLS::WRP (CWO/cwoPLS.cpp:522-591): w = (int)(|dz| lambda/(2p)/p)
  *   (:557), Fresnel phase r = dz + (dx^2 + dy^2)/(2 dz) (:573), amplitude a
  *   (the reference's unit amplitude when amp is NULL; no 1/(r + 0.05)).
- * amp may be NULL (unit amplitude, as the reference's cwoObjPoint).
+ * mode 3 = the MATLAB statement (OpenHolo_WRP/WRP_matlab_ver 1.0/
+ *   Hologram_WRP.m:19-36, fn_wrp.m): N = round(|lambda dz/p^2/2| + 0.5) 2 - 1
+ *   as the FULL width, a square window wx, wy in [-(N-1)/2, (N-1)/2]
+ *   centred at round(x/p) + (Nx-1)/2 - 1 (the script's 1-based
+ *   round(x/p) + (res-1)/2, made 0-based; its matrix holds x along rows,
+ *   here along columns), sub-hologram exp(i phase_n) exp(i k r)/r with
+ *   r = sign(dz) sqrt(dx^2 + dy^2 + dz^2) (:29-30; the script's random
+ *   phase rand*2*pi is the caller's phase_n); points with dz = 0 (r = 0 at
+ *   the centre) contribute nothing.
+ * amp may be NULL (unit amplitude, as the reference's cwoObjPoint); phase
+ * (radians) is used by mode 3 only and may be NULL (zero).
  * The reference adds points in parallel with a racing AddPixel; here each
  * WRP pixel sums its contributions in point order n = 0..P-1.
  */
 void orc_wrp(int nx, int ny, double pitch, double lambda, double z_wrp, int mode,
              int64_t n, const double* x, const double* y, const double* z,
-             const double* amp, c128* wrp, int nthreads) {
+             const double* amp, const double* phase, c128* wrp, int nthreads) {
     set_threads(nthreads);
-    const double k = TWO_PI / lambda;
     const int nxh = nx >> 1, nyh = ny >> 1;
-    int* tx = (int*)malloc(sizeof(int) * (size_t)n * 3);
-    int *ty = tx + n, *tw = ty + n;
+    int* tx = (int*)malloc(sizeof(int) * (size_t)n * 4);
+    int *ty = tx + n, *tw = ty + n, *tlo = tw + n;
     const double ga = tan(asin(lambda / (2.0 * pitch)));
     for (int64_t m = 0; m < n; ++m) {
         const double dz = z_wrp - z[m];
@@ -335,8 +344,16 @@ void orc_wrp(int nx, int ny, double pitch, double lambda, double z_wrp, int mode
             tw[m] = (int)(fabs(dz) * ga / pitch);
         else if (mode == 1)
             tw[m] = (int)fabs(lambda * dz / pitch / pitch / 2 + 0.5) * 2 - 1;
-        else
+        else if (mode == 2)
             tw[m] = (int)(fabs(dz) * (lambda / (2.0 * pitch)) / pitch);
+        tlo[m] = -tw[m];
+        if (mode == 3) {
+            const int N = (int)round(fabs(lambda * dz / (pitch * pitch) / 2) + 0.5) * 2 - 1;
+            tx[m] = (int)round(x[m] / pitch) + (nx - 1) / 2 - 1;
+            ty[m] = (int)round(y[m] / pitch) + (ny - 1) / 2 - 1;
+            tlo[m] = -(N - 1) / 2;
+            tw[m] = dz == 0.0 ? -(N - 1) / 2 : (N - 1) / 2 + 1;  /* window [tlo, tw) */
+        }
     }
     memset(wrp, 0, sizeof(c128) * (size_t)nx * ny);
     int row;
@@ -345,26 +362,26 @@ void orc_wrp(int nx, int ny, double pitch, double lambda, double z_wrp, int mode
         for (int64_t m = 0; m < n; ++m) {
             const int w = tw[m];
             const int wy = row - ty[m];
-            if (wy < -w || wy >= w) continue;
+            if (wy < tlo[m] || wy >= w) continue;
             const double dz = z_wrp - z[m];
             const double sgn = dz > 0.0 ? 1.0 : -1.0;
             const double a = amp ? amp[m] : 1.0;
-            for (int wx = -w; wx < w; ++wx) {
+            const double ph = (mode == 3 && phase) ? phase[m] / TWO_PI : 0.0;
+            for (int wx = tlo[m]; wx < w; ++wx) {
                 const int col = tx[m] + wx;
                 if (col < 0 || col >= nx) continue;
                 const double dx = wx * pitch, dy = wy * pitch;
                 const double r = mode == 2 ? dz + (dx * dx + dy * dy) / (2.0 * dz)
                                            : sgn * sqrt(dx * dx + dy * dy + dz * dz);
-                double t = r / lambda;
+                double t = r / lambda + ph;
                 t -= floor(t);
-                const double inv = mode == 2 ? a : a / (r + 0.05);
+                const double inv = mode == 2 ? a : mode == 3 ? a / r : a / (r + 0.05);
                 c128* o = &wrp[(size_t)row * nx + col];
                 o->re += cos(TWO_PI * t) * inv;
                 o->im += sin(TWO_PI * t) * inv;
             }
         }
     }
-    (void)k;
     free(tx);
 }
 

@@ -56,7 +56,7 @@ def lib():
         _lib.orc_huygens.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
                                      C.c_int64, P, P, P, P, P, P, C.c_int]
         _lib.orc_wrp.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
-                                 C.c_int, C.c_int64, P, P, P, P, P, C.c_int]
+                                 C.c_int, C.c_int64, P, P, P, P, P, P, C.c_int]
         _lib.orc_fft2d.argtypes = [P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
         _lib.orc_asm.argtypes = [P, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                  C.c_double, C.c_int, C.c_int, C.c_int]
@@ -178,10 +178,10 @@ def huygens(nx, ny, pitch, lam, z, x, y, zz, amp=None, phase=None, ph=0.0, nthre
     return out
 
 
-def wrp(nx, ny, pitch, lam, z_wrp, x, y, z, amp=None, mode=0, nthreads=0):
-    x, y, z, amp = map(_d, (x, y, z, amp))
+def wrp(nx, ny, pitch, lam, z_wrp, x, y, z, amp=None, mode=0, phase=None, nthreads=0):
+    x, y, z, amp, phase = map(_d, (x, y, z, amp, phase))
     out = np.zeros((ny, nx), np.complex128)
-    lib().orc_wrp(nx, ny, pitch, lam, z_wrp, mode, len(x), _p(x), _p(y), _p(z), _p(amp),
+    lib().orc_wrp(nx, ny, pitch, lam, z_wrp, mode, len(x), _p(x), _p(y), _p(z), _p(amp), _p(phase),
                   _p(out), nthreads)
     return out
 

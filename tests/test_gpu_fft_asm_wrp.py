@@ -323,3 +323,30 @@ def test_wrp_mode2_fp64_path(ctx):
     err = rel_l2(out, ref)
     print("rel-L2", err)
     assert err < 1e-5
+
+
+@pytest.mark.parametrize("res,pitch,lam", [
+    (1025, 3.9e-6, 532e-9),   # the MATLAB script's plane: FP64 per-pixel path (corner u > 2e-3)
+    (1001, 8e-6, 473e-9),     # small corner angle: the series path with amplitude 1/r
+])
+def test_wrp_mode3_matlab_vs_oracle(ctx, res, pitch, lam):
+    """Mode 3 (the MATLAB statement: N x N window, 1/r, per-point phase,
+    rounded centre on an odd plane) against the oracle, which is pinned by
+    a transcription of the script; both signs of dz; deterministic."""
+    print(f"case: wrp mode 3 {res} p={pitch} lam={lam}", flush=True)
+    rng = np.random.default_rng(17)
+    n = 2000
+    zw = 0.5e-3
+    x, y = rng.uniform(-400, 400, (2, n)) * pitch
+    z = rng.uniform(-1.0e-3, 0.3e-3, n)
+    a = rng.uniform(0.2, 1.0, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    out = ctx.wrp(res, res, pitch, lam, zw, x, y, z, a, mode=3, phase=ph)
+    ref = O.wrp(res, res, pitch, lam, zw, x, y, z, a, mode=3, phase=ph)
+    err = rel_l2(out, ref)
+    print("rel-L2", err)
+    assert err < 1e-5
+    assert np.array_equal(out, ctx.wrp(res, res, pitch, lam, zw, x, y, z, a, mode=3, phase=ph))
+    # the phase is read by mode 3 only
+    assert np.array_equal(ctx.wrp(128, 128, pitch, lam, zw, x * 0.1, y * 0.1, z, a, mode=1, phase=ph),
+                          ctx.wrp(128, 128, pitch, lam, zw, x * 0.1, y * 0.1, z, a, mode=1))

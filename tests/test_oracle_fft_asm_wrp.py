@@ -274,3 +274,61 @@ def test_wrp_mode2_is_windowed_fresnel():
     assert np.abs(out - ref).max() < 1e-9
     # no 1/(r + 0.05) and a different phase from mode 0
     assert rel_l2(O.wrp(n, n, p, lam, zw, x, y, z, a, mode=0), out) > 0.5
+
+
+def _mround(v):
+    """MATLAB round: half away from zero"""
+    return np.sign(v) * np.floor(np.abs(v) + 0.5)
+
+
+def _matlab_wrp(obj, lam, p, res, z_wrp, phases):
+    """Hologram_WRP.m:16-36 transcribed (OpenHolo_WRP/WRP_matlab_ver 1.0):
+    1-based index arithmetic, the temp array of res + N and the crop,
+    meshgrid as MATLAB's, the script's rand*2*pi replaced by `phases`."""
+    k = 2 * np.pi / lam
+    z = z_wrp - obj[:, 2]
+    N = _mround(np.abs(lam * z / (p ** 2) / 2) + 0.5) * 2 - 1
+    Nx = _mround(obj[:, 0] / p) + (res - 1) / 2
+    Ny = _mround(obj[:, 1] / p) + (res - 1) / 2
+    H = np.zeros((res, res), np.complex128)
+    for o in range(len(obj)):
+        n = int(N[o])
+        v = np.arange(-(n - 1) / 2, (n - 1) / 2 + 1) * p
+        y_run, x_run = np.meshgrid(v, v)
+        r = np.sign(z[o]) * np.sqrt(z[o] ** 2 + y_run ** 2 + x_run ** 2)
+        sub = np.exp(1j * phases[o]) * np.exp(1j * k * r) / r
+        temp = np.zeros((res + n, res + n), np.complex128)
+        a, b = int(Nx[o]), int(Ny[o])
+        temp[a - 1:a - 1 + n, b - 1:b - 1 + n] = sub
+        H += temp[(n + 1) // 2 - 1:res + (n - 1) // 2, (n + 1) // 2 - 1:res + (n - 1) // 2]
+    return H
+
+
+def test_wrp_mode3_matches_matlab_statement():
+    """Mode 3 against the MATLAB script's own construction (square N x N
+    window, 1/r, per-point phase, x along the script's rows): the oracle's
+    plane is the transpose of the script's matrix.  Odd resolution as in the
+    script (1025), points of both signs of dz, some windows clipped."""
+    rng = np.random.default_rng(8)
+    res, p, lam, zw = 41, 3.9e-6, 532e-9, 0.5e-3
+    n = 9
+    obj = np.stack([rng.uniform(-12, 12, n) * p, rng.uniform(-12, 12, n) * p,
+                    rng.uniform(0.2e-3, 0.9e-3, n)], axis=1)
+    obj[0, 2] = 0.1e-3      # dz > 0
+    obj[1, :2] = 0.0        # on the axis
+    ph = rng.uniform(0, 2 * np.pi, n)
+    H = _matlab_wrp(obj, lam, p, res, zw, ph)
+    U = O.wrp(res, res, p, lam, zw, obj[:, 0], obj[:, 1], obj[:, 2], mode=3, phase=ph)
+    assert rel_l2(U, H.T) < 1e-10
+    # the transposition, the phase and the 1/r amplitude each matter
+    assert rel_l2(U, H) > 0.1
+    assert rel_l2(O.wrp(res, res, p, lam, zw, obj[:, 0], obj[:, 1], obj[:, 2], mode=3), H.T) > 0.1
+    # one point: the window is N = 2 floor(|lambda dz / (2 p^2)|) + 1 pixels wide
+    one = O.wrp(res, res, p, lam, zw, [0.0], [0.0], [0.0])  # mode 0 window for contrast
+    m3 = O.wrp(res, res, p, lam, zw, [0.0], [0.0], [0.0], mode=3)
+    w = int(np.floor(lam * zw / (2 * p * p)))
+    cols = np.nonzero(np.abs(m3).sum(axis=0))[0]
+    assert cols.min() == (res - 1) // 2 - 1 - w and cols.max() == (res - 1) // 2 - 1 + w
+    assert np.abs(one).sum() > 0
+    # dz = 0 contributes nothing
+    assert np.all(O.wrp(res, res, p, lam, zw, [0.0], [0.0], [zw], mode=3) == 0)

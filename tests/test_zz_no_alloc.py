@@ -53,7 +53,7 @@ def test_noalloc_limits(ctx):
     # plane of more than 1024 tiles, host pointers, too many points: errors, no launch
     assert _call(ctx, lib.oph_wrp_noalloc, 2112, pts, out, DEVICE_PTR, zw) == -1
     with pytest.raises(OphError):
-        ctx.check(_call(ctx, lib.oph_wrp_noalloc, 64, pts, out, DEVICE_PTR, zw, mode=3))
+        ctx.check(_call(ctx, lib.oph_wrp_noalloc, 64, pts, out, DEVICE_PTR, zw, mode=4))
     host = np.zeros((64, 64), np.complex64)
     hp = Points(*[v.ctypes.data_as(C.c_void_p).value for v in (x, y, z, a)], None, len(x))
     assert lib.oph_wrp_noalloc(ctx.h, 64, 64, 8e-6, 532e-9, zw, 0, C.byref(hp), ptr(host), 0) == -1
