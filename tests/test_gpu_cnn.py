@@ -141,3 +141,40 @@ def test_learned_correction_gains_over_coarse(ctx):
         assert info["psnr_db"] < 35.0 and np.array_equal(u, full)
     else:
         assert info["psnr_db"] >= 35.0
+
+
+def test_learned_correction_1024_configuration(ctx):
+    """The learned-correction configuration's own sizes: 1024^2 scenes
+    (2048^2 transform grid), 16-layer field in, 256-layer field as target,
+    8 blocks x 32 channels.  A short run (48 training pairs, 400 Adam steps
+    on 128^2 patches) must improve both the field and the reconstruction
+    PSNR at the 3 sampled depths on held-out 1024^2 scenes; the per-frame
+    gate returns the 256-layer field whenever that PSNR is below 35 dB."""
+    print("case: learned correction 1024^2", flush=True)
+    n = 1024
+    xs, ts = learned.make_pairs(ctx, range(1000, 1048), n)
+    scale = float(np.sqrt(np.mean(np.abs(xs) ** 2)))
+    net = oph.lib.Cnn(ctx, blocks=8, channels=32, seed=1)
+    hist = learned.train(ctx, net, xs, ts, scale, steps=400, batch=8, patch=128, lr=1e-3)
+    assert hist[-1] < hist[0]
+    del xs, ts
+    depths = learned.sampled_depths()
+    fgain, pgain = [], []
+    for img, dep in learned.make_scenes(range(9000, 9003), n):
+        c = learned.field(ctx, img, dep, n, 16)
+        f = learned.field(ctx, img, dep, n, 256)
+        u = learned.correct(ctx, net, c, scale)
+        pc, pl = learned.recon_psnr(ctx, c, f, depths), learned.recon_psnr(ctx, u, f, depths)
+        print(f"held-out 1024^2: coarse {pc:.2f} dB (field {rel_l2(c, f):.3e}) -> learned {pl:.2f} dB "
+              f"(field {rel_l2(u, f):.3e})", flush=True)
+        fgain.append(rel_l2(c, f) - rel_l2(u, f))
+        pgain.append(pl - pc)
+    assert min(fgain) > 0 and np.mean(pgain) > 0
+    img, dep = learned.make_scenes([9000], n)[0]
+    u, info = learned.generate(ctx, net, img, dep, n, scale)
+    print("gate", info)
+    full = learned.field(ctx, img, dep, n, 256)
+    if info["psnr_db"] < 35.0:
+        assert info["fallback"] and np.array_equal(u, full)
+    else:
+        assert not info["fallback"]
