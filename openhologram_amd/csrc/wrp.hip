@@ -367,7 +367,6 @@ struct GatherArgs {
     unsigned capacity;
     int nx, ny, ntx;
     double pitch, inv_lambda;
-    float lambda_f;
     float r_off;     // spherical amplitude a/(r + r_off): 0.05, or 0 for mode 3
     int lo_add;      // 1 for mode 3 (window [t - w + 1, t + w)), else 0
     float2* wrp;
@@ -388,15 +387,32 @@ __device__ __forceinline__ void abs_range(int w0, int w1, int& lo, int& hi) {
     }
 }
 
-// Point record as the gather stages it in LDS (32 bytes): ip = index << 2 |
-// evaluation path; the amplitude slope cd = -sign|dz|/D of the spherical
-// modes is formed again from ks (sign|dz| = ks lambda, D = ks lambda + r_off).
-struct GRec {
-    int tx, ty, w;
-    unsigned ip;
-    float f0, ks, p2dz2, ampd;
+// Per (point, tile) setup of the gather, formed once by the thread that
+// stages the point (one lane per point) and read by the wave that walks the
+// point's window (an LDS broadcast), so the wave-uniform work of a window --
+// its clipping to the tile, the walk's steps, the fixed-point scale and the
+// evaluation coefficients -- is not repeated on 64 lanes.  Series paths
+// (spherical and Fresnel) evaluate, with v = wx^2 + wy^2,
+//   phase = c0 + v (c1 + v (c2 + v (c3 + v c4)))        (turns)
+//   amplitude = a0 + v (a1 + v a2)
+// where for the spherical modes the coefficients fold ks, p^2/dz^2, the
+// series of P(u) and the amplitude slope cd in (the amplitude keeps P's first
+// two terms; the next is below 1e-10 relative for the windows this path
+// takes), and for the Fresnel mode c1 = ks p^2/(2 dz^2), the rest 0.  The
+// exact paths use a0 (amplitude x segment scale) and the point record.
+struct GSetup {
+    int ax0, ay0, na, nay;        // |wx| in [ax0, ax0 + na), |wy| in [ay0, ay0 + nay); na = 0: no overlap
+    int sx, sy;                   // advancing 64 walk positions = sy rows plus sx columns
+    float inv_na;
+    int cbyte;                    // byte offset of the window centre in the accumulator
+    int wx0, wy0;                 // this tile's part of the window offsets: w in [w0, w0 + dw)
+    unsigned dwx, dwy, nwx, nwy;  // mirror tests: |w| - w0 < dw (positive side), |w| - 1 < nw (negative)
+    float c0, c1, c2, c3, c4;
+    float a0, a1, a2;
+    unsigned ip;                  // point index << 2 | evaluation path
+    unsigned pad;
 };
-constexpr int REC_BATCH = 128;
+constexpr int REC_BATCH = 64;
 
 // The window value depends on (|wx|, |wy|) only: each (ax, ay) is evaluated
 // once and added to the up-to-four mirrored pixels (+-ax, +-ay) that lie in
@@ -452,7 +468,7 @@ __device__ unsigned segment_end(const unsigned* pre, int nchunks, unsigned total
 template <bool FRESNEL_MODE>
 __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
     __shared__ unsigned long long acc[TILE * TILE];
-    __shared__ GRec recs[REC_BATCH];
+    __shared__ GSetup recs[REC_BATCH];
     __shared__ unsigned seg_hi;
     // an overflowed binning leaves the output untouched (the call is redone
     // at the exact size, or reported)
@@ -482,45 +498,69 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
             if ((int)threadIdx.x < cnt_b) {
                 const unsigned idx = g.list[bs + threadIdx.x];
                 const WrpPoint q = g.pts[idx];
-                GRec r;
-                r.tx = q.tx;
-                r.ty = q.ty;
-                r.w = q.w;
-                r.ip = idx << 2 | (unsigned)q.exact;
-                r.f0 = q.f0;
-                r.ks = q.ks;
-                r.p2dz2 = q.p2dz2;
-                r.ampd = q.ampd;
-                recs[threadIdx.x] = r;
-            }
-            __syncthreads();
-            for (int kk = wave; kk < cnt_b; kk += NW) {
-                const GRec q = recs[kk];
+                GSetup r;
                 const int tx = q.tx, ty = q.ty, w = q.w;
                 // window offsets of this tile's part: wx in [wx0, wx1), wy in [wy0, wy1)
                 const int wx0 = max(tx - w + g.lo_add, max(X0, 0)) - tx, wx1 = min(tx + w, min(X0 + TILE, g.nx)) - tx;
                 const int wy0 = max(ty - w + g.lo_add, max(Y0, 0)) - ty, wy1 = min(ty + w, min(Y0 + TILE, g.ny)) - ty;
-                if (wx1 <= wx0 || wy1 <= wy0) continue;
-                int ax0, ax1, ay0, ay1;
-                abs_range(wx0, wx1, ax0, ax1);
-                abs_range(wy0, wy1, ay0, ay1);
-                const int na = ax1 - ax0 + 1, nay = ay1 - ay0 + 1;
+                int ax0 = 0, ax1 = -1, ay0 = 0, ay1 = -1;
+                if (wx1 > wx0 && wy1 > wy0) {
+                    abs_range(wx0, wx1, ax0, ax1);
+                    abs_range(wy0, wy1, ay0, ay1);
+                }
+                const int na = ax1 - ax0 + 1;
+                r.ax0 = ax0;
+                r.ay0 = ay0;
+                r.na = na;
+                r.nay = ay1 - ay0 + 1;
                 // lane positions qq = lane + 64 j walk the na x nay grid of
                 // (|wx|, |wy|) row by row; advancing 64 positions is sy rows
                 // plus sx columns, so no division or multiply per position
-                const float inv_na = 1.f / (float)na;
-                const int sy = (int)(64.5f * inv_na), sx = 64 - sy * na;
-                int iy = (int)(((float)lane + 0.5f) * inv_na);
-                int ix = lane - iy * na;
-                // byte address of the window centre in the accumulator
-                const int cbyte = ((ty - Y0) * TILE + (tx - X0)) * 8;
+                const float inv_na = na > 0 ? 1.f / (float)na : 0.f;
+                r.inv_na = inv_na;
+                r.sy = (int)(64.5f * inv_na);
+                r.sx = 64 - r.sy * na;
+                r.cbyte = ((ty - Y0) * TILE + (tx - X0)) * 8;
+                r.wx0 = wx0;
+                r.wy0 = wy0;
+                r.dwx = (unsigned)(wx1 - wx0);
+                r.dwy = (unsigned)(wy1 - wy0);
+                r.nwx = (unsigned)max(-wx0, 0);
+                r.nwy = (unsigned)max(-wy0, 0);
                 // the point's amplitude with the segment's fixed-point scale folded in
                 const float amps = q.ampd * scale;
+                const float P1 = q.p2dz2, P2 = P1 * P1, P3 = P2 * P1, P4 = P3 * P1;
+                r.c0 = q.f0;
+                r.a0 = amps;
+                if constexpr (FRESNEL_MODE) {
+                    r.c1 = q.ks * 0.5f * P1;
+                    r.c2 = r.c3 = r.c4 = 0.f;
+                    r.a1 = r.a2 = 0.f;
+                } else {
+                    r.c1 = q.ks * 0.5f * P1;
+                    r.c2 = q.ks * -0.125f * P2;
+                    r.c3 = q.ks * (1.f / 16.f) * P3;
+                    r.c4 = q.ks * (-5.f / 128.f) * P4;
+                    r.a1 = amps * q.cd * 0.5f * P1;
+                    r.a2 = amps * q.cd * -0.125f * P2;
+                }
+                r.ip = idx << 2 | (unsigned)q.exact;
+                r.pad = 0u;
+                recs[threadIdx.x] = r;
+            }
+            __syncthreads();
+            for (int kk = wave; kk < cnt_b; kk += NW) {
+                const GSetup q = recs[kk];
+                if (q.na <= 0) continue;
+                const int na = q.na, nay = q.nay, ax0 = q.ax0, ay0 = q.ay0, sx = q.sx, sy = q.sy;
+                int iy = (int)(((float)lane + 0.5f) * q.inv_na);
+                int ix = lane - iy * na;
+                const int cbyte = q.cbyte;
                 const unsigned qidx = q.ip >> 2;
                 const int qpath = (int)(q.ip & 3u);
-                // spherical modes: amplitude slope -sign|dz|/D, sign|dz| = ks lambda
-                const float sdz = q.ks * g.lambda_f;
-                const float qcd = FRESNEL_MODE ? 0.f : -sdz / (sdz + g.r_off);
+                const float amps = q.a0;
+                const int wx0 = q.wx0, wy0 = q.wy0;
+                const unsigned dwx = q.dwx, dwy = q.dwy, nwx = q.nwx, nwy = q.nwy;
                 // one loop per evaluation path (the path is uniform over the point)
                 auto walk = [&](auto exact_tag, auto fresnel_tag) {
                     constexpr bool EXACT = decltype(exact_tag)::value;
@@ -528,12 +568,13 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                     while (iy < nay) {
                         const int ax = ax0 + ix, ay = ay0 + iy;
                         float re, im;
-                        if constexpr (FRESNEL && !EXACT) {
-                            const float fx = (float)ax, fy = (float)ay;
-                            const float u = fmaf(fx, fx, fy * fy) * q.p2dz2;
-                            const float ph = fmaf(q.ks, 0.5f * u, q.f0);
-                            re = amps * cos_turns(ph);
-                            im = amps * sin_turns(ph);
+                        if constexpr (!EXACT) {
+                            // (24-bit multiplies: full rate, where a 32-bit one is quarter rate)
+                            const float v = (float)(__umul24(ax, ax) + __umul24(ay, ay));
+                            const float ph = fmaf(v, fmaf(v, fmaf(v, fmaf(v, q.c4, q.c3), q.c2), q.c1), q.c0);
+                            const float am = fmaf(v, fmaf(v, q.a2, q.a1), q.a0);
+                            re = am * cos_turns(ph);
+                            im = am * sin_turns(ph);
                         } else if constexpr (FRESNEL) {
                             const double dz = g.pts[qidx].dz;
                             const double dx = ax * g.pitch, dy = ay * g.pitch;
@@ -541,14 +582,6 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                             t -= rint(t);
                             re = amps * cos_turns((float)t);
                             im = amps * sin_turns((float)t);
-                        } else if constexpr (!EXACT) {
-                            const float fx = (float)ax, fy = (float)ay;
-                            const float u = fmaf(fx, fx, fy * fy) * q.p2dz2;
-                            const float p = u * fmaf(u, fmaf(u, fmaf(u, -5.f / 128.f, 1.f / 16.f), -0.125f), 0.5f);
-                            const float ph = fmaf(q.ks, p, q.f0);
-                            const float am = amps * fmaf(qcd, p, 1.f);
-                            re = am * cos_turns(ph);
-                            im = am * sin_turns(ph);
                         } else {
                             const double dz = g.pts[qidx].dz;
                             const double dx = ax * g.pitch, dy = ay * g.pitch;
@@ -570,8 +603,8 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                         // 2^32 im + re modulo 2^64
                         const unsigned long long pk =
                             (unsigned long long)(long long)ire + ((unsigned long long)(long long)iim << 32);
-                        const bool xp = ax >= wx0 && ax < wx1, xm = ax > 0 && -ax >= wx0;
-                        const bool yp = ay >= wy0 && ay < wy1, ym = ay > 0 && -ay >= wy0;
+                        const bool xp = (unsigned)(ax - wx0) < dwx, xm = (unsigned)(ax - 1) < nwx;
+                        const bool yp = (unsigned)(ay - wy0) < dwy, ym = (unsigned)(ay - 1) < nwy;
                         // byte offsets of the mirrored pixels (8 bytes per pixel)
                         const int rp = cbyte + (ay << 9), rm = cbyte - (ay << 9), cx = ax << 3;
                         char* ba = reinterpret_cast<char*>(acc);
@@ -724,7 +757,6 @@ int wrp_core(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, double z
         g.ntx = a.ntx;
         g.pitch = pitch;
         g.inv_lambda = a.inv_lambda;
-        g.lambda_f = (float)lambda;
         g.r_off = mode == 3 ? 0.f : 0.05f;
         g.lo_add = mode == 3 ? 1 : 0;
         g.wrp = wrp;
