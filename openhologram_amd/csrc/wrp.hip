@@ -561,6 +561,10 @@ __global__ __launch_bounds__(GATHER_THREADS) void wrp_gather(GatherArgs g) {
                 const float amps = q.a0;
                 const int wx0 = q.wx0, wy0 = q.wy0;
                 const unsigned dwx = q.dwx, dwy = q.dwy, nwx = q.nwx, nwy = q.nwy;
+                // the setup's LDS reads complete here, before the walk: left to
+                // the first use inside the loop, the wait would also cover the
+                // previous position's LDS atomics on every iteration
+                __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
                 // one loop per evaluation path (the path is uniform over the point)
                 auto walk = [&](auto exact_tag, auto fresnel_tag) {
                     constexpr bool EXACT = decltype(exact_tag)::value;
