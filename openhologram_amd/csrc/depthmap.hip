@@ -27,8 +27,8 @@
 // whose first term is reduced per axis in FP64.
 //
 // Passes (per colour; the depth index is shared):
-//   dm_index  depth index of every pixel in one pass (the reference makes one
-//             pass per level);
+//   dm_index  depth index of every pixel by a 256-entry table of the u8 depth
+//             (built on the host; the reference makes one pass per level);
 //   dm_cols   a workgroup owns 4 scene columns and a range of layers; two
 //             real columns are packed into one complex column, transformed
 //             along y (length Ny) and separated; only rows ky <= Ny/2 are
@@ -90,26 +90,12 @@ __device__ __forceinline__ int sgnidx(int k, int n) { return k < n / 2 ? k : k -
 
 // ---- depth index -------------------------------------------------------------
 
-__global__ void dm_index(const unsigned char* dep, int n, int mode, int Q, const double* lev, int L,
-                         double dstep, double near_m, double far_m, unsigned short* idx) {
+// The depth index depends on the u8 depth value only: the host forms the
+// 256-entry table (depth_index_table) and every pixel is one lookup.
+__global__ void dm_index(const unsigned char* dep, int n, const double* table, unsigned short* idx) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    if (mode == 0) {
-        const int v = Q - (int)dep[i];
-        idx[i] = (unsigned short)(v < 0 ? 0 : v);
-        return;
-    }
-    // change_depth_quan (CPU.cpp:74-103): every level adds (l+1) where the
-    // depth falls in [d1, d2) ([d1, d2] for the last)
-    const double dsrc = (double)dep[i] / 255.0;
-    const double dmap = (1 - dsrc) * (far_m - near_m) + near_m;
-    int acc = 0;
-    for (int l = 0; l < L; ++l) {
-        const double d1 = lev[l] - dstep / 2.0, d2 = lev[l] + dstep / 2.0;
-        const int t = (l < L - 1) ? ((dmap >= d1) && (dmap < d2)) : ((dmap >= d1) && (dmap <= d2));
-        acc += t * (l + 1);
-    }
-    idx[i] = (unsigned short)(acc > 65535 ? 65535 : acc);
+    idx[i] = (unsigned short)table[dep[i]];
 }
 
 __global__ void dm_zero(unsigned char* p, int n) {
@@ -1019,6 +1005,37 @@ int launch_dm_final_m(oph_ctx* ctx, const DmArgs& a) {
     return OPH_OK;
 }
 
+// Depth index of each u8 depth value d, as the reference forms it per pixel:
+// mode 0 (FLAG_CHANGE_DEPTH_QUANTIZATION = 0): Q - d (0, no layer, below 0);
+// mode 1, change_depth_quan (CPU.cpp:74-103): dmap = (1 - d/255)(far - near)
+// + near, and every level l adds (l + 1) where dmap falls in
+// [lev_l - dstep/2, lev_l + dstep/2) ([.., ..] for the last), capped at
+// 65535.  Each product and sum is its own statement (no fused multiply-add),
+// as the reference's double arithmetic.
+void depth_index_table(int mode, int Q, const std::vector<double>& lev, double dstep, double near_m,
+                       double far_m, double* table) {
+    const int L = (int)lev.size();
+    for (int d = 0; d < 256; ++d) {
+        if (mode == 0) {
+            const int v = Q - d;
+            table[d] = (double)(unsigned short)(v < 0 ? 0 : v);
+            continue;
+        }
+        const double dsrc = (double)d / 255.0;
+        const double span = far_m - near_m;
+        const double scaled = (1 - dsrc) * span;
+        const double dmap = scaled + near_m;
+        long long acc = 0;
+        for (int l = 0; l < L; ++l) {
+            const double half = dstep / 2.0;
+            const double d1 = lev[l] - half, d2 = lev[l] + half;
+            const bool t = (l < L - 1) ? (dmap >= d1 && dmap < d2) : (dmap >= d1 && dmap <= d2);
+            if (t) acc += l + 1;
+        }
+        table[d] = (double)(acc > 65535 ? 65535 : acc);
+    }
+}
+
 }  // namespace
 
 extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, const double* lambdas,
@@ -1061,7 +1078,8 @@ extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, cons
     const double dstep = L > 1 ? (s->far_m - s->near_m) / (L - 1) : (s->far_m + s->near_m) / 2;
     const double Sx = s->pitch_x * Nx, Sy = s->pitch_y * Ny;
     std::vector<double>& hc = ctx->host_scratch;
-    hc.assign((size_t)3 * L * ncolor + L, 0.0);
+    // per colour and layer 3 constants, the L levels, the 256-entry depth-index table
+    hc.assign((size_t)3 * L * ncolor + L + 256, 0.0);
     double zmax = 0.0;
     for (int c = 0; c < ncolor; ++c) {
         const double lam = lambdas[c];
@@ -1076,6 +1094,7 @@ extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, cons
         }
     }
     for (int l = 0; l < L; ++l) hc[(size_t)3 * L * ncolor + l] = lev[l];
+    depth_index_table(s->quant_mode, s->default_q, lev, dstep, s->near_m, s->far_m, &hc[(size_t)3 * L * ncolor + L]);
 
     const size_t npx = (size_t)w * h;
     const int ngroups = (w + GROUP - 1) / GROUP;
@@ -1115,7 +1134,7 @@ extern "C" int oph_depthmap(oph_ctx* ctx, const oph_dm_spec* s, int ncolor, cons
     }
 
     OPH_LAUNCH(ctx, "dm_index", dm_index, dim3((unsigned)((npx + 255) / 256)), dim3(256), 0, ddep, (int)npx,
-               s->quant_mode, s->default_q, dlc + (size_t)3 * L * ncolor, L, dstep, s->near_m, s->far_m, didx);
+               dlc + (size_t)3 * L * ncolor + L, didx);
 
     // truncated-series accuracy at the grid corner (see dm_transfer): the
     // first dropped term is 21/1024 u^6 z/lambda turns
