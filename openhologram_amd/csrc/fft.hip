@@ -275,6 +275,80 @@ __global__ __launch_bounds__(1024) void fft_cols_kernel(float2* data, int ncols,
     }
 }
 
+// 4096-point columns, persistent: workgroups of four columns (32-byte row
+// pieces, 1024 threads, 139 KB of LDS: one per CU) walk column groups; the
+// next group's rows are fetched into registers while the current group is
+// transformed, so the loads' HBM latency overlaps the LDS work.  Groups are
+// split by XCD (the hardware places workgroup b on XCD b mod 8): XCD x owns
+// groups [x G/8, (x + 1) G/8), its workgroups take consecutive groups at
+// each step.  The middle stage's twiddles sit in LDS, the last stage's are
+// read where used (registers go to the prefetch).  ncols is a multiple of
+// 32, the grid a multiple of 8.
+constexpr int COLP_L = 4096, COLP_B = 4, COLP_THREADS = COLP_B * (COLP_L / 16);
+
+template <int SIGN>
+__global__ __launch_bounds__(COLP_THREADS) void fft_cols_p(float2* data, int ncols, int stride, float scale,
+                                                          const float2* __restrict__ tw) {
+    constexpr int L = COLP_L, B = COLP_B, TPL = L / 16, PL = fft::padded_len(L), NTHR = COLP_THREADS;
+    constexpr int PER = (L * B / 2) / NTHR;  // 16-byte pieces per lane
+    extern __shared__ float2 lds[];
+    float2* tw1 = lds;  // middle stage (Ns = 16): 240 twiddles, k = t mod 16
+    float2* lines = lds + 256;
+    const int per = (ncols / B) / 8, steps = gridDim.x / 8;
+    const int xcd = blockIdx.x % 8;
+    const int gend = (xcd + 1) * per;
+    int g = xcd * per + blockIdx.x / 8;
+    const auto rs = oph_buf_rsrc(data, (unsigned)((size_t)L * stride * sizeof(float2)));
+    float4 ld[PER];
+    auto fetch = [&](int grp) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int idx = threadIdx.x + j * NTHR;
+            const int pr = idx % (B / 2), i = idx / (B / 2);
+            ld[j] = oph_buf_load_f4(rs, (unsigned)(((size_t)i * stride + B * grp + 2 * pr) * sizeof(float2)));
+        }
+    };
+    if (g < gend) fetch(g);
+    for (int e = threadIdx.x; e < 15 * 16; e += NTHR) tw1[e] = tw[e];
+    const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
+    float2* line = lines + b * PL;
+    const auto twl = oph_buf_rsrc(tw + fft::last_stage_tw_offset<L>(), 15 * TPL * sizeof(float2));
+    for (; g < gend; g += steps) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int idx = threadIdx.x + j * NTHR;
+            const int pr = idx % (B / 2), i = idx / (B / 2);
+            lines[(2 * pr) * PL + fft::pad16(i)] = make_float2(ld[j].x, ld[j].y);
+            lines[(2 * pr + 1) * PL + fft::pad16(i)] = make_float2(ld[j].z, ld[j].w);
+        }
+        __syncthreads();
+        if (g + steps < gend) fetch(g + steps);
+        // a zero the compiler cannot see through, added to the lane index of
+        // the loop-invariant twiddle loads, which hoisted out of the loop
+        // would hold their registers across it and spill
+        int zero;
+        asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+        const int tz = t + zero;
+        float2 w[1][16];
+        fft::stage<L, 16, SIGN, TPL>(line, w, 1, t);
+#pragma unroll
+        for (int r = 1; r < 16; ++r) w[0][r] = tw1[(r - 1) * 16 + (t & 15)];
+        fft::stage<L, 16, SIGN, TPL>(line, w, 16, t);
+#pragma unroll
+        for (int r = 1; r < 16; ++r) w[0][r] = oph_buf_load_f2(twl, tz * sizeof(float2), (r - 1) * TPL * sizeof(float2));
+        fft::stage<L, 16, SIGN, TPL>(line, w, 16 * 16, t);
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int idx = threadIdx.x + j * NTHR;
+            const int pr = idx % (B / 2), i = idx / (B / 2);
+            const float2 u = lines[(2 * pr) * PL + fft::pad16(i)], v = lines[(2 * pr + 1) * PL + fft::pad16(i)];
+            oph_buf_store_f4(make_float4(u.x * scale, u.y * scale, v.x * scale, v.y * scale), rs,
+                             (unsigned)(((size_t)i * stride + B * g + 2 * pr) * sizeof(float2)));
+        }
+        __syncthreads();  // the lines are read out before the next group is stored
+    }
+}
+
 template <int L, int SIGN>
 int launch_rows(oph_ctx* ctx, float2* data, int rows, int stride, float scale) {
     const float2* tw = oph_twiddles(ctx, L);
@@ -323,6 +397,23 @@ int launch_cols_b(oph_ctx* ctx, float2* data, int ncols, int stride, float scale
     return OPH_OK;
 }
 
+template <int SIGN>
+int launch_cols_p(oph_ctx* ctx, float2* data, int ncols, int stride, float scale, const float2* tw) {
+    const size_t shm = ((size_t)COLP_B * fft::padded_len(COLP_L) + 256) * sizeof(float2);
+    static bool attr = false;
+    if (!attr) {
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_cols_p<SIGN>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipGetLastError();
+        attr = true;
+    }
+    // one workgroup per CU (LDS), a multiple of 8, at most one per group
+    const int grid = std::max(8, std::min(ncols / COLP_B, std::max(ctx->cu_count, 1)) / 8 * 8);
+    OPH_LAUNCH(ctx, "fft_cols", (fft_cols_p<SIGN>), dim3(grid), dim3(COLP_THREADS), shm, data, ncols, stride,
+               scale, tw);
+    return OPH_OK;
+}
+
 template <int L, int SIGN>
 int launch_cols(oph_ctx* ctx, float2* data, int ncols, int stride, float scale) {
     const float2* tw = oph_twiddles(ctx, L);
@@ -333,6 +424,12 @@ int launch_cols(oph_ctx* ctx, float2* data, int ncols, int stride, float scale) 
         const char* e = getenv("OPH_FFT_COLUMNS");
         b = e ? atoi(e) : BMAX;
         if (b > BMAX || b < 1 || (b & (b - 1))) b = BMAX;
+    }
+    if constexpr (L == COLP_L) {
+        // the persistent form: four columns, 32-byte pieces, buffer offsets in 32 bits
+        if (b == COLP_B && ncols % (8 * COLP_B) == 0 && stride % 2 == 0 &&
+            (size_t)L * stride * sizeof(float2) < (1u << 31))
+            return launch_cols_p<SIGN>(ctx, data, ncols, stride, scale, tw);
     }
     switch (b) {
         case 1: return launch_cols_b<L, SIGN, 1>(ctx, data, ncols, stride, scale, tw);
