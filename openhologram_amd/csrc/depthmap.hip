@@ -369,8 +369,9 @@ __device__ __forceinline__ void finish_row8(const DmArgs& a, const float2 (&acc)
     __syncthreads();
 }
 
-// PAIR0: workgroup 0, the real rows 0 and Ny/2 packed as re/im.
-template <int LX, bool PAIR0>
+// PAIR0: workgroup 0, the real rows 0 and Ny/2 packed as re/im.  FBITS:
+// the presence bits of every (layer, group) fit in LDS (a.fbits_lds).
+template <int LX, bool PAIR0, bool FBITS>
 __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __restrict__ tw8, float2* sm, int kya) {
     constexpr int TPL = LX / 8;
     constexpr int PL = fft::padded_len(LX);
@@ -387,7 +388,7 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
     const int t = threadIdx.x;
     for (int i = t; i < NTW; i += TPL) tws[i] = tw8[i];
     for (int i = t; i < MAX_LAYERS / 32; i += TPL) amask[i] = 0u;
-    if (a.fbits_lds)
+    if constexpr (FBITS)
         for (int i = t; i < a.L * a.fwpl; i += TPL) fb[i] = a.fbits[i];
     __syncthreads();
     for (int i = t; i < a.L; i += TPL)
@@ -436,14 +437,15 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
         }
     };
     float2 nv[8];
-    unsigned nf[8];
+    unsigned nf;  // presence of the prefetched elements, bit m for element m
     auto fetch = [&](int l) {
         const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * GROUP;
-        if (a.fbits_lds) {
+        if constexpr (FBITS) {
             // flags from LDS first; then range-checked buffer loads whose
             // offset is out of range for absent groups (no memory access, a
             // zero result, no branch): the rows of absent groups were never
-            // written and are not read
+            // written and are not read, and the loaded values need no mask
+            nf = 0xFFu;
             const unsigned* fl = fb + (size_t)(l - 1) * a.fwpl;
             const auto rs = oph_buf_rsrc(a.I + (size_t)(l - 1) * lstride, (unsigned)(lstride * sizeof(float2)));
 #pragma unroll
@@ -452,7 +454,6 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
                 const int gx = x >> 2;  // x / GROUP for x >= 0
                 const bool on = (x >= 0 && x < a.w) && ((fl[gx >> 5] >> (gx & 31)) & 1u);
                 const unsigned o = (unsigned)(((size_t)gx * gstride + (x & 3) + (size_t)kya * GROUP) * sizeof(float2));
-                nf[m] = on ? 1u : 0u;
                 if constexpr (PAIR0) {
                     const float2 lo = oph_buf_load_f2(rs, on ? o : OPH_BUF_OOB);
                     const float2 hi = oph_buf_load_f2(rs, on ? o + (unsigned)(nyq * sizeof(float2)) : OPH_BUF_OOB);
@@ -464,12 +465,13 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
             return;
         }
         const unsigned char* fl = a.flags + (size_t)(l - 1) * a.ngroups;
+        nf = 0u;
 #pragma unroll
         for (int m = 0; m < 8; ++m) {
             const int x = t + m * TPL - a.ox;
             const bool in = x >= 0 && x < a.w;
             const size_t o = (size_t)(x / GROUP) * gstride + x % GROUP;
-            nf[m] = in ? fl[x / GROUP] : 0u;
+            nf |= (in && fl[x / GROUP]) ? 1u << m : 0u;
             if constexpr (PAIR0)
                 nv[m] = in ? make_float2(src[o].x, src[nyq + o].x) : make_float2(0.f, 0.f);
             else
@@ -497,7 +499,7 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
             // l is the prefetched layer
             float2 v[8];
 #pragma unroll
-            for (int m = 0; m < 8; ++m) v[m] = nf[m] ? nv[m] : make_float2(0.f, 0.f);
+            for (int m = 0; m < 8; ++m) v[m] = ((nf >> m) & 1u) ? nv[m] : make_float2(0.f, 0.f);
             pl = next_layer(l);
             if (pl) fetch(pl);
             __syncthreads();  // the previous transform's readers are done with the line
@@ -560,16 +562,18 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
     finish_row8<LX>(a, accb, kyb, line, tws, t);
 }
 
-template <int LX>
-__global__ __launch_bounds__(LX / 8) void dm_rows_h(DmArgs a, const float2* __restrict__ tw8) {
+// (two waves per SIMD: 256 registers, the budget of two 2048-point
+// workgroups per CU or one 4096-point workgroup)
+template <int LX, bool FBITS>
+__global__ __launch_bounds__(LX / 8, 2) void dm_rows_h(DmArgs a, const float2* __restrict__ tw8) {
     extern __shared__ float2 sm[];
     // XCD-aware: neighbouring rows, which share the cache lines of I, run on
     // the same XCD at the same time
     const int kya = xcd_remap(blockIdx.x, gridDim.x);  // 0 .. Ny/2 - 1
     if (kya == 0)
-        rows_h_body<LX, true>(a, tw8, sm, 0);
+        rows_h_body<LX, true, FBITS>(a, tw8, sm, 0);
     else
-        rows_h_body<LX, false>(a, tw8, sm, kya);
+        rows_h_body<LX, false, FBITS>(a, tw8, sm, kya);
 }
 
 // ---- final column pass ---------------------------------------------------------
@@ -897,8 +901,9 @@ int launch_dm_rows(oph_ctx* ctx, const DmArgs& a) {
             if (!tw8) return OPH_ENOMEM;
             static bool attr = false;
             if (!attr) {
-                hipFuncSetAttribute(reinterpret_cast<const void*>(&dm_rows_h<LX>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+                for (const void* k : {reinterpret_cast<const void*>(&dm_rows_h<LX, true>),
+                                      reinterpret_cast<const void*>(&dm_rows_h<LX, false>)})
+                    hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
                 hipGetLastError();
                 attr = true;
             }
@@ -906,7 +911,10 @@ int launch_dm_rows(oph_ctx* ctx, const DmArgs& a) {
             const size_t fbytes = (size_t)a.L * a.fwpl * sizeof(unsigned);
             b.fbits_lds = rows_h_lds<LX>() + fbytes <= 160 * 1024 ? 1 : 0;
             const size_t shm = rows_h_lds<LX>() + (b.fbits_lds ? fbytes : 0);
-            OPH_LAUNCH(ctx, "dm_rows", dm_rows_h<LX>, dim3(a.Ny / 2), dim3(LX / 8), shm, b, tw8);
+            if (b.fbits_lds)
+                OPH_LAUNCH(ctx, "dm_rows", (dm_rows_h<LX, true>), dim3(a.Ny / 2), dim3(LX / 8), shm, b, tw8);
+            else
+                OPH_LAUNCH(ctx, "dm_rows", (dm_rows_h<LX, false>), dim3(a.Ny / 2), dim3(LX / 8), shm, b, tw8);
             return OPH_OK;
         }
     }
