@@ -299,6 +299,7 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
 // are read and written through one L2 at about the same time.  npx is a
 // multiple of 8 B and the grid a multiple of 8.
 constexpr int MID_L = 4096, MID_B = 4;
+constexpr int MID_TW = 768;   // LDS entries ahead of the lines: middle-stage table, W^t, W^(4t)
 
 template <int B, bool EXACT>
 __global__ __launch_bounds__(B * (MID_L / 16), 4) void asm_cols_mid(
@@ -308,8 +309,9 @@ __global__ __launch_bounds__(B * (MID_L / 16), 4) void asm_cols_mid(
     // float4 row pieces (two columns each) per lane: H per row, rows [L/4, 3L/4)
     constexpr int PER = (L / 2) * H / NTHR;
     extern __shared__ float2 lds[];
-    float2* tw1 = lds;  // middle stage's 240 twiddles (k = t mod 16)
-    float2* lines = lds + 256;
+    float2* tw1 = lds;        // middle stage's 240 twiddles (k = t mod 16)
+    float2* wp = lds + 256;   // [t]: W^t, [TPL + t]: W^(4t), W = exp(-2 pi i/L)
+    float2* lines = lds + MID_TW;
     const int per = (npx / B) / 8, steps = gridDim.x / 8;
     const int xcd = blockIdx.x % 8;
     const int gend = (xcd + 1) * per;
@@ -328,10 +330,24 @@ __global__ __launch_bounds__(B * (MID_L / 16), 4) void asm_cols_mid(
     // it fetches each group at the top of its step instead
     constexpr bool PREFETCH = !EXACT;
     if (PREFETCH && g < gend) fetch(g);
-    for (int e = threadIdx.x; e < 15 * 16; e += NTHR) tw1[e] = tw[e];
+    constexpr int TWO = fft::last_stage_tw_offset<L>();
+    for (int e = threadIdx.x; e < TWO; e += NTHR) tw1[e] = tw[e];
+    for (int e = threadIdx.x; e < 2 * TPL; e += NTHR) wp[e] = tw[TWO + (e < TPL ? e : 2 * TPL + e)];
     const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
     float2* line = lines + b * PL;
-    const auto twl = oph_buf_rsrc(tw + fft::last_stage_tw_offset<L>(), 15 * TPL * sizeof(float2));
+    const auto twl = oph_buf_rsrc(tw + TWO, 15 * TPL * sizeof(float2));
+    // the last stages' 15 twiddles: with the prefetch in flight they are
+    // rebuilt from W^t and W^(4t) in LDS (vector memory counts complete in
+    // issue order, so a table load issued behind the prefetch would wait for
+    // the next group's rows); without it they are read from the table
+    auto last_tw = [&](float2 (&wl)[16], int tz) {
+        if constexpr (PREFETCH) {
+            fft::twiddle_powers(wp[t], wp[TPL + t], wl);
+        } else {
+#pragma unroll
+            for (int r = 1; r < 16; ++r) wl[r] = oph_buf_load_f2(twl, tz * sizeof(float2), (r - 1) * TPL * sizeof(float2));
+        }
+    };
     for (; g < gend; g += steps) {
         if constexpr (!PREFETCH) fetch(g);
 #pragma unroll
@@ -362,8 +378,7 @@ __global__ __launch_bounds__(B * (MID_L / 16), 4) void asm_cols_mid(
         for (int r = 1; r < 16; ++r) w[0][r] = tw1[(r - 1) * 16 + (t & 15)];
         fft::stage<L, 16, -1, TPL>(line, w, 16, t);
         float2 v[16];
-#pragma unroll
-        for (int r = 1; r < 16; ++r) w[0][r] = oph_buf_load_f2(twl, tz * sizeof(float2), (r - 1) * TPL * sizeof(float2));
+        last_tw(w[0], tz);
         // the forward transform's last butterfly leaves elements t + m 256 in
         // registers, the inverse's first butterfly's inputs: the transfer is
         // applied in between, without a pass through LDS
@@ -378,8 +393,7 @@ __global__ __launch_bounds__(B * (MID_L / 16), 4) void asm_cols_mid(
 #pragma unroll
         for (int r = 1; r < 16; ++r) w[0][r] = tw1[(r - 1) * 16 + (t & 15)];
         fft::stage<L, 16, +1, TPL>(line, w, 16, t);
-#pragma unroll
-        for (int r = 1; r < 16; ++r) w[0][r] = oph_buf_load_f2(twl, tz * sizeof(float2), (r - 1) * TPL * sizeof(float2));
+        last_tw(w[0], tz);
         // the last butterfly reads and writes only the lane's own elements
         // t + m 256, so no barrier separates the two
         fft::last_stage_to_regs_w<L, +1>(line, w[0], t, v);
@@ -698,7 +712,7 @@ template <int B, bool EXACT>
 int launch_cols_mid(oph_ctx* ctx, float2* work, int npx, const float4* tx, const float4* ty, const AsmConst& k,
                     const float2* tw) {
     constexpr int NTHR = B * (MID_L / 16);
-    const size_t shm = ((size_t)B * fft::padded_len(MID_L) + 256) * sizeof(float2);
+    const size_t shm = ((size_t)B * fft::padded_len(MID_L) + MID_TW) * sizeof(float2);
     static const int occ = [&] {
         hipFuncSetAttribute(reinterpret_cast<const void*>(&asm_cols_mid<B, EXACT>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);

@@ -275,77 +275,76 @@ __global__ __launch_bounds__(1024) void fft_cols_kernel(float2* data, int ncols,
     }
 }
 
-// 4096-point columns, persistent: workgroups of four columns (32-byte row
-// pieces, 1024 threads, 139 KB of LDS: one per CU) walk column groups; the
-// next group's rows are fetched into registers while the current group is
-// transformed, so the loads' HBM latency overlaps the LDS work.  Groups are
-// split by XCD (the hardware places workgroup b on XCD b mod 8): XCD x owns
-// groups [x G/8, (x + 1) G/8), its workgroups take consecutive groups at
-// each step.  The middle stage's twiddles sit in LDS, the last stage's are
-// read where used (registers go to the prefetch).  ncols is a multiple of
-// 32, the grid a multiple of 8.
+// 4096-point columns, persistent: workgroups of four columns (1024 threads,
+// 140 KB of LDS: one per CU) walk column groups.  Thread T transforms column
+// T mod 4 as butterfly t = T/4 of the row transform's register-fused form:
+// its stage-0 inputs and last-stage outputs are rows t + 256 m (m < 16), so
+// the columns go from global memory straight into the first butterfly and
+// from the last butterfly straight back -- four LDS passes per column
+// instead of eight -- and a wave's 8-byte accesses cover 16 rows x 32
+// bytes (the four columns).  The next group is loaded into the registers
+// the current one was read from while it is transformed.  Groups are split
+// by XCD (the hardware places workgroup b on XCD b mod 8): XCD x owns groups
+// [x G/8, (x + 1) G/8), its workgroups take consecutive groups at each step.
+// The middle stage's twiddles sit in LDS and the last stage's 15 per thread
+// are rebuilt from two powers (also in LDS), so no global load but the
+// columns' is issued in the loop; with the first group peeled off the loop,
+// the memory wait before a group's first butterfly covers that group's loads
+// only (the wait counter is in order).  ncols is a multiple of 32, the grid a
+// multiple of 8.
 constexpr int COLP_L = 4096, COLP_B = 4, COLP_THREADS = COLP_B * (COLP_L / 16);
+// LDS entries ahead of the lines: the middle stage's 240 twiddles, then W^t
+// and W^(4t) (W = exp(-2 pi i/4096)) for t < 256
+constexpr int COLP_TW = 256 + 2 * 256;
 
 template <int SIGN>
 __global__ __launch_bounds__(COLP_THREADS) void fft_cols_p(float2* data, int ncols, int stride, float scale,
                                                           const float2* __restrict__ tw) {
-    constexpr int L = COLP_L, B = COLP_B, TPL = L / 16, PL = fft::padded_len(L), NTHR = COLP_THREADS;
-    constexpr int PER = (L * B / 2) / NTHR;  // 16-byte pieces per lane
+    constexpr int L = COLP_L, B = COLP_B, TPL = L / 16, PL = fft::padded_len(L);
+    constexpr int TWO = fft::last_stage_tw_offset<L>();
+    static_assert(TWO <= COLP_TW, "middle-stage table");
     extern __shared__ float2 lds[];
-    float2* tw1 = lds;  // middle stage (Ns = 16): 240 twiddles, k = t mod 16
-    float2* lines = lds + 256;
+    float2* twm = lds;
+    const int b = threadIdx.x % B, t = threadIdx.x / B;
+    float2* buf = lds + COLP_TW + b * PL;
+    float2* wp = lds + 256;   // [t]: W^t, [TPL + t]: W^(4t)
+    for (int i = threadIdx.x; i < TWO; i += COLP_THREADS) twm[i] = tw[i];
+    for (int i = threadIdx.x; i < 2 * TPL; i += COLP_THREADS) wp[i] = tw[TWO + (i < TPL ? i : 2 * TPL + i)];
     const int per = (ncols / B) / 8, steps = gridDim.x / 8;
     const int xcd = blockIdx.x % 8;
     const int gend = (xcd + 1) * per;
-    int g = xcd * per + blockIdx.x / 8;
-    const auto rs = oph_buf_rsrc(data, (unsigned)((size_t)L * stride * sizeof(float2)));
-    float4 ld[PER];
+    const auto R = oph_buf_rsrc(data, (unsigned)((size_t)L * stride * sizeof(float2)));
+    // element (t + m TPL, B grp + b): a per-lane offset plus m TPL rows as
+    // the uniform (scalar) offset
+    const unsigned rowstep = (unsigned)((size_t)TPL * stride * sizeof(float2));
+    auto lane_off = [&](int grp) { return (unsigned)(((size_t)t * stride + B * grp + b) * sizeof(float2)); };
+    float2 in[16];
     auto fetch = [&](int grp) {
+        const unsigned o = lane_off(grp);
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            const int idx = threadIdx.x + j * NTHR;
-            const int pr = idx % (B / 2), i = idx / (B / 2);
-            ld[j] = oph_buf_load_f4(rs, (unsigned)(((size_t)i * stride + B * grp + 2 * pr) * sizeof(float2)));
-        }
+        for (int m = 0; m < 16; ++m) in[m] = oph_buf_load_f2(R, o, m * rowstep);
     };
-    if (g < gend) fetch(g);
-    for (int e = threadIdx.x; e < 15 * 16; e += NTHR) tw1[e] = tw[e];
-    const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
-    float2* line = lines + b * PL;
-    const auto twl = oph_buf_rsrc(tw + fft::last_stage_tw_offset<L>(), 15 * TPL * sizeof(float2));
-    for (; g < gend; g += steps) {
+    auto group = [&](int grp) {
+        float2 v[16];
 #pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            const int idx = threadIdx.x + j * NTHR;
-            const int pr = idx % (B / 2), i = idx / (B / 2);
-            lines[(2 * pr) * PL + fft::pad16(i)] = make_float2(ld[j].x, ld[j].y);
-            lines[(2 * pr + 1) * PL + fft::pad16(i)] = make_float2(ld[j].z, ld[j].w);
-        }
+        for (int m = 0; m < 16; ++m) v[m] = in[m];
+        __syncthreads();  // the previous group's last-stage reads are done
+        fft::stage0_from_regs<L, SIGN>(v, buf, t);
+        if (grp + steps < gend) fetch(grp + steps);
         __syncthreads();
-        if (g + steps < gend) fetch(g + steps);
-        // a zero the compiler cannot see through, added to the lane index of
-        // the loop-invariant twiddle loads, which hoisted out of the loop
-        // would hold their registers across it and spill
-        int zero;
-        asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
-        const int tz = t + zero;
-        float2 w[1][16];
-        fft::stage<L, 16, SIGN, TPL>(line, w, 1, t);
+        fft::middle_stages_tw<L, SIGN>(buf, twm, t);
+        float2 wl[16];
+        fft::twiddle_powers(wp[t], wp[TPL + t], wl);
+        fft::last_stage_to_regs_w<L, SIGN>(buf, wl, t, v);
+        const unsigned o = lane_off(grp);
 #pragma unroll
-        for (int r = 1; r < 16; ++r) w[0][r] = tw1[(r - 1) * 16 + (t & 15)];
-        fft::stage<L, 16, SIGN, TPL>(line, w, 16, t);
-#pragma unroll
-        for (int r = 1; r < 16; ++r) w[0][r] = oph_buf_load_f2(twl, tz * sizeof(float2), (r - 1) * TPL * sizeof(float2));
-        fft::stage<L, 16, SIGN, TPL>(line, w, 16 * 16, t);
-#pragma unroll
-        for (int j = 0; j < PER; ++j) {
-            const int idx = threadIdx.x + j * NTHR;
-            const int pr = idx % (B / 2), i = idx / (B / 2);
-            const float2 u = lines[(2 * pr) * PL + fft::pad16(i)], v = lines[(2 * pr + 1) * PL + fft::pad16(i)];
-            oph_buf_store_f4(make_float4(u.x * scale, u.y * scale, v.x * scale, v.y * scale), rs,
-                             (unsigned)(((size_t)i * stride + B * g + 2 * pr) * sizeof(float2)));
-        }
-        __syncthreads();  // the lines are read out before the next group is stored
+        for (int m = 0; m < 16; ++m) oph_buf_store_f2(make_float2(v[m].x * scale, v[m].y * scale), R, o, m * rowstep);
+    };
+    int g = xcd * per + blockIdx.x / 8;
+    if (g < gend) {
+        fetch(g);
+        group(g);
+        for (g += steps; g < gend; g += steps) group(g);
     }
 }
 
@@ -399,7 +398,7 @@ int launch_cols_b(oph_ctx* ctx, float2* data, int ncols, int stride, float scale
 
 template <int SIGN>
 int launch_cols_p(oph_ctx* ctx, float2* data, int ncols, int stride, float scale, const float2* tw) {
-    const size_t shm = ((size_t)COLP_B * fft::padded_len(COLP_L) + 256) * sizeof(float2);
+    const size_t shm = ((size_t)COLP_B * fft::padded_len(COLP_L) + COLP_TW) * sizeof(float2);
     static bool attr = false;
     if (!attr) {
         hipFuncSetAttribute(reinterpret_cast<const void*>(&fft_cols_p<SIGN>),

@@ -162,6 +162,10 @@ __device__ __forceinline__ void oph_buf_store_f2(float2 x, __amdgpu_buffer_rsrc_
     const oph_f32x2 v = {x.x, x.y};
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(oph_u32x2, v), r, off, 0, 0);
 }
+__device__ __forceinline__ void oph_buf_store_f2(float2 x, __amdgpu_buffer_rsrc_t r, unsigned off, unsigned soff) {
+    const oph_f32x2 v = {x.x, x.y};
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(oph_u32x2, v), r, off, soff, 0);
+}
 
 // Workgroup index remap for the 8 XCDs of MI355X: consecutive hardware ids go
 // round-robin over XCDs; this returns a logical id such that each XCD receives
