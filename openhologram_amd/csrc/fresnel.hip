@@ -167,22 +167,22 @@ __global__ void fresnel_sum_chunks(const float2* partial, int chunks, int64_t np
 }
 
 // ---------------------------------------------------------------------------
-// MFMA contraction.  A workgroup of 4 waves owns a 128 x 128 output tile; for
-// each chunk of 32 points it writes the chirp factors A (128 rows of y) and
-// B (128 columns of x) to LDS as 16-bit planes [re_hi, im_hi, re_lo, im_lo]
-// laid out [row][point] (generated by a phase recurrence, below), then each
-// wave contracts a 64 x 64 sub-tile with
+// MFMA contraction.  A workgroup of 8 waves owns a 256 x 128 output tile
+// (256 rows of y, 128 columns of x); for each chunk of 32 points it writes
+// the chirp factors A (256 rows) and B (128 columns) to LDS as 16-bit planes
+// [re_hi, im_hi, re_lo, im_lo] laid out [row][point] (generated by a phase
+// recurrence, below), then each wave contracts a 64 x 64 sub-tile with
 // v_mfma_f32_32x32x16 (lane l supplies row l%32, points 8(l/32)..+7 of A and
 // column l%32, the same points of B; accumulator register v of lane l is
-// output row 8(v/4) + 4(l/32) + v%4, column l%32).
+// output row 8(v/4) + 4(l/32) + v%4, column l%32).  A chirp entry is
+// generated once per tile and used by every output of its row or column, so
+// the taller tile spends 3/4 of the generation per output of a square
+// 128 x 128 one; the workgroup takes the whole CU (two waves per SIMD).
 
-constexpr int MT = 128;         // output tile (rows, columns)
-constexpr int KC = 32;          // point padding granularity (largest chunk)
-// points per chunk: 32 for the single-precision forms; 16 for the hi+lo
-// forms, whose four operand planes would otherwise need 83 KB of LDS and
-// leave one workgroup per CU (no overlap of generation and MFMA phases)
-template <bool SPLIT>
-__host__ __device__ constexpr int chunk_points() { return SPLIT ? 16 : 32; }
+constexpr int MR = 256;         // output tile rows
+constexpr int MC = 128;         // output tile columns
+constexpr int KC = 32;          // points per chunk (and padding granularity)
+constexpr int MFMA_THREADS = 512;
 
 template <typename T>
 struct Mfma;
@@ -223,23 +223,87 @@ struct MfmaArgs {
 };
 
 template <typename T, bool SPLIT>
-__global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
+struct FrLds {
+    static constexpr int NP = SPLIT ? 4 : 2;
+    static constexpr int KS = KC + 8;   // row stride (elements): 16-byte aligned rows
+    T A[NP][MR][KS];
+    T B[NP][MC][KS];
+};
+
+// Chirp factors by a phase recurrence: thread (k, gph) generates point k's
+// entries on rows gph + G m (m < MR/G) and columns gph + G m (m < MC/G).
+// Along a run the phase c (Y - y)^2 has constant second difference
+// delta = 2 c (G p)^2, so with R = exp(i 2 pi (phi_{m+1} - phi_m)),
+// A_{m+1} = A_m R_m and R_{m+1} = R_m exp(i 2 pi delta): two complex
+// multiplies per entry; the anchor phi_0, the first step and delta are
+// reduced in FP64 once per run (phases of ~1e4 turns).  FULL: every row and
+// column of the tile is inside the image (no per-entry bounds select).
+template <typename T, bool SPLIT, bool FULL>
+__device__ __forceinline__ void fr_generate(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y,
+                                            const double* p_c, const float* p_wr, const float* p_wi, int tid,
+                                            int i0, int j0, const MfmaArgs& a) {
+    constexpr int G = MFMA_THREADS / KC;   // row (column) phases per point
+    const int k = tid % KC, gph = tid / KC;
+    const double c = p_c[k];
+    const double sp = (double)G * a.p;
+    double td = 2.0 * c * sp * sp;
+    td -= rint(td);
+    const float2 E = make_float2(cos_turns((float)td), sin_turns((float)td));
+    // one run over rows (A, with the point weight) and one over columns (B)
+#pragma unroll
+    for (int ab = 0; ab < 2; ++ab) {
+        const int M = ab == 0 ? MR / G : MC / G;
+        const double d0 = ab == 0 ? (i0 + gph - a.ny / 2) * a.p - p_y[k] : (j0 + gph - a.nx / 2) * a.p - p_x[k];
+        double t0 = c * d0 * d0;
+        t0 -= rint(t0);
+        double t1 = c * (2.0 * d0 * sp + sp * sp);
+        t1 -= rint(t1);
+        float2 v = make_float2(cos_turns((float)t0), sin_turns((float)t0));
+        if (ab == 0) {
+            const float wr_ = p_wr[k], wi_ = p_wi[k];
+            v = make_float2(wr_ * v.x - wi_ * v.y, wr_ * v.y + wi_ * v.x);
+        }
+        float2 R = make_float2(cos_turns((float)t1), sin_turns((float)t1));
+        const int lim = ab == 0 ? a.ny - i0 : a.nx - j0;   // rows / columns inside the image
+#pragma unroll
+        for (int m = 0; m < MR / G; ++m) {
+            if (m >= M) break;
+            const int row = gph + G * m;
+            const bool ok = FULL || row < lim;
+            const float re = ok ? v.x : 0.f, im = ok ? v.y : 0.f;
+            const T rh = Mfma<T>::cvt(re), ih = Mfma<T>::cvt(im);
+            T* p0 = ab == 0 ? &L.A[0][row][k] : &L.B[0][row][k];
+            const int PS = (ab == 0 ? MR : MC) * FrLds<T, SPLIT>::KS;   // plane stride
+            p0[0] = rh;
+            p0[PS] = ih;
+            if constexpr (SPLIT) {
+                p0[2 * PS] = Mfma<T>::cvt(re - Mfma<T>::back(rh));
+                p0[3 * PS] = Mfma<T>::cvt(im - Mfma<T>::back(ih));
+            }
+            v = make_float2(fmaf(v.x, R.x, -v.y * R.y), fmaf(v.x, R.y, v.y * R.x));
+            R = make_float2(fmaf(R.x, E.x, -R.y * E.y), fmaf(R.x, E.y, R.y * E.x));
+        }
+    }
+}
+
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(MFMA_THREADS) void fresnel_mfma(MfmaArgs a) {
     typedef typename Mfma<T>::V V;
-    constexpr int NP = SPLIT ? 4 : 2;
-    constexpr int KCH = chunk_points<SPLIT>();
-    constexpr int KS = KCH + 8;   // LDS row stride (elements), 16-byte aligned rows
-    __shared__ __attribute__((aligned(16))) T As[NP][MT][KS];
-    __shared__ __attribute__((aligned(16))) T Bs[NP][MT][KS];
-    __shared__ double p_x[KCH], p_y[KCH], p_c[KCH];
-    __shared__ float p_wr[KCH], p_wi[KCH];
+    typedef FrLds<T, SPLIT> Lds;
+    constexpr int NP = Lds::NP;
+    extern __shared__ __attribute__((aligned(16))) unsigned char fr_dyn[];
+    Lds& L = *reinterpret_cast<Lds*>(fr_dyn);
+    __shared__ double p_x[KC], p_y[KC], p_c[KC];
+    __shared__ float p_wr[KC], p_wi[KC];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int wr = wave >> 1, wc = wave & 1;
     // XCD-aware order of the tiles: consecutive logical tiles on one XCD
-    const int ntx = (a.nx + MT - 1) / MT;
+    const int ntx = (a.nx + MC - 1) / MC;
     const int g = xcd_remap(blockIdx.x, gridDim.x);
-    const int i0 = (g / ntx) * MT, j0 = (g % ntx) * MT;
+    const int i0 = (g / ntx) * MR, j0 = (g % ntx) * MC;
+    const bool full = i0 + MR <= a.ny && j0 + MC <= a.nx;
 
     f32x16 cr[2][2], ci[2][2];
 #pragma unroll
@@ -254,10 +318,10 @@ __global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
     // chunk is generated and multiplied (the loop issues no other memory
     // operation, so the wait for them falls at the next chunk's start)
     FrPoint nq{};
-    if (tid < KCH) nq = a.pts[tid];
-    for (int64_t k0 = 0; k0 < a.npad; k0 += KCH) {
+    if (tid < KC) nq = a.pts[tid];
+    for (int64_t k0 = 0; k0 < a.npad; k0 += KC) {
         __syncthreads();
-        if (tid < KCH) {
+        if (tid < KC) {
             p_x[tid] = nq.x;
             p_y[tid] = nq.y;
             p_c[tid] = nq.c;
@@ -265,59 +329,12 @@ __global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
             p_wi[tid] = nq.wim;
         }
         __syncthreads();
-        if (tid < KCH && k0 + KCH < a.npad) nq = a.pts[k0 + KCH + tid];
-        // Chirp factors by a phase recurrence: thread (k, gph) generates
-        // point k's entries on rows (and columns) gph + G m, m < MT/G.  Along
-        // that run the phase c (Y - y)^2 has constant second difference
-        // delta = 2 c (G p)^2, so with R = exp(i 2 pi (phi_{m+1} - phi_m)),
-        // A_{m+1} = A_m R_m and R_{m+1} = R_m exp(i 2 pi delta): two complex
-        // multiplies per entry; the anchor phi_0, the first step and delta
-        // are reduced in FP64 once per run (phases of ~1e4 turns).
-        {
-            constexpr int G = 256 / KCH;     // row phases per point
-            constexpr int M = MT / G;        // entries per run
-            const int k = tid % KCH, gph = tid / KCH;
-            const double c = p_c[k];
-            const double sp = (double)G * a.p;
-            double td = 2.0 * c * sp * sp;
-            td -= rint(td);
-            const float2 E = make_float2(cos_turns((float)td), sin_turns((float)td));
-            // one run over rows (A, with the point weight) and one over columns (B)
-#pragma unroll
-            for (int ab = 0; ab < 2; ++ab) {
-                const double d0 = ab == 0 ? (i0 + gph - a.ny / 2) * a.p - p_y[k] : (j0 + gph - a.nx / 2) * a.p - p_x[k];
-                double t0 = c * d0 * d0;
-                t0 -= rint(t0);
-                double t1 = c * (2.0 * d0 * sp + sp * sp);
-                t1 -= rint(t1);
-                float2 v = make_float2(cos_turns((float)t0), sin_turns((float)t0));
-                if (ab == 0) {
-                    const float wr_ = p_wr[k], wi_ = p_wi[k];
-                    v = make_float2(wr_ * v.x - wi_ * v.y, wr_ * v.y + wi_ * v.x);
-                }
-                float2 R = make_float2(cos_turns((float)t1), sin_turns((float)t1));
-                const int lim = ab == 0 ? a.ny - i0 : a.nx - j0;   // rows / columns inside the image
-                T (*P)[MT][KS] = ab == 0 ? As : Bs;
-#pragma unroll
-                for (int m = 0; m < M; ++m) {
-                    const int row = gph + G * m;
-                    const bool ok = row < lim;
-                    const float re = ok ? v.x : 0.f, im = ok ? v.y : 0.f;
-                    const T rh = Mfma<T>::cvt(re), ih = Mfma<T>::cvt(im);
-                    P[0][row][k] = rh;
-                    P[1][row][k] = ih;
-                    if constexpr (SPLIT) {
-                        P[2][row][k] = Mfma<T>::cvt(re - Mfma<T>::back(rh));
-                        P[3][row][k] = Mfma<T>::cvt(im - Mfma<T>::back(ih));
-                    }
-                    v = make_float2(fmaf(v.x, R.x, -v.y * R.y), fmaf(v.x, R.y, v.y * R.x));
-                    R = make_float2(fmaf(R.x, E.x, -R.y * E.y), fmaf(R.x, E.y, R.y * E.x));
-                }
-            }
-        }
+        if (tid < KC && k0 + KC < a.npad) nq = a.pts[k0 + KC + tid];
+        if (full) fr_generate<T, SPLIT, true>(L, p_x, p_y, p_c, p_wr, p_wi, tid, i0, j0, a);
+        else fr_generate<T, SPLIT, false>(L, p_x, p_y, p_c, p_wr, p_wi, tid, i0, j0, a);
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < KCH / 16; ++s) {
+        for (int s = 0; s < KC / 16; ++s) {
             const int kb = 16 * s + 8 * (lane >> 5);
             V A[2][NP], B[2][NP];
 #pragma unroll
@@ -326,8 +343,8 @@ __global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
                 const int col = wc * 64 + 32 * x + (lane & 31);
 #pragma unroll
                 for (int q = 0; q < NP; ++q) {
-                    A[x][q] = *reinterpret_cast<const V*>(&As[q][row][kb]);
-                    B[x][q] = *reinterpret_cast<const V*>(&Bs[q][col][kb]);
+                    A[x][q] = *reinterpret_cast<const V*>(&L.A[q][row][kb]);
+                    B[x][q] = *reinterpret_cast<const V*>(&L.B[q][col][kb]);
                 }
             }
 #pragma unroll
@@ -367,6 +384,20 @@ __global__ __launch_bounds__(256) void fresnel_mfma(MfmaArgs a) {
                 const int col = j0 + wc * 64 + 32 * y + (lane & 31);
                 if (row < a.ny && col < a.nx) a.out[(size_t)row * a.nx + col] = make_float2(cr[x][y][v], ci[x][y][v]);
             }
+}
+
+template <typename T, bool SPLIT>
+int launch_fresnel_mfma(oph_ctx* ctx, const MfmaArgs& m, unsigned tiles) {
+    const size_t shm = sizeof(FrLds<T, SPLIT>);
+    static const bool attr = [&] {
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&fresnel_mfma<T, SPLIT>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipGetLastError();
+        return true;
+    }();
+    (void)attr;
+    OPH_LAUNCH(ctx, "fresnel_mfma", (fresnel_mfma<T, SPLIT>), dim3(tiles), dim3(MFMA_THREADS), shm, m);
+    return OPH_OK;
 }
 
 }  // namespace
@@ -433,21 +464,14 @@ extern "C" int oph_pc_fresnel(oph_ctx* ctx, int nx, int ny, double pitch, double
         m.ny = ny;
         m.p = pitch;
         m.out = o;
-        const unsigned tiles = (unsigned)(((nx + MT - 1) / MT) * ((ny + MT - 1) / MT));
+        const unsigned tiles = (unsigned)(((nx + MC - 1) / MC) * ((ny + MR - 1) / MR));
         switch (method) {
-            case OPH_FRESNEL_MFMA_BF16:
-                OPH_LAUNCH(ctx, "fresnel_mfma", (fresnel_mfma<__bf16, false>), dim3(tiles), dim3(256), 0, m);
-                break;
-            case OPH_FRESNEL_MFMA_BF16_SPLIT:
-                OPH_LAUNCH(ctx, "fresnel_mfma", (fresnel_mfma<__bf16, true>), dim3(tiles), dim3(256), 0, m);
-                break;
-            case OPH_FRESNEL_MFMA_FP16:
-                OPH_LAUNCH(ctx, "fresnel_mfma", (fresnel_mfma<_Float16, false>), dim3(tiles), dim3(256), 0, m);
-                break;
-            default:
-                OPH_LAUNCH(ctx, "fresnel_mfma", (fresnel_mfma<_Float16, true>), dim3(tiles), dim3(256), 0, m);
-                break;
+            case OPH_FRESNEL_MFMA_BF16: rc = launch_fresnel_mfma<__bf16, false>(ctx, m, tiles); break;
+            case OPH_FRESNEL_MFMA_BF16_SPLIT: rc = launch_fresnel_mfma<__bf16, true>(ctx, m, tiles); break;
+            case OPH_FRESNEL_MFMA_FP16: rc = launch_fresnel_mfma<_Float16, false>(ctx, m, tiles); break;
+            default: rc = launch_fresnel_mfma<_Float16, true>(ctx, m, tiles); break;
         }
+        if (rc) return rc;
     }
     if (!(flags & OPH_DEVICE_PTR))
         OPH_HIP_CHECK(ctx, hipMemcpyAsync(out, o, npix * sizeof(float2), hipMemcpyDeviceToHost, ctx->stream));
