@@ -181,7 +181,11 @@ __global__ void fresnel_sum_chunks(const float2* partial, int chunks, int64_t np
 
 constexpr int MR = 256;         // output tile rows
 constexpr int MC = 128;         // output tile columns
-constexpr int KC = 32;          // points per chunk (and padding granularity)
+constexpr int KC = 32;          // padding granularity: the largest chunk
+// points per chunk: 32 for the single-precision forms; 16 for the hi+lo
+// forms, whose four operand planes in two buffers would not fit LDS at 32
+template <bool SPLIT>
+constexpr int KCH = SPLIT ? 16 : 32;
 constexpr int MFMA_THREADS = 512;
 
 template <typename T>
@@ -225,76 +229,145 @@ struct MfmaArgs {
 template <typename T, bool SPLIT>
 struct FrLds {
     static constexpr int NP = SPLIT ? 4 : 2;
-    static constexpr int KS = KC + 8;   // row stride (elements): 16-byte aligned rows
+    static constexpr int KS = KCH<SPLIT> + 8;   // row stride (elements): 16-byte aligned rows
     T A[NP][MR][KS];
     T B[NP][MC][KS];
 };
 
-// Chirp factors by a phase recurrence: thread (k, gph) generates point k's
-// entries on rows gph + G m (m < MR/G) and columns gph + G m (m < MC/G).
-// Along a run the phase c (Y - y)^2 has constant second difference
+// Chirp factors by a phase recurrence.  A run is one point's entries on
+// rows (A, with the point weight) or columns (B) gph + G m, G = 16.  Along a
+// run the phase c (Y - y)^2 has constant second difference
 // delta = 2 c (G p)^2, so with R = exp(i 2 pi (phi_{m+1} - phi_m)),
 // A_{m+1} = A_m R_m and R_{m+1} = R_m exp(i 2 pi delta): two complex
 // multiplies per entry; the anchor phi_0, the first step and delta are
-// reduced in FP64 once per run (phases of ~1e4 turns).  FULL: every row and
-// column of the tile is inside the image (no per-entry bounds select).
+// reduced in FP64 once per run (phases of ~1e4 turns).  With 32 points a
+// chunk, thread (k, gph) takes point k's A run (16 entries) and B run (8);
+// with 16, the first four waves take the A runs and the last four the B
+// runs, one run (one FP64 set-up) per thread -- a SIMD holds one wave of
+// each, so its share is the same.  FULL: every row and column of the tile
+// is inside the image (no per-entry bounds select).
+constexpr int GPH = 16;   // run stride (rows or columns)
+
+template <typename T, bool SPLIT, bool FULL, int AB>
+__device__ __forceinline__ void fr_run(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y, const double* p_c,
+                                       const float* p_wr, const float* p_wi, int k, int gph, int i0, int j0,
+                                       const MfmaArgs& a) {
+    constexpr int M = (AB == 0 ? MR : MC) / GPH;
+    constexpr int PS = (AB == 0 ? MR : MC) * FrLds<T, SPLIT>::KS;   // plane stride
+    const double c = p_c[k];
+    const double sp = (double)GPH * a.p;
+    double td = 2.0 * c * sp * sp;
+    td -= rint(td);
+    const float2 E = make_float2(cos_turns((float)td), sin_turns((float)td));
+    const double d0 = AB == 0 ? (i0 + gph - a.ny / 2) * a.p - p_y[k] : (j0 + gph - a.nx / 2) * a.p - p_x[k];
+    double t0 = c * d0 * d0;
+    t0 -= rint(t0);
+    double t1 = c * (2.0 * d0 * sp + sp * sp);
+    t1 -= rint(t1);
+    float2 v = make_float2(cos_turns((float)t0), sin_turns((float)t0));
+    if constexpr (AB == 0) {
+        const float wr_ = p_wr[k], wi_ = p_wi[k];
+        v = make_float2(wr_ * v.x - wi_ * v.y, wr_ * v.y + wi_ * v.x);
+    }
+    float2 R = make_float2(cos_turns((float)t1), sin_turns((float)t1));
+    const int lim = AB == 0 ? a.ny - i0 : a.nx - j0;   // rows / columns inside the image
+    T* p0 = AB == 0 ? &L.A[0][gph][k] : &L.B[0][gph][k];
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const bool ok = FULL || gph + GPH * m < lim;
+        const float re = ok ? v.x : 0.f, im = ok ? v.y : 0.f;
+        const T rh = Mfma<T>::cvt(re), ih = Mfma<T>::cvt(im);
+        T* q = p0 + m * GPH * FrLds<T, SPLIT>::KS;
+        q[0] = rh;
+        q[PS] = ih;
+        if constexpr (SPLIT) {
+            q[2 * PS] = Mfma<T>::cvt(re - Mfma<T>::back(rh));
+            q[3 * PS] = Mfma<T>::cvt(im - Mfma<T>::back(ih));
+        }
+        v = make_float2(fmaf(v.x, R.x, -v.y * R.y), fmaf(v.x, R.y, v.y * R.x));
+        R = make_float2(fmaf(R.x, E.x, -R.y * E.y), fmaf(R.x, E.y, R.y * E.x));
+    }
+}
+
 template <typename T, bool SPLIT, bool FULL>
 __device__ __forceinline__ void fr_generate(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y,
                                             const double* p_c, const float* p_wr, const float* p_wi, int tid,
                                             int i0, int j0, const MfmaArgs& a) {
-    constexpr int G = MFMA_THREADS / KC;   // row (column) phases per point
-    const int k = tid % KC, gph = tid / KC;
-    const double c = p_c[k];
-    const double sp = (double)G * a.p;
-    double td = 2.0 * c * sp * sp;
-    td -= rint(td);
-    const float2 E = make_float2(cos_turns((float)td), sin_turns((float)td));
-    // one run over rows (A, with the point weight) and one over columns (B)
+    if constexpr (!SPLIT) {
+        static_assert(KCH<SPLIT> * GPH == MFMA_THREADS, "one thread per (point, phase)");
+        const int k = tid % KCH<SPLIT>, gph = tid / KCH<SPLIT>;
+        fr_run<T, SPLIT, FULL, 0>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
+        fr_run<T, SPLIT, FULL, 1>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
+    } else {
+        static_assert(2 * KCH<SPLIT> * GPH == MFMA_THREADS, "one thread per (point, phase, factor)");
+        const int h = tid % (MFMA_THREADS / 2);
+        const int k = h % KCH<SPLIT>, gph = h / KCH<SPLIT>;
+        if (tid < MFMA_THREADS / 2)
+            fr_run<T, SPLIT, FULL, 0>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
+        else
+            fr_run<T, SPLIT, FULL, 1>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
+    }
+}
+
+// Contraction of one chunk from L by wave (wr, wc): its 64 x 64 sub-tile.
+template <typename T, bool SPLIT>
+__device__ __forceinline__ void fr_contract(const FrLds<T, SPLIT>& L, int wr, int wc, int lane, f32x16 (&cr)[2][2],
+                                            f32x16 (&ci)[2][2]) {
+    typedef typename Mfma<T>::V V;
+    constexpr int NP = FrLds<T, SPLIT>::NP;
 #pragma unroll
-    for (int ab = 0; ab < 2; ++ab) {
-        const int M = ab == 0 ? MR / G : MC / G;
-        const double d0 = ab == 0 ? (i0 + gph - a.ny / 2) * a.p - p_y[k] : (j0 + gph - a.nx / 2) * a.p - p_x[k];
-        double t0 = c * d0 * d0;
-        t0 -= rint(t0);
-        double t1 = c * (2.0 * d0 * sp + sp * sp);
-        t1 -= rint(t1);
-        float2 v = make_float2(cos_turns((float)t0), sin_turns((float)t0));
-        if (ab == 0) {
-            const float wr_ = p_wr[k], wi_ = p_wi[k];
-            v = make_float2(wr_ * v.x - wi_ * v.y, wr_ * v.y + wi_ * v.x);
-        }
-        float2 R = make_float2(cos_turns((float)t1), sin_turns((float)t1));
-        const int lim = ab == 0 ? a.ny - i0 : a.nx - j0;   // rows / columns inside the image
+    for (int s = 0; s < KCH<SPLIT> / 16; ++s) {
+        const int kb = 16 * s + 8 * (lane >> 5);
+        V A[2][NP], B[2][NP];
 #pragma unroll
-        for (int m = 0; m < MR / G; ++m) {
-            if (m >= M) break;
-            const int row = gph + G * m;
-            const bool ok = FULL || row < lim;
-            const float re = ok ? v.x : 0.f, im = ok ? v.y : 0.f;
-            const T rh = Mfma<T>::cvt(re), ih = Mfma<T>::cvt(im);
-            T* p0 = ab == 0 ? &L.A[0][row][k] : &L.B[0][row][k];
-            const int PS = (ab == 0 ? MR : MC) * FrLds<T, SPLIT>::KS;   // plane stride
-            p0[0] = rh;
-            p0[PS] = ih;
-            if constexpr (SPLIT) {
-                p0[2 * PS] = Mfma<T>::cvt(re - Mfma<T>::back(rh));
-                p0[3 * PS] = Mfma<T>::cvt(im - Mfma<T>::back(ih));
+        for (int x = 0; x < 2; ++x) {
+            const int row = wr * 64 + 32 * x + (lane & 31);
+            const int col = wc * 64 + 32 * x + (lane & 31);
+#pragma unroll
+            for (int q = 0; q < NP; ++q) {
+                A[x][q] = *reinterpret_cast<const V*>(&L.A[q][row][kb]);
+                B[x][q] = *reinterpret_cast<const V*>(&L.B[q][col][kb]);
             }
-            v = make_float2(fmaf(v.x, R.x, -v.y * R.y), fmaf(v.x, R.y, v.y * R.x));
-            R = make_float2(fmaf(R.x, E.x, -R.y * E.y), fmaf(R.x, E.y, R.y * E.x));
+        }
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            const V nbi = negate(B[y][1]);
+            const V nbil = SPLIT ? negate(B[y][NP - 1]) : nbi;
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+                // re: Ar Br - Ai Bi ; im: Ar Bi + Ai Br  (hi x hi)
+                cr[x][y] = Mfma<T>::mac(A[x][0], B[y][0], cr[x][y]);
+                cr[x][y] = Mfma<T>::mac(A[x][1], nbi, cr[x][y]);
+                ci[x][y] = Mfma<T>::mac(A[x][0], B[y][1], ci[x][y]);
+                ci[x][y] = Mfma<T>::mac(A[x][1], B[y][0], ci[x][y]);
+                if constexpr (SPLIT) {
+                    // hi x lo and lo x hi
+                    cr[x][y] = Mfma<T>::mac(A[x][0], B[y][2], cr[x][y]);
+                    cr[x][y] = Mfma<T>::mac(A[x][1], nbil, cr[x][y]);
+                    cr[x][y] = Mfma<T>::mac(A[x][2], B[y][0], cr[x][y]);
+                    cr[x][y] = Mfma<T>::mac(A[x][3], nbi, cr[x][y]);
+                    ci[x][y] = Mfma<T>::mac(A[x][0], B[y][3], ci[x][y]);
+                    ci[x][y] = Mfma<T>::mac(A[x][1], B[y][2], ci[x][y]);
+                    ci[x][y] = Mfma<T>::mac(A[x][2], B[y][1], ci[x][y]);
+                    ci[x][y] = Mfma<T>::mac(A[x][3], B[y][0], ci[x][y]);
+                }
+            }
         }
     }
 }
 
+// The factors of chunk c + 1 are generated into one LDS buffer while chunk c
+// is contracted from the other: one barrier per chunk.  The points reach
+// LDS two chunks ahead (double-buffered too) and are loaded into registers
+// three chunks ahead.
 template <typename T, bool SPLIT>
 __global__ __launch_bounds__(MFMA_THREADS) void fresnel_mfma(MfmaArgs a) {
-    typedef typename Mfma<T>::V V;
     typedef FrLds<T, SPLIT> Lds;
-    constexpr int NP = Lds::NP;
+    constexpr int K = KCH<SPLIT>;
     extern __shared__ __attribute__((aligned(16))) unsigned char fr_dyn[];
-    Lds& L = *reinterpret_cast<Lds*>(fr_dyn);
-    __shared__ double p_x[KC], p_y[KC], p_c[KC];
-    __shared__ float p_wr[KC], p_wi[KC];
+    Lds* L = reinterpret_cast<Lds*>(fr_dyn);   // two buffers
+    __shared__ double p_x[2][K], p_y[2][K], p_c[2][K];
+    __shared__ float p_wr[2][K], p_wi[2][K];
 
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
@@ -304,6 +377,7 @@ __global__ __launch_bounds__(MFMA_THREADS) void fresnel_mfma(MfmaArgs a) {
     const int g = xcd_remap(blockIdx.x, gridDim.x);
     const int i0 = (g / ntx) * MR, j0 = (g % ntx) * MC;
     const bool full = i0 + MR <= a.ny && j0 + MC <= a.nx;
+    const int64_t nch = a.npad / K;
 
     f32x16 cr[2][2], ci[2][2];
 #pragma unroll
@@ -313,65 +387,40 @@ __global__ __launch_bounds__(MFMA_THREADS) void fresnel_mfma(MfmaArgs a) {
             cr[x][y] = (f32x16)0.f;
             ci[x][y] = (f32x16)0.f;
         }
-
-    // the next chunk's points are loaded into registers while the current
-    // chunk is generated and multiplied (the loop issues no other memory
-    // operation, so the wait for them falls at the next chunk's start)
+    auto stage_points = [&](int slot, const FrPoint& q) {
+        p_x[slot][tid] = q.x;
+        p_y[slot][tid] = q.y;
+        p_c[slot][tid] = q.c;
+        p_wr[slot][tid] = q.wre;
+        p_wi[slot][tid] = q.wim;
+    };
+    auto generate = [&](int64_t c) {
+        const int slot = (int)(c & 1);
+        if (full)
+            fr_generate<T, SPLIT, true>(L[slot], p_x[slot], p_y[slot], p_c[slot], p_wr[slot], p_wi[slot], tid, i0,
+                                        j0, a);
+        else
+            fr_generate<T, SPLIT, false>(L[slot], p_x[slot], p_y[slot], p_c[slot], p_wr[slot], p_wi[slot], tid,
+                                         i0, j0, a);
+    };
     FrPoint nq{};
-    if (tid < KC) nq = a.pts[tid];
-    for (int64_t k0 = 0; k0 < a.npad; k0 += KC) {
-        __syncthreads();
-        if (tid < KC) {
-            p_x[tid] = nq.x;
-            p_y[tid] = nq.y;
-            p_c[tid] = nq.c;
-            p_wr[tid] = nq.wre;
-            p_wi[tid] = nq.wim;
+    if (tid < K) {
+        if (nch > 0) stage_points(0, a.pts[tid]);
+        if (nch > 1) stage_points(1, a.pts[K + tid]);
+        if (nch > 2) nq = a.pts[2 * K + tid];
+    }
+    __syncthreads();
+    if (nch > 0) generate(0);
+    __syncthreads();
+    for (int64_t c = 0; c < nch; ++c) {
+        if (c + 1 < nch) generate(c + 1);
+        fr_contract<T, SPLIT>(L[c & 1], wr, wc, lane, cr, ci);
+        // slot c & 1 held chunk c's points, read by generate(c) one step ago
+        if (tid < K && c + 2 < nch) {
+            stage_points((int)(c & 1), nq);
+            if (c + 3 < nch) nq = a.pts[(c + 3) * K + tid];
         }
         __syncthreads();
-        if (tid < KC && k0 + KC < a.npad) nq = a.pts[k0 + KC + tid];
-        if (full) fr_generate<T, SPLIT, true>(L, p_x, p_y, p_c, p_wr, p_wi, tid, i0, j0, a);
-        else fr_generate<T, SPLIT, false>(L, p_x, p_y, p_c, p_wr, p_wi, tid, i0, j0, a);
-        __syncthreads();
-#pragma unroll
-        for (int s = 0; s < KC / 16; ++s) {
-            const int kb = 16 * s + 8 * (lane >> 5);
-            V A[2][NP], B[2][NP];
-#pragma unroll
-            for (int x = 0; x < 2; ++x) {
-                const int row = wr * 64 + 32 * x + (lane & 31);
-                const int col = wc * 64 + 32 * x + (lane & 31);
-#pragma unroll
-                for (int q = 0; q < NP; ++q) {
-                    A[x][q] = *reinterpret_cast<const V*>(&L.A[q][row][kb]);
-                    B[x][q] = *reinterpret_cast<const V*>(&L.B[q][col][kb]);
-                }
-            }
-#pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                const V nbi = negate(B[y][1]);
-                const V nbil = SPLIT ? negate(B[y][NP - 1]) : nbi;
-#pragma unroll
-                for (int x = 0; x < 2; ++x) {
-                    // re: Ar Br - Ai Bi ; im: Ar Bi + Ai Br  (hi x hi)
-                    cr[x][y] = Mfma<T>::mac(A[x][0], B[y][0], cr[x][y]);
-                    cr[x][y] = Mfma<T>::mac(A[x][1], nbi, cr[x][y]);
-                    ci[x][y] = Mfma<T>::mac(A[x][0], B[y][1], ci[x][y]);
-                    ci[x][y] = Mfma<T>::mac(A[x][1], B[y][0], ci[x][y]);
-                    if constexpr (SPLIT) {
-                        // hi x lo and lo x hi
-                        cr[x][y] = Mfma<T>::mac(A[x][0], B[y][2], cr[x][y]);
-                        cr[x][y] = Mfma<T>::mac(A[x][1], nbil, cr[x][y]);
-                        cr[x][y] = Mfma<T>::mac(A[x][2], B[y][0], cr[x][y]);
-                        cr[x][y] = Mfma<T>::mac(A[x][3], nbi, cr[x][y]);
-                        ci[x][y] = Mfma<T>::mac(A[x][0], B[y][3], ci[x][y]);
-                        ci[x][y] = Mfma<T>::mac(A[x][1], B[y][2], ci[x][y]);
-                        ci[x][y] = Mfma<T>::mac(A[x][2], B[y][1], ci[x][y]);
-                        ci[x][y] = Mfma<T>::mac(A[x][3], B[y][0], ci[x][y]);
-                    }
-                }
-            }
-        }
     }
     // epilogue
 #pragma unroll
@@ -388,7 +437,7 @@ __global__ __launch_bounds__(MFMA_THREADS) void fresnel_mfma(MfmaArgs a) {
 
 template <typename T, bool SPLIT>
 int launch_fresnel_mfma(oph_ctx* ctx, const MfmaArgs& m, unsigned tiles) {
-    const size_t shm = sizeof(FrLds<T, SPLIT>);
+    const size_t shm = 2 * sizeof(FrLds<T, SPLIT>);
     static const bool attr = [&] {
         hipFuncSetAttribute(reinterpret_cast<const void*>(&fresnel_mfma<T, SPLIT>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
