@@ -22,6 +22,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 namespace {
 
@@ -195,7 +198,19 @@ template <>
 struct Mfma<__bf16> {
     typedef bf16x8 V;
     static __device__ __forceinline__ __bf16 cvt(float f) { return (__bf16)f; }
+    // both parts in one packed conversion (round to nearest even)
+    static __device__ __forceinline__ void cvt2(float re, float im, __bf16& rh, __bf16& ih) {
+        const f32x2 v = {re, im};
+        const bf16x2 h = __builtin_convertvector(v, bf16x2);
+        rh = h.x;
+        ih = h.y;
+    }
     static __device__ __forceinline__ float back(__bf16 h) { return (float)h; }
+    // the low parts (re - rh, im - ih, exact in FP32) in one packed conversion
+    static __device__ __forceinline__ void lo2(float re, float im, __bf16 rh, __bf16 ih, float, __bf16& rl,
+                                               __bf16& il) {
+        cvt2(re - (float)rh, im - (float)ih, rl, il);
+    }
     static __device__ __forceinline__ f32x16 mac(V a, V b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
     }
@@ -205,7 +220,24 @@ template <>
 struct Mfma<_Float16> {
     typedef f16x8 V;
     static __device__ __forceinline__ _Float16 cvt(float f) { return (_Float16)f; }
+    static __device__ __forceinline__ void cvt2(float re, float im, _Float16& rh, _Float16& ih) {
+        const f32x2 v = {re, im};
+        const f16x2 h = __builtin_convertvector(v, f16x2);
+        rh = h.x;
+        ih = h.y;
+    }
     static __device__ __forceinline__ float back(_Float16 h) { return (float)h; }
+    // f - h rounded to FP16 (exact in FP32 before the rounding) as
+    // f + h m1 with m1 = -1 from a register (a literal -1 would fold the FMA
+    // into a subtraction): one mixed-precision FMA, FP16 operand widened
+    static __device__ __forceinline__ _Float16 lo(float f, _Float16 h, float m1) {
+        return (_Float16)fmaf((float)h, m1, f);
+    }
+    static __device__ __forceinline__ void lo2(float re, float im, _Float16 rh, _Float16 ih, float m1, _Float16& rl,
+                                               _Float16& il) {
+        rl = lo(re, rh, m1);
+        il = lo(im, ih, m1);
+    }
     static __device__ __forceinline__ f32x16 mac(V a, V b, f32x16 c) {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
     }
@@ -272,17 +304,22 @@ __device__ __forceinline__ void fr_run(FrLds<T, SPLIT>& L, const double* p_x, co
     float2 R = make_float2(cos_turns((float)t1), sin_turns((float)t1));
     const int lim = AB == 0 ? a.ny - i0 : a.nx - j0;   // rows / columns inside the image
     T* p0 = AB == 0 ? &L.A[0][gph][k] : &L.B[0][gph][k];
+    float m1 = -1.f;
+    if constexpr (SPLIT) asm volatile("v_mov_b32 %0, -1.0" : "=v"(m1));
 #pragma unroll
     for (int m = 0; m < M; ++m) {
         const bool ok = FULL || gph + GPH * m < lim;
         const float re = ok ? v.x : 0.f, im = ok ? v.y : 0.f;
-        const T rh = Mfma<T>::cvt(re), ih = Mfma<T>::cvt(im);
+        T rh, ih;
+        Mfma<T>::cvt2(re, im, rh, ih);
         T* q = p0 + m * GPH * FrLds<T, SPLIT>::KS;
         q[0] = rh;
         q[PS] = ih;
         if constexpr (SPLIT) {
-            q[2 * PS] = Mfma<T>::cvt(re - Mfma<T>::back(rh));
-            q[3 * PS] = Mfma<T>::cvt(im - Mfma<T>::back(ih));
+            T rl, il;
+            Mfma<T>::lo2(re, im, rh, ih, m1, rl, il);
+            q[2 * PS] = rl;
+            q[3 * PS] = il;
         }
         v = make_float2(fmaf(v.x, R.x, -v.y * R.y), fmaf(v.x, R.y, v.y * R.x));
         R = make_float2(fmaf(R.x, E.x, -R.y * E.y), fmaf(R.x, E.y, R.y * E.x));
