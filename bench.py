@@ -29,7 +29,7 @@ VALU_LANE_OPS = 256 * 4 * 16 * 2.4e9      # VALU issue peak, unpacked FP32 lane-
 # SQ_INSTS_VALU ... -- python3 bench.py --workload <rs|wrp> --only --steps 2
 # --warmup 1 --no-cpu` commands recorded there); valid for the kernels as
 # built at that time: re-measure after changing them.
-PROFILED_VALU_INSTS = {"rs_accumulate": 181600256, "asm_cols": 24274944, "wrp_gather": 23896245}
+PROFILED_VALU_INSTS = {"rs_accumulate": 181600256, "asm_cols": 25980928, "wrp_gather": 23896245}
 PROFILE_SOURCES = {"valu": "profiles/valu_counters_rs_wrp.md (rocprofv3 --pmc SQ_INSTS_VALU)",
                    "traffic": "profiles/hbm_traffic_wrp_depthmap.md (rocprofv3 --pmc FETCH_SIZE, then "
                               "WRITE_SIZE, calibrated on copy_kernel)"}
@@ -163,7 +163,7 @@ WRP_KERNELS = ["wrp_bin_count", "wrp_bin_colscan", "wrp_bin_fill", "wrp_gather",
 # FETCH_SIZE x 2.000, WRITE_SIZE x 1.000); summary in
 # profiles/hbm_traffic_wrp_depthmap.md.  Valid for the kernels as built here;
 # re-measure after changing them.
-PROFILED_TRAFFIC = {"asm_cols": (65.7 + 66.4) * 2**20, "asm_rows_fwd": (32.4 + 64.2) * 2**20,
+PROFILED_TRAFFIC = {"asm_cols": (65.2 + 69.1) * 2**20, "asm_rows_fwd": (32.4 + 64.2) * 2**20,
                     "asm_rows_inv": (64.2 + 32.0) * 2**20}
 
 
@@ -543,7 +543,7 @@ def bench_fft2d(ctx, args):
     # HBM bytes per launch from the FETCH_SIZE / WRITE_SIZE runs of
     # `bench.py --workload fft2d --only --steps 2 --warmup 1 --no-cpu` (same
     # calibration as PROFILED_TRAFFIC)
-    traffic = {"fft_rows": (128.2 + 128.0) * 2**20, "fft_cols": (136.1 + 149.7) * 2**20}
+    traffic = {"fft_rows": (128.2 + 128.0) * 2**20, "fft_cols": (132.8 + 179.1) * 2**20}
     roof = {}
     for k in ("fft_rows", "fft_cols"):
         if k in per:
