@@ -250,6 +250,11 @@ def cpu_wrp(x, y, z, a, zw, n_side, pitch, lam, dist, pad):
 
 
 DM_KERNELS = ["dm_index", "dm_zero", "dm_cols", "dm_rows", "dm_final"]
+# The depth-map row pass (`dm_rows_h<4096, true>`, one launch per colour of
+# the RGB workload): wave-level VALU instructions (SQ_INSTS_VALU) and HBM
+# bytes (FETCH_SIZE + WRITE_SIZE, calibrated as PROFILED_TRAFFIC) per launch,
+# from the `--workload depthmap --only --steps 2 --warmup 1 --no-cpu` runs.
+PROFILED_DM_ROWS = {"valu_insts": 1.022e9, "traffic": (1905.2 + 15.0) * 2**20}
 
 
 def bench_depthmap(ctx, args):
@@ -280,6 +285,17 @@ def bench_depthmap(ctx, args):
     if "dm_rows" in per:
         t = (per["dm_rows"]["avg_us"] + per["dm_cols"]["avg_us"]) * 1e-6
         line["hbm_gbs_layer_passes"] = 2 * inter / t / 1e9
+        # the dominant kernel, the row pass (one launch per colour): VALU-bound;
+        # its counted VALU instructions and HBM bytes per launch are constants
+        # of the committed PMC runs
+        tr = per["dm_rows"]["avg_us"] * 1e-6
+        ops = PROFILED_DM_ROWS["valu_insts"] * 64
+        line["roofline"] = {"kernel": "dm_rows", "bound": "valu", "achieved": ops / tr / 1e12,
+                            "peak": VALU_LANE_OPS / 1e12, "unit": "Tlane-ops/s", "frac": ops / tr / VALU_LANE_OPS,
+                            "traffic": PROFILED_DM_ROWS["traffic"], "algorithmic_bytes": inter + (Ny // 2 + 1) * w * 8,
+                            "hbm_frac": PROFILED_DM_ROWS["traffic"] / tr / (HBM_PEAK_GBS * 1e9),
+                            "counted_from": "profiles/sq_counters_fresnel_depthmap.md (SQ_INSTS_VALU) and "
+                                            "profiles/hbm_traffic_wrp_depthmap.md (FETCH_SIZE, WRITE_SIZE)"}
     if not args.no_cpu:
         from oracle import oracle as O
         l0 = int(np.argmax(np.bincount(idx.ravel(), minlength=257)))
