@@ -389,7 +389,9 @@ def bench_fresnel(ctx, args):
         a2.steps, a2.warmup = 3, 1
         wall, per = _time_steps(ctx, step, a2, ["fresnel_mfma", "fresnel_prep"])
         mult = 3 if m.endswith("split") else 1
-        kt = per["fresnel_mfma"]["avg_us"] * 1e-6
+        # the MFMA kernel time per hologram (the form not chosen also launches
+        # and returns at once)
+        kt = per["fresnel_mfma"]["avg_us"] * per["fresnel_mfma"]["launches_per_step"] * 1e-6
         res[m] = {"holograms_per_s": 3 / wall, "ms": wall / 3 * 1e3,
                   "mfma_tflops": flops * mult / kt / 1e12, "mfma_frac": flops * mult / kt / 2.5e15}
         if ref is not None:

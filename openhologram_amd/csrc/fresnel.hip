@@ -14,7 +14,11 @@
 // factors are rounded to BF16 or FP16, optionally split as hi + lo (the lo
 // part carries the rounding error of hi), and contracted on MFMA with FP32
 // accumulation (A B = Ah Bh + Ah Bl + Al Bh).
+#include <algorithm>
 #include <cmath>
+
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "oph_internal.h"
 
@@ -34,30 +38,41 @@ struct FrPoint {
     float f0, a;      // frac(z/lambda + phi/2pi), amplitude
 };
 
-__global__ void fresnel_prep(const double* x, const double* y, const double* z, const double* amp,
-                             const double* phase, int64_t n, int64_t npad, double lambda, FrPoint* out) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= npad) return;
+__device__ FrPoint make_frpoint(const double* x, const double* y, const double* z, const double* amp,
+                                const double* phase, int64_t i, double lambda) {
     FrPoint q;
-    if (i < n) {
-        q.x = x[i];
-        q.y = y[i];
-        q.c = 1.0 / (2.0 * lambda * z[i]);
-        double t = z[i] / lambda + (phase ? phase[i] * 0.15915494309189535 : 0.0);
-        t -= rint(t);
-        const float a = amp ? (float)amp[i] : 1.f;
-        q.f0 = (float)t;
-        q.a = a;
-        q.wre = a * cos_turns((float)t);
-        q.wim = a * sin_turns((float)t);
-    } else {  // padding up to the contraction chunk: zero weight
-        q.x = q.y = 0.0;
-        q.c = 0.0;
-        q.f0 = 0.f;
-        q.a = 0.f;
-        q.wre = q.wim = 0.f;
-    }
-    out[i] = q;
+    q.x = x[i];
+    q.y = y[i];
+    q.c = 1.0 / (2.0 * lambda * z[i]);
+    double t = z[i] / lambda + (phase ? phase[i] * 0.15915494309189535 : 0.0);
+    t -= rint(t);
+    const float a = amp ? (float)amp[i] : 1.f;
+    q.f0 = (float)t;
+    q.a = a;
+    q.wre = a * cos_turns((float)t);
+    q.wim = a * sin_turns((float)t);
+    return q;
+}
+
+// a zero-weight point (padding up to whole contraction chunks) on plane c
+__device__ FrPoint zero_frpoint(double c) {
+    FrPoint q;
+    q.x = q.y = 0.0;
+    q.c = c;
+    q.f0 = 0.f;
+    q.a = 0.f;
+    q.wre = q.wim = 0.f;
+    return q;
+}
+
+// skip: when set and *skip != 0 the plane-factored form computes the field
+// and this general preparation (and contraction) does nothing
+__global__ void fresnel_prep(const double* x, const double* y, const double* z, const double* amp,
+                             const double* phase, int64_t n, int64_t npad, double lambda, FrPoint* out,
+                             const int* skip) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad || (skip && *skip)) return;
+    out[i] = i < n ? make_frpoint(x, y, z, amp, phase, i, lambda) : zero_frpoint(0.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -243,6 +258,12 @@ struct Mfma<_Float16> {
     }
 };
 
+// two 16-bit values as one word, the first in the low half
+template <typename T>
+__device__ __forceinline__ unsigned pack2(T lo, T hi) {
+    return (unsigned)__builtin_bit_cast(unsigned short, lo) | ((unsigned)__builtin_bit_cast(unsigned short, hi) << 16);
+}
+
 template <typename V>
 __device__ __forceinline__ V negate(V v) {
     u32x4 u = __builtin_bit_cast(u32x4, v);
@@ -256,6 +277,7 @@ struct MfmaArgs {
     int nx, ny;
     double p;
     float2* out;
+    const int* skip;  // when set and nonzero, the plane-factored form has the field
 };
 
 template <typename T, bool SPLIT>
@@ -406,6 +428,7 @@ __global__ __launch_bounds__(MFMA_THREADS) void fresnel_mfma(MfmaArgs a) {
     __shared__ double p_x[2][K], p_y[2][K], p_c[2][K];
     __shared__ float p_wr[2][K], p_wi[2][K];
 
+    if (a.skip && *a.skip) return;
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int wr = wave >> 1, wc = wave & 1;
@@ -486,6 +509,500 @@ int launch_fresnel_mfma(oph_ctx* ctx, const MfmaArgs& m, unsigned tiles) {
     return OPH_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Plane-factored contraction.  Points on one plane z_p share
+// c_p = 1/(2 lambda z_p), and
+//   A[i][n] = D_p(Y_i) w_n exp(i 2 pi c_p (y_n^2 - 2 Y_i y_n)),  D_p(Y) = exp(i 2 pi c_p Y^2),
+// B likewise with X_j and x_n (no weight).  D_p is common to the plane's
+// points, so the plane's part of the field is
+//   U_p[i][j] = D_p(Y_i) D_p(X_j) sum_{n on p} g_n[i] h_n[j],
+// and g, h are geometric along a run of rows (ratio exp(-i 2 pi 2 c_p y_n G p)):
+// one complex multiply per entry instead of two.  The points are grouped by
+// plane (a stable sort by z, so a plane's points keep their input order; each
+// plane padded to whole 32-point chunks with zero-weight points), the
+// contraction runs plane by plane, and at each plane's last chunk the
+// accumulators are multiplied by D_p(Y_i) D_p(X_j) and added to the output.
+// Partial sums per plane also keep the FP32 accumulation error of a long
+// sum to the plane's share.  Used when the points lie on at most PLANE_MAX
+// distinct planes with at least PLANE_MIN_AVG points per plane on average
+// (the device decides after grouping, and the general kernel runs otherwise).
+constexpr int PLANE_MAX = 4096;
+constexpr int PLANE_MIN_AVG = 256;
+constexpr int64_t PLANE_D_MAX = (int64_t)1 << 25;   // entries of the plane-chirp tables
+
+__global__ void fr_keys(const double* z, int64_t n, unsigned long long* key, unsigned* idx) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    key[i] = (unsigned long long)__double_as_longlong(z[i]);
+    idx[i] = (unsigned)i;
+}
+
+__global__ void fr_flags(const unsigned long long* key, int64_t n, unsigned* flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    flag[i] = (i == 0 || key[i] != key[i - 1]) ? 1u : 0u;
+}
+
+// pid[i] = 1 + plane of sorted point i; start of each plane, its c, the count
+__global__ void fr_starts(const unsigned long long* key, const unsigned* flag, const unsigned* pid, int64_t n,
+                          double lambda, unsigned* start, double* pc, int* info) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (flag[i]) {
+        const unsigned p = pid[i] - 1;
+        if (p < (unsigned)PLANE_MAX) {
+            start[p] = (unsigned)i;
+            pc[p] = 1.0 / (2.0 * lambda * __longlong_as_double((long long)key[i]));
+        }
+    }
+    if (i == n - 1) info[1] = (int)min(pid[i], (unsigned)PLANE_MAX + 1);
+}
+
+// One workgroup: decides the form (info[0]), pads each plane to whole
+// chunks (poff: exclusive prefix of the padded counts), and writes the chunk
+// table: the plane of each 32-point chunk, bit 31 on the plane's last one.
+__global__ __launch_bounds__(1024) void fr_plan(int64_t n, const unsigned* start, unsigned* poff, int* info,
+                                                int* chunk, int64_t cap_chunks, int64_t dlen) {
+    __shared__ unsigned part[1024];
+    const int np = info[1];
+    const int t = threadIdx.x;
+    if (!(np >= 1 && np <= PLANE_MAX && (int64_t)np * PLANE_MIN_AVG <= n && (int64_t)np * dlen <= PLANE_D_MAX)) {
+        if (t == 0) info[0] = 0;
+        return;
+    }
+    unsigned pad[4], sum = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int q = 4 * t + j;
+        const unsigned cnt = q < np ? (q + 1 < np ? start[q + 1] : (unsigned)n) - start[q] : 0u;
+        pad[j] = (cnt + KC - 1) / KC * KC;
+        sum += pad[j];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        const unsigned v = t >= off ? part[t - off] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    const unsigned total = part[1023];
+    if ((int64_t)(total / KC) > cap_chunks) {
+        if (t == 0) info[0] = 0;
+        return;
+    }
+    unsigned run = part[t] - sum;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int q = 4 * t + j;
+        if (q < np) {
+            poff[q] = run;
+            const unsigned c0 = run / KC, c1 = (run + pad[j]) / KC;
+            for (unsigned c = c0; c < c1; ++c) chunk[c] = q | (c + 1 == c1 ? (int)0x80000000u : 0);
+        }
+        run += pad[j];
+    }
+    if (t == 0) {
+        poff[np] = total;
+        info[2] = (int)(total / KC);
+        info[0] = 1;
+    }
+}
+
+// grouped points: sorted point i to poff[p] + (i - start[p]); the tail of
+// each plane's last chunk gets zero-weight points
+__global__ void fr_fill(const double* x, const double* y, const double* z, const double* amp, const double* phase,
+                        int64_t n, double lambda, const unsigned* order, const unsigned* pid, const unsigned* start,
+                        const unsigned* poff, const double* pc, const int* info, FrPoint* pts) {
+    if (!info[0]) return;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int np = info[1];
+    if (i < n) {
+        const unsigned p = pid[i] - 1;
+        pts[poff[p] + (unsigned)(i - start[p])] = make_frpoint(x, y, z, amp, phase, order[i], lambda);
+    }
+    if (i < (int64_t)np * KC) {
+        const int p = (int)(i / KC), r = (int)(i % KC);
+        const unsigned cnt = (p + 1 < np ? start[p + 1] : (unsigned)n) - start[p];
+        if (poff[p] + cnt + r < poff[p + 1]) pts[poff[p] + cnt + r] = zero_frpoint(pc[p]);
+    }
+}
+
+// plane chirps D_p(Y_i) = exp(i 2 pi c_p Y_i^2) for the rows (dr[p ny + i])
+// and D_p(X_j) for the columns (dc[p nx + j]), phases reduced in FP64
+__global__ void fr_chirps(const double* pc, const int* info, int nx, int ny, double pitch, float2* dr, float2* dc) {
+    if (!info[0]) return;
+    const int np = info[1];
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t nr = (int64_t)np * ny, ncl = (int64_t)np * nx;
+    if (i >= nr + ncl) return;
+    const bool row = i < nr;
+    const int64_t e = row ? i : i - nr;
+    const int len = row ? ny : nx;
+    const int p = (int)(e / len), k = (int)(e % len);
+    const double X = (k - len / 2) * pitch;
+    double t = pc[p] * X * X;
+    t -= rint(t);
+    const float2 v = make_float2(cos_turns((float)t), sin_turns((float)t));
+    if (row)
+        dr[e] = v;
+    else
+        dc[e] = v;
+}
+
+struct PlaneMfmaArgs {
+    const FrPoint* pts;   // grouped by plane, padded
+    const int* chunk;     // plane of each 32-point chunk, bit 31: the plane's last
+    const double* pc;     // c of each plane
+    const int* info;      // [0] this form on, [2] 32-point chunks
+    const float2* dr;     // plane chirps of the rows, [p ny + i]
+    const float2* dc;     // and of the columns, [p nx + j]
+    int nx, ny;
+    double p;
+    float2* out;
+};
+
+// A run of a plane's geometric factor: entries g[m] on rows (A, with the
+// point weight) or columns (B) gph + G m; the phase c (u^2 - 2 P u) at the
+// run's first row is reduced in FP64, the step -2 c u G p likewise.
+template <typename T, bool SPLIT, bool FULL, int AB>
+__device__ __forceinline__ void fr_run_plane(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y,
+                                             const double* p_c, const float* p_wr, const float* p_wi, int k, int gph,
+                                             int i0, int j0, const PlaneMfmaArgs& a) {
+    constexpr int M = (AB == 0 ? MR : MC) / GPH;
+    constexpr int PS = (AB == 0 ? MR : MC) * FrLds<T, SPLIT>::KS;   // plane stride
+    const double c = p_c[k];
+    const double u = AB == 0 ? p_y[k] : p_x[k];
+    const double P0 = AB == 0 ? (i0 + gph - a.ny / 2) * a.p : (j0 + gph - a.nx / 2) * a.p;
+    double t0 = c * u * (u - 2.0 * P0);
+    t0 -= rint(t0);
+    double t1 = -2.0 * c * u * ((double)GPH * a.p);
+    t1 -= rint(t1);
+    float2 v = make_float2(cos_turns((float)t0), sin_turns((float)t0));
+    if constexpr (AB == 0) {
+        const float wr_ = p_wr[k], wi_ = p_wi[k];
+        v = make_float2(wr_ * v.x - wi_ * v.y, wr_ * v.y + wi_ * v.x);
+    }
+    const float2 R = make_float2(cos_turns((float)t1), sin_turns((float)t1));
+    const int lim = AB == 0 ? a.ny - i0 : a.nx - j0;   // rows / columns inside the image
+    T* p0 = AB == 0 ? &L.A[0][gph][k] : &L.B[0][gph][k];
+    float m1 = -1.f;
+    if constexpr (SPLIT) asm volatile("v_mov_b32 %0, -1.0" : "=v"(m1));
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const bool ok = FULL || gph + GPH * m < lim;
+        const float re = ok ? v.x : 0.f, im = ok ? v.y : 0.f;
+        T rh, ih;
+        Mfma<T>::cvt2(re, im, rh, ih);
+        T* q = p0 + m * GPH * FrLds<T, SPLIT>::KS;
+        q[0] = rh;
+        q[PS] = ih;
+        if constexpr (SPLIT) {
+            T rl, il;
+            Mfma<T>::lo2(re, im, rh, ih, m1, rl, il);
+            q[2 * PS] = rl;
+            q[3 * PS] = il;
+        }
+        v = make_float2(fmaf(v.x, R.x, -v.y * R.y), fmaf(v.x, R.y, v.y * R.x));
+    }
+}
+
+// Two neighbouring points per thread: a thread runs points k and k + 1 over
+// the same rows, so each operand plane takes one 32-bit LDS store per row
+// (both points' 16-bit parts) instead of two 16-bit stores.  The generation
+// is bound by the LDS store instructions, not by its arithmetic; the extra
+// FP64 set-up of the second run is the cheaper side.
+constexpr int GPH2 = 32;   // run stride with point pairs
+
+template <typename T, bool SPLIT, bool FULL, int AB>
+__device__ __forceinline__ void fr_run_plane2(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y,
+                                              const double* p_c, const float* p_wr, const float* p_wi, int k,
+                                              int gph, int i0, int j0, const PlaneMfmaArgs& a) {
+    constexpr int M = (AB == 0 ? MR : MC) / GPH2;
+    constexpr int KS = FrLds<T, SPLIT>::KS;
+    constexpr int PS = (AB == 0 ? MR : MC) * KS / 2;   // plane stride in 32-bit words
+    const double P0 = AB == 0 ? (i0 + gph - a.ny / 2) * a.p : (j0 + gph - a.nx / 2) * a.p;
+    float2 v[2], R[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const double c = p_c[k + h];
+        const double u = AB == 0 ? p_y[k + h] : p_x[k + h];
+        double t0 = c * u * (u - 2.0 * P0);
+        t0 -= rint(t0);
+        double t1 = -2.0 * c * u * ((double)GPH2 * a.p);
+        t1 -= rint(t1);
+        v[h] = make_float2(cos_turns((float)t0), sin_turns((float)t0));
+        if constexpr (AB == 0) {
+            const float wr_ = p_wr[k + h], wi_ = p_wi[k + h];
+            v[h] = make_float2(wr_ * v[h].x - wi_ * v[h].y, wr_ * v[h].y + wi_ * v[h].x);
+        }
+        R[h] = make_float2(cos_turns((float)t1), sin_turns((float)t1));
+    }
+    const int lim = AB == 0 ? a.ny - i0 : a.nx - j0;   // rows / columns inside the image
+    unsigned* p0 = reinterpret_cast<unsigned*>(AB == 0 ? &L.A[0][gph][k] : &L.B[0][gph][k]);
+    float m1 = -1.f;
+    if constexpr (SPLIT) asm volatile("v_mov_b32 %0, -1.0" : "=v"(m1));
+#pragma unroll
+    for (int m = 0; m < M; ++m) {
+        const bool ok = FULL || gph + GPH2 * m < lim;
+        float re[2], im[2];
+        T rh[2], ih[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            re[h] = ok ? v[h].x : 0.f;
+            im[h] = ok ? v[h].y : 0.f;
+        }
+        Mfma<T>::cvt2(re[0], re[1], rh[0], rh[1]);
+        Mfma<T>::cvt2(im[0], im[1], ih[0], ih[1]);
+        unsigned* q = p0 + m * GPH2 * KS / 2;
+        q[0] = pack2(rh[0], rh[1]);
+        q[PS] = pack2(ih[0], ih[1]);
+        if constexpr (SPLIT) {
+            T rl[2], il[2];
+            Mfma<T>::lo2(re[0], re[1], rh[0], rh[1], m1, rl[0], rl[1]);
+            Mfma<T>::lo2(im[0], im[1], ih[0], ih[1], m1, il[0], il[1]);
+            q[2 * PS] = pack2(rl[0], rl[1]);
+            q[3 * PS] = pack2(il[0], il[1]);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+            v[h] = make_float2(fmaf(v[h].x, R[h].x, -v[h].y * R[h].y), fmaf(v[h].x, R[h].y, v[h].y * R[h].x));
+    }
+}
+
+template <typename T, bool SPLIT, bool FULL>
+__device__ __forceinline__ void fr_generate_plane(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y,
+                                                  const double* p_c, const float* p_wr, const float* p_wi, int tid,
+                                                  int i0, int j0, const PlaneMfmaArgs& a) {
+    constexpr int NPAIR = KCH<SPLIT> / 2;
+    if constexpr (!SPLIT) {
+        static_assert(NPAIR * GPH2 == MFMA_THREADS, "one thread per (point pair, phase)");
+        const int k = 2 * (tid % NPAIR), gph = tid / NPAIR;
+        fr_run_plane2<T, SPLIT, FULL, 0>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
+        fr_run_plane2<T, SPLIT, FULL, 1>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
+    } else {
+        static_assert(2 * NPAIR * GPH2 == MFMA_THREADS, "one thread per (point pair, phase, factor)");
+        const int h = tid % (MFMA_THREADS / 2);
+        const int k = 2 * (h % NPAIR), gph = h / NPAIR;
+        if (tid < MFMA_THREADS / 2)
+            fr_run_plane2<T, SPLIT, FULL, 0>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
+        else
+            fr_run_plane2<T, SPLIT, FULL, 1>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
+    }
+}
+
+// The end of a plane: out (+)= D(Y) D(X) acc over the wave's 64 x 64 sub-tile
+// (stored on the tile's first plane, added on the others), then acc = 0.
+__device__ __forceinline__ void fr_flush(f32x16 (&cr)[2][2], f32x16 (&ci)[2][2], int pl, int wr, int wc, int lane,
+                                         int i0, int j0, const PlaneMfmaArgs& a) {
+    const bool first = pl == 0;
+    // the lane index through a zero the compiler cannot see through: the
+    // flush's addresses are the same at every plane and, hoisted out of the
+    // chunk loop, would hold ~70 registers across it
+    int zero;
+    asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
+    lane += zero;
+    float2 dc[2];
+    int col[2];
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+        col[y] = j0 + wc * 64 + 32 * y + (lane & 31);
+        dc[y] = col[y] < a.nx ? a.dc[(size_t)pl * a.nx + col[y]] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+            const int row = i0 + wr * 64 + 32 * x + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
+            if (row >= a.ny) continue;
+            const float2 dr = a.dr[(size_t)pl * a.ny + row];
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+                if (col[y] >= a.nx) continue;
+                const float2 d = make_float2(dr.x * dc[y].x - dr.y * dc[y].y, dr.x * dc[y].y + dr.y * dc[y].x);
+                const float re = cr[x][y][v], im = ci[x][y][v];
+                float2 val = make_float2(re * d.x - im * d.y, re * d.y + im * d.x);
+                float2* o = a.out + (size_t)row * a.nx + col[y];
+                if (!first) {
+                    const float2 prev = *o;
+                    val = make_float2(prev.x + val.x, prev.y + val.y);
+                }
+                *o = val;
+            }
+            // four rows at a time: the loads of later rows are not hoisted
+            // above these stores (their registers would be live at once)
+            if ((v & 3) == 3) asm volatile("" ::: "memory");
+        }
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            cr[x][y] = (f32x16)0.f;
+            ci[x][y] = (f32x16)0.f;
+        }
+}
+
+template <typename T, bool SPLIT>
+__global__ __launch_bounds__(MFMA_THREADS) void fresnel_mfma_planes(PlaneMfmaArgs a) {
+    typedef FrLds<T, SPLIT> Lds;
+    constexpr int K = KCH<SPLIT>;
+    if (!a.info[0]) return;
+    extern __shared__ __attribute__((aligned(16))) unsigned char fr_dyn[];
+    Lds* L = reinterpret_cast<Lds*>(fr_dyn);   // two buffers
+    __shared__ double p_x[2][K], p_y[2][K], p_c[2][K];
+    __shared__ float p_wr[2][K], p_wi[2][K];
+
+    const int tid = threadIdx.x;
+    const int wave = tid >> 6, lane = tid & 63;
+    const int wr = wave >> 1, wc = wave & 1;
+    const int ntx = (a.nx + MC - 1) / MC;
+    const int g = xcd_remap(blockIdx.x, gridDim.x);
+    const int i0 = (g / ntx) * MR, j0 = (g % ntx) * MC;
+    const bool full = i0 + MR <= a.ny && j0 + MC <= a.nx;
+    const int64_t nch = (int64_t)a.info[2] * (KC / K);
+
+    f32x16 cr[2][2], ci[2][2];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) {
+            cr[x][y] = (f32x16)0.f;
+            ci[x][y] = (f32x16)0.f;
+        }
+    auto stage_points = [&](int slot, const FrPoint& q) {
+        p_x[slot][tid] = q.x;
+        p_y[slot][tid] = q.y;
+        p_c[slot][tid] = q.c;
+        p_wr[slot][tid] = q.wre;
+        p_wi[slot][tid] = q.wim;
+    };
+    auto generate = [&](int64_t c) {
+        const int slot = (int)(c & 1);
+        if (full)
+            fr_generate_plane<T, SPLIT, true>(L[slot], p_x[slot], p_y[slot], p_c[slot], p_wr[slot], p_wi[slot],
+                                              tid, i0, j0, a);
+        else
+            fr_generate_plane<T, SPLIT, false>(L[slot], p_x[slot], p_y[slot], p_c[slot], p_wr[slot], p_wi[slot],
+                                               tid, i0, j0, a);
+    };
+    FrPoint nq{};
+    if (tid < K) {
+        if (nch > 0) stage_points(0, a.pts[tid]);
+        if (nch > 1) stage_points(1, a.pts[K + tid]);
+        if (nch > 2) nq = a.pts[2 * K + tid];
+    }
+    __syncthreads();
+    if (nch > 0) generate(0);
+    __syncthreads();
+    for (int64_t c = 0; c < nch; ++c) {
+        if (c + 1 < nch) generate(c + 1);
+        fr_contract<T, SPLIT>(L[c & 1], wr, wc, lane, cr, ci);
+        // slot c & 1 held chunk c's points, read by generate(c) one step ago
+        if (tid < K && c + 2 < nch) {
+            stage_points((int)(c & 1), nq);
+            if (c + 3 < nch) nq = a.pts[(c + 3) * K + tid];
+        }
+        // a plane ends with the last K-point part of a chunk marked last
+        const int ce = a.chunk[(c * K) / KC];
+        if (ce < 0 && ((c + 1) * K) % KC == 0) fr_flush(cr, ci, ce & 0x7fffffff, wr, wc, lane, i0, j0, a);
+        __syncthreads();
+    }
+}
+
+template <typename T, bool SPLIT>
+int launch_fresnel_mfma_planes(oph_ctx* ctx, const PlaneMfmaArgs& m, unsigned tiles) {
+    const size_t shm = 2 * sizeof(FrLds<T, SPLIT>);
+    static const bool attr = [&] {
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&fresnel_mfma_planes<T, SPLIT>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
+        hipGetLastError();
+        return true;
+    }();
+    (void)attr;
+    OPH_LAUNCH(ctx, "fresnel_mfma", (fresnel_mfma_planes<T, SPLIT>), dim3(tiles), dim3(MFMA_THREADS), shm, m);
+    return OPH_OK;
+}
+
+// Groups the points by plane (device side) and runs the plane-factored
+// contraction; *info_out receives the device flag word the general path
+// checks (its [0] is nonzero when this form produced the field).
+int fresnel_planes(oph_ctx* ctx, int method, int nx, int ny, double pitch, double lambda, const double* dx,
+                   const double* dy, const double* dz, const double* da, const double* dp, int64_t n, float2* out,
+                   unsigned tiles, const int** info_out) {
+    const int64_t cap_chunks = (n + (int64_t)PLANE_MAX * (KC - 1)) / KC + 1;
+    char* w0 = static_cast<char*>(oph_scratch(ctx, SLOT_WORK0, 32 * (size_t)n + 256));
+    char* tb = static_cast<char*>(oph_scratch(ctx, SLOT_TABLE, 16 * (PLANE_MAX + 1) + 4 * cap_chunks + 256));
+    // plane chirp tables: at most PLANE_D_MAX entries (the form is off beyond)
+    const int64_t dlen = (int64_t)nx + ny;
+    const int64_t dent = std::min<int64_t>((n / PLANE_MIN_AVG) * dlen, PLANE_D_MAX);
+    float2* dtab = static_cast<float2*>(oph_scratch(ctx, SLOT_TABLE2, sizeof(float2) * (size_t)std::max<int64_t>(dent, 1)));
+    if (!dtab) return OPH_ENOMEM;
+    FrPoint* gp = static_cast<FrPoint*>(oph_scratch(ctx, SLOT_WORK1, sizeof(FrPoint) * cap_chunks * KC));
+    if (!w0 || !tb || !gp) return OPH_ENOMEM;
+    auto* kin = reinterpret_cast<unsigned long long*>(w0);
+    auto* kout = kin + n;
+    auto* iin = reinterpret_cast<unsigned*>(kout + n);
+    unsigned* iout = iin + n;
+    unsigned* flag = iout + n;
+    unsigned* pid = flag + n;
+    auto* pc = reinterpret_cast<double*>(tb);
+    auto* start = reinterpret_cast<unsigned*>(pc + PLANE_MAX);
+    unsigned* poff = start + PLANE_MAX + 1;
+    int* info = reinterpret_cast<int*>(poff + PLANE_MAX + 1);
+    int* chunk = info + 4;
+    size_t sort_bytes = 0, scan_bytes = 0;
+    if (rocprim::radix_sort_pairs(nullptr, sort_bytes, kin, kout, iin, iout, (size_t)n, 0, 64, ctx->stream) !=
+            hipSuccess ||
+        rocprim::inclusive_scan(nullptr, scan_bytes, flag, pid, (size_t)n, rocprim::plus<unsigned>(), ctx->stream) !=
+            hipSuccess)
+        return oph_fail(ctx, OPH_EDEVICE, "plane grouping: temporary storage query failed");
+    size_t tmp_bytes = std::max(sort_bytes, scan_bytes);
+    void* tmp = oph_scratch(ctx, SLOT_AUX1, tmp_bytes);
+    if (!tmp) return OPH_ENOMEM;
+    const unsigned gb = (unsigned)((n + 255) / 256);
+    OPH_LAUNCH(ctx, "fresnel_group", fr_keys, dim3(gb), dim3(256), 0, dz, n, kin, iin);
+    if (rocprim::radix_sort_pairs(tmp, sort_bytes, kin, kout, iin, iout, (size_t)n, 0, 64, ctx->stream) !=
+        hipSuccess)
+        return oph_fail(ctx, OPH_EDEVICE, "plane grouping: sort failed");
+    OPH_LAUNCH(ctx, "fresnel_group", fr_flags, dim3(gb), dim3(256), 0, kout, n, flag);
+    if (rocprim::inclusive_scan(tmp, scan_bytes, flag, pid, (size_t)n, rocprim::plus<unsigned>(), ctx->stream) !=
+        hipSuccess)
+        return oph_fail(ctx, OPH_EDEVICE, "plane grouping: scan failed");
+    OPH_LAUNCH(ctx, "fresnel_group", fr_starts, dim3(gb), dim3(256), 0, kout, flag, pid, n, lambda, start, pc,
+               info);
+    OPH_LAUNCH(ctx, "fresnel_group", fr_plan, dim3(1), dim3(1024), 0, n, start, poff, info, chunk, cap_chunks,
+               dlen);
+    const int64_t fill_n = std::max<int64_t>(n, (int64_t)PLANE_MAX * KC);
+    OPH_LAUNCH(ctx, "fresnel_group", fr_fill, dim3((unsigned)((fill_n + 255) / 256)), dim3(256), 0, dx, dy, dz, da,
+               dp, n, lambda, iout, pid, start, poff, pc, info, gp);
+    // rows' tables first, then the columns': np (ny + nx) <= dent entries
+    float2* dr = dtab;
+    float2* dcl = dtab + (size_t)(dent / dlen) * ny;
+    OPH_LAUNCH(ctx, "fresnel_group", fr_chirps, dim3((unsigned)((dent + 255) / 256)), dim3(256), 0, pc, info, nx, ny,
+               pitch, dr, dcl);
+    PlaneMfmaArgs m{};
+    m.pts = gp;
+    m.dr = dr;
+    m.dc = dcl;
+    m.chunk = chunk;
+    m.pc = pc;
+    m.info = info;
+    m.nx = nx;
+    m.ny = ny;
+    m.p = pitch;
+    m.out = out;
+    int rc;
+    switch (method) {
+        case OPH_FRESNEL_MFMA_BF16: rc = launch_fresnel_mfma_planes<__bf16, false>(ctx, m, tiles); break;
+        case OPH_FRESNEL_MFMA_BF16_SPLIT: rc = launch_fresnel_mfma_planes<__bf16, true>(ctx, m, tiles); break;
+        case OPH_FRESNEL_MFMA_FP16: rc = launch_fresnel_mfma_planes<_Float16, false>(ctx, m, tiles); break;
+        default: rc = launch_fresnel_mfma_planes<_Float16, true>(ctx, m, tiles); break;
+    }
+    if (rc) return rc;
+    *info_out = info;
+    return OPH_OK;
+}
+
 }  // namespace
 
 extern "C" int oph_pc_fresnel(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, const oph_points* pts,
@@ -514,15 +1031,25 @@ extern "C" int oph_pc_fresnel(oph_ctx* ctx, int nx, int ny, double pitch, double
     if (n > 0 && (!dx || !dy || !dz)) return OPH_ENOMEM;
     FrPoint* fp = static_cast<FrPoint*>(oph_scratch(ctx, SLOT_AUX0, sizeof(FrPoint) * (npad > 0 ? npad : 1)));
     if (!fp) return OPH_ENOMEM;
-    if (npad > 0)
-        OPH_LAUNCH(ctx, "fresnel_prep", fresnel_prep, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0, dx, dy,
-                   dz, da, dp, n, npad, lambda, fp);
     const size_t npix = (size_t)nx * ny;
     float2* o = reinterpret_cast<float2*>(out);
     if (!(flags & OPH_DEVICE_PTR)) {
         o = static_cast<float2*>(oph_scratch(ctx, SLOT_OUT, npix * sizeof(float2)));
         if (!o) return OPH_ENOMEM;
     }
+    const unsigned tiles = (unsigned)(((nx + MC - 1) / MC) * ((ny + MR - 1) / MR));
+    // MFMA forms: the plane-factored contraction when the points lie on few
+    // planes (decided on the device; OPH_FRESNEL_PLANES=0 disables it), the
+    // general one otherwise -- both are launched, the one not chosen returns
+    const int* skip = nullptr;
+    const char* pe = getenv("OPH_FRESNEL_PLANES");
+    if (method != OPH_FRESNEL_VALU && n >= PLANE_MIN_AVG && n < ((int64_t)1 << 31) && !(pe && pe[0] == '0')) {
+        rc = fresnel_planes(ctx, method, nx, ny, pitch, lambda, dx, dy, dz, da, dp, n, o, tiles, &skip);
+        if (rc) return rc;
+    }
+    if (npad > 0)
+        OPH_LAUNCH(ctx, "fresnel_prep", fresnel_prep, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0, dx, dy,
+                   dz, da, dp, n, npad, lambda, fp, skip);
     if (method == OPH_FRESNEL_VALU) {
         const int gx = (nx + 63) / 64, gy = (ny + 63) / 64;
         int64_t chunks = (4LL * ctx->cu_count + (int64_t)gx * gy - 1) / ((int64_t)gx * gy);
@@ -550,7 +1077,7 @@ extern "C" int oph_pc_fresnel(oph_ctx* ctx, int nx, int ny, double pitch, double
         m.ny = ny;
         m.p = pitch;
         m.out = o;
-        const unsigned tiles = (unsigned)(((nx + MC - 1) / MC) * ((ny + MR - 1) / MR));
+        m.skip = skip;
         switch (method) {
             case OPH_FRESNEL_MFMA_BF16: rc = launch_fresnel_mfma<__bf16, false>(ctx, m, tiles); break;
             case OPH_FRESNEL_MFMA_BF16_SPLIT: rc = launch_fresnel_mfma<__bf16, true>(ctx, m, tiles); break;

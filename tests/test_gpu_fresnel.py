@@ -24,6 +24,60 @@ def test_fresnel_small_vs_oracle(ctx, method):
     assert err < BOUND[method]
 
 
+def _ragged_planes(seed):
+    """4000 points on 5 planes with ragged counts (3000, 600, 366, 33, 1): each
+    plane's last 32-point chunk is partly padding, one plane is a single point."""
+    rng = np.random.default_rng(seed)
+    counts = [3000, 600, 366, 33, 1]
+    zp = [0.1, 0.104, 0.1013, 0.11, 0.1077]
+    z = np.concatenate([np.full(c, zz) for c, zz in zip(counts, zp)])
+    rng.shuffle(z)
+    n = len(z)
+    x = rng.uniform(-0.4e-3, 0.4e-3, n)
+    y = rng.uniform(-0.4e-3, 0.4e-3, n)
+    a = rng.uniform(0, 1, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    return x, y, z, a, ph
+
+
+@pytest.mark.parametrize("method", ["fp16", "fp16_split", "bf16", "bf16_split"])
+def test_fresnel_plane_form_vs_general(ctx, method, monkeypatch):
+    """Points on a few planes take the plane-factored contraction (grouping on
+    the device, per-plane chirps applied at each plane's end); the general
+    contraction (OPH_FRESNEL_PLANES=0) and the FP64 oracle agree with it to
+    the method's bound, on a 300 x 200 field (partial tiles on both axes)."""
+    print(f"case: fresnel ragged planes 300x200 {method}", flush=True)
+    x, y, z, a, ph = _ragged_planes(11)
+    ref = O.fresnel(300, 200, 8e-6, 532e-9, x, y, z, a, ph)
+    U = ctx.fresnel(300, 200, 8e-6, 532e-9, x, y, z, a, ph, method=method)
+    U2 = ctx.fresnel(300, 200, 8e-6, 532e-9, x, y, z, a, ph, method=method)
+    assert np.array_equal(U, U2)  # deterministic
+    monkeypatch.setenv("OPH_FRESNEL_PLANES", "0")
+    G = ctx.fresnel(300, 200, 8e-6, 532e-9, x, y, z, a, ph, method=method)
+    monkeypatch.delenv("OPH_FRESNEL_PLANES")
+    e1, e2 = rel_l2(U, ref), rel_l2(G, ref)
+    print(method, "planes rel-L2", e1, "general rel-L2", e2)
+    assert e1 < BOUND[method] and e2 < BOUND[method]
+
+
+def test_fresnel_distinct_depths_general_form(ctx):
+    """Every point on its own depth: the device keeps the general contraction
+    (too few points per plane for the factored one); parity as usual."""
+    print("case: fresnel distinct depths 160x96", flush=True)
+    rng = np.random.default_rng(5)
+    n = 3000
+    x = rng.uniform(-0.3e-3, 0.3e-3, n)
+    y = rng.uniform(-0.3e-3, 0.3e-3, n)
+    z = rng.uniform(0.1, 0.12, n)
+    a = rng.uniform(0, 1, n)
+    ph = rng.uniform(0, 2 * np.pi, n)
+    ref = O.fresnel(160, 96, 8e-6, 532e-9, x, y, z, a, ph)
+    for m in ("fp16", "fp16_split"):
+        err = rel_l2(ctx.fresnel(160, 96, 8e-6, 532e-9, x, y, z, a, ph, method=m), ref)
+        print(m, "rel-L2", err)
+        assert err < BOUND[m]
+
+
 def test_fresnel_edge_cases(ctx):
     e = np.zeros(0)
     for m in ("valu", "fp16_split"):
@@ -55,3 +109,15 @@ def test_fresnel_config4_sampled(ctx):
         err = rel_l2(U, ref)
         print(m, "config4 sampled rel-L2", err, flush=True)
         assert err < CONFIG4_BOUND[m]
+    # the general contraction on the same points (the plane-factored form is
+    # the default here: 64 planes of ~15,600 points)
+    import os
+    os.environ["OPH_FRESNEL_PLANES"] = "0"
+    try:
+        for m in ("fp16", "fp16_split"):
+            ctx.fresnel(4096, 4096, 8e-6, 532e-9, *d, method=m, out=out, device=True)
+            err = rel_l2(out.numpy()[rows, cols], ref)
+            print(m, "general form config4 sampled rel-L2", err, flush=True)
+            assert err < CONFIG4_BOUND[m]
+    finally:
+        del os.environ["OPH_FRESNEL_PLANES"]
