@@ -289,82 +289,117 @@ struct FrLds {
 };
 
 // Chirp factors by a phase recurrence.  A run is one point's entries on
-// rows (A, with the point weight) or columns (B) gph + G m, G = 16.  Along a
+// rows (A, with the point weight) or columns (B) gph + G m, G = 32.  Along a
 // run the phase c (Y - y)^2 has constant second difference
 // delta = 2 c (G p)^2, so with R = exp(i 2 pi (phi_{m+1} - phi_m)),
 // A_{m+1} = A_m R_m and R_{m+1} = R_m exp(i 2 pi delta): two complex
 // multiplies per entry; the anchor phi_0, the first step and delta are
-// reduced in FP64 once per run (phases of ~1e4 turns).  With 32 points a
-// chunk, thread (k, gph) takes point k's A run (16 entries) and B run (8);
-// with 16, the first four waves take the A runs and the last four the B
-// runs, one run (one FP64 set-up) per thread -- a SIMD holds one wave of
-// each, so its share is the same.  FULL: every row and column of the tile
-// is inside the image (no per-entry bounds select).
-constexpr int GPH = 16;   // run stride (rows or columns)
+// reduced in FP64 once per run (phases of ~1e4 turns).  In the
+// plane-factored form (below) the factor is geometric: one multiply.
+// FULL: every row and column of the tile is inside the image (no per-entry
+// bounds select).
+//
+// Two neighbouring points per thread: a thread runs points k and k + 1 over
+// the same rows, so each operand plane takes one 32-bit LDS store per row
+// (both points' 16-bit parts) instead of two 16-bit stores.  The generation
+// is bound by the LDS store instructions, not by its arithmetic; the extra
+// FP64 set-up of the second run is the cheaper side.
+constexpr int GPH2 = 32;   // run stride with point pairs
 
-template <typename T, bool SPLIT, bool FULL, int AB>
-__device__ __forceinline__ void fr_run(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y, const double* p_c,
-                                       const float* p_wr, const float* p_wi, int k, int gph, int i0, int j0,
-                                       const MfmaArgs& a) {
-    constexpr int M = (AB == 0 ? MR : MC) / GPH;
-    constexpr int PS = (AB == 0 ? MR : MC) * FrLds<T, SPLIT>::KS;   // plane stride
-    const double c = p_c[k];
-    const double sp = (double)GPH * a.p;
-    double td = 2.0 * c * sp * sp;
-    td -= rint(td);
-    const float2 E = make_float2(cos_turns((float)td), sin_turns((float)td));
-    const double d0 = AB == 0 ? (i0 + gph - a.ny / 2) * a.p - p_y[k] : (j0 + gph - a.nx / 2) * a.p - p_x[k];
-    double t0 = c * d0 * d0;
-    t0 -= rint(t0);
-    double t1 = c * (2.0 * d0 * sp + sp * sp);
-    t1 -= rint(t1);
-    float2 v = make_float2(cos_turns((float)t0), sin_turns((float)t0));
-    if constexpr (AB == 0) {
-        const float wr_ = p_wr[k], wi_ = p_wi[k];
-        v = make_float2(wr_ * v.x - wi_ * v.y, wr_ * v.y + wi_ * v.x);
+template <typename T, bool SPLIT, bool FULL, int AB, bool PLANE, typename Args>
+__device__ __forceinline__ void fr_run_pair(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y,
+                                            const double* p_c, const float* p_wr, const float* p_wi, int k, int gph,
+                                            int i0, int j0, const Args& a) {
+    constexpr int M = (AB == 0 ? MR : MC) / GPH2;
+    constexpr int KS = FrLds<T, SPLIT>::KS;
+    constexpr int PS = (AB == 0 ? MR : MC) * KS / 2;   // plane stride in 32-bit words
+    const double sp = (double)GPH2 * a.p;
+    const double P0 = AB == 0 ? (i0 + gph - a.ny / 2) * a.p : (j0 + gph - a.nx / 2) * a.p;
+    float2 v[2], R[2], E[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const double c = p_c[k + h];
+        const double u = AB == 0 ? p_y[k + h] : p_x[k + h];
+        double t0, t1;
+        if constexpr (PLANE) {
+            // the plane's factor: phase c (u^2 - 2 P u), step -2 c u G p
+            t0 = c * u * (u - 2.0 * P0);
+            t1 = -2.0 * c * u * sp;
+        } else {
+            // the full chirp: phase c d^2, first step c (2 d G p + (G p)^2),
+            // second difference 2 c (G p)^2
+            const double d0 = P0 - u;
+            t0 = c * d0 * d0;
+            t1 = c * (2.0 * d0 * sp + sp * sp);
+            double td = 2.0 * c * sp * sp;
+            td -= rint(td);
+            E[h] = make_float2(cos_turns((float)td), sin_turns((float)td));
+        }
+        t0 -= rint(t0);
+        t1 -= rint(t1);
+        v[h] = make_float2(cos_turns((float)t0), sin_turns((float)t0));
+        if constexpr (AB == 0) {
+            const float wr_ = p_wr[k + h], wi_ = p_wi[k + h];
+            v[h] = make_float2(wr_ * v[h].x - wi_ * v[h].y, wr_ * v[h].y + wi_ * v[h].x);
+        }
+        R[h] = make_float2(cos_turns((float)t1), sin_turns((float)t1));
     }
-    float2 R = make_float2(cos_turns((float)t1), sin_turns((float)t1));
     const int lim = AB == 0 ? a.ny - i0 : a.nx - j0;   // rows / columns inside the image
-    T* p0 = AB == 0 ? &L.A[0][gph][k] : &L.B[0][gph][k];
+    unsigned* p0 = reinterpret_cast<unsigned*>(AB == 0 ? &L.A[0][gph][k] : &L.B[0][gph][k]);
     float m1 = -1.f;
     if constexpr (SPLIT) asm volatile("v_mov_b32 %0, -1.0" : "=v"(m1));
 #pragma unroll
     for (int m = 0; m < M; ++m) {
-        const bool ok = FULL || gph + GPH * m < lim;
-        const float re = ok ? v.x : 0.f, im = ok ? v.y : 0.f;
-        T rh, ih;
-        Mfma<T>::cvt2(re, im, rh, ih);
-        T* q = p0 + m * GPH * FrLds<T, SPLIT>::KS;
-        q[0] = rh;
-        q[PS] = ih;
-        if constexpr (SPLIT) {
-            T rl, il;
-            Mfma<T>::lo2(re, im, rh, ih, m1, rl, il);
-            q[2 * PS] = rl;
-            q[3 * PS] = il;
+        const bool ok = FULL || gph + GPH2 * m < lim;
+        float re[2], im[2];
+        T rh[2], ih[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            re[h] = ok ? v[h].x : 0.f;
+            im[h] = ok ? v[h].y : 0.f;
         }
-        v = make_float2(fmaf(v.x, R.x, -v.y * R.y), fmaf(v.x, R.y, v.y * R.x));
-        R = make_float2(fmaf(R.x, E.x, -R.y * E.y), fmaf(R.x, E.y, R.y * E.x));
+        Mfma<T>::cvt2(re[0], re[1], rh[0], rh[1]);
+        Mfma<T>::cvt2(im[0], im[1], ih[0], ih[1]);
+        unsigned* q = p0 + m * GPH2 * KS / 2;
+        q[0] = pack2(rh[0], rh[1]);
+        q[PS] = pack2(ih[0], ih[1]);
+        if constexpr (SPLIT) {
+            T rl[2], il[2];
+            Mfma<T>::lo2(re[0], re[1], rh[0], rh[1], m1, rl[0], rl[1]);
+            Mfma<T>::lo2(im[0], im[1], ih[0], ih[1], m1, il[0], il[1]);
+            q[2 * PS] = pack2(rl[0], rl[1]);
+            q[3 * PS] = pack2(il[0], il[1]);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            v[h] = make_float2(fmaf(v[h].x, R[h].x, -v[h].y * R[h].y), fmaf(v[h].x, R[h].y, v[h].y * R[h].x));
+            if constexpr (!PLANE)
+                R[h] = make_float2(fmaf(R[h].x, E[h].x, -R[h].y * E[h].y), fmaf(R[h].x, E[h].y, R[h].y * E[h].x));
+        }
     }
 }
 
-template <typename T, bool SPLIT, bool FULL>
-__device__ __forceinline__ void fr_generate(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y,
-                                            const double* p_c, const float* p_wr, const float* p_wi, int tid,
-                                            int i0, int j0, const MfmaArgs& a) {
+// One thread per (point pair, phase) -- or, with 16-point chunks, per
+// (point pair, phase, factor): the first four waves take the A runs and the
+// last four the B runs, one of each per SIMD.
+template <typename T, bool SPLIT, bool FULL, bool PLANE, typename Args>
+__device__ __forceinline__ void fr_generate_pairs(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y,
+                                                  const double* p_c, const float* p_wr, const float* p_wi, int tid,
+                                                  int i0, int j0, const Args& a) {
+    constexpr int NPAIR = KCH<SPLIT> / 2;
     if constexpr (!SPLIT) {
-        static_assert(KCH<SPLIT> * GPH == MFMA_THREADS, "one thread per (point, phase)");
-        const int k = tid % KCH<SPLIT>, gph = tid / KCH<SPLIT>;
-        fr_run<T, SPLIT, FULL, 0>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
-        fr_run<T, SPLIT, FULL, 1>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
+        static_assert(NPAIR * GPH2 == MFMA_THREADS, "one thread per (point pair, phase)");
+        const int k = 2 * (tid % NPAIR), gph = tid / NPAIR;
+        fr_run_pair<T, SPLIT, FULL, 0, PLANE>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
+        fr_run_pair<T, SPLIT, FULL, 1, PLANE>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
     } else {
-        static_assert(2 * KCH<SPLIT> * GPH == MFMA_THREADS, "one thread per (point, phase, factor)");
+        static_assert(2 * NPAIR * GPH2 == MFMA_THREADS, "one thread per (point pair, phase, factor)");
         const int h = tid % (MFMA_THREADS / 2);
-        const int k = h % KCH<SPLIT>, gph = h / KCH<SPLIT>;
+        const int k = 2 * (h % NPAIR), gph = h / NPAIR;
         if (tid < MFMA_THREADS / 2)
-            fr_run<T, SPLIT, FULL, 0>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
+            fr_run_pair<T, SPLIT, FULL, 0, PLANE>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
         else
-            fr_run<T, SPLIT, FULL, 1>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
+            fr_run_pair<T, SPLIT, FULL, 1, PLANE>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
     }
 }
 
@@ -457,11 +492,11 @@ __global__ __launch_bounds__(MFMA_THREADS) void fresnel_mfma(MfmaArgs a) {
     auto generate = [&](int64_t c) {
         const int slot = (int)(c & 1);
         if (full)
-            fr_generate<T, SPLIT, true>(L[slot], p_x[slot], p_y[slot], p_c[slot], p_wr[slot], p_wi[slot], tid, i0,
-                                        j0, a);
+            fr_generate_pairs<T, SPLIT, true, false>(L[slot], p_x[slot], p_y[slot], p_c[slot], p_wr[slot], p_wi[slot],
+                                                     tid, i0, j0, a);
         else
-            fr_generate<T, SPLIT, false>(L[slot], p_x[slot], p_y[slot], p_c[slot], p_wr[slot], p_wi[slot], tid,
-                                         i0, j0, a);
+            fr_generate_pairs<T, SPLIT, false, false>(L[slot], p_x[slot], p_y[slot], p_c[slot], p_wr[slot],
+                                                      p_wi[slot], tid, i0, j0, a);
     };
     FrPoint nq{};
     if (tid < K) {
@@ -662,135 +697,6 @@ struct PlaneMfmaArgs {
     float2* out;
 };
 
-// A run of a plane's geometric factor: entries g[m] on rows (A, with the
-// point weight) or columns (B) gph + G m; the phase c (u^2 - 2 P u) at the
-// run's first row is reduced in FP64, the step -2 c u G p likewise.
-template <typename T, bool SPLIT, bool FULL, int AB>
-__device__ __forceinline__ void fr_run_plane(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y,
-                                             const double* p_c, const float* p_wr, const float* p_wi, int k, int gph,
-                                             int i0, int j0, const PlaneMfmaArgs& a) {
-    constexpr int M = (AB == 0 ? MR : MC) / GPH;
-    constexpr int PS = (AB == 0 ? MR : MC) * FrLds<T, SPLIT>::KS;   // plane stride
-    const double c = p_c[k];
-    const double u = AB == 0 ? p_y[k] : p_x[k];
-    const double P0 = AB == 0 ? (i0 + gph - a.ny / 2) * a.p : (j0 + gph - a.nx / 2) * a.p;
-    double t0 = c * u * (u - 2.0 * P0);
-    t0 -= rint(t0);
-    double t1 = -2.0 * c * u * ((double)GPH * a.p);
-    t1 -= rint(t1);
-    float2 v = make_float2(cos_turns((float)t0), sin_turns((float)t0));
-    if constexpr (AB == 0) {
-        const float wr_ = p_wr[k], wi_ = p_wi[k];
-        v = make_float2(wr_ * v.x - wi_ * v.y, wr_ * v.y + wi_ * v.x);
-    }
-    const float2 R = make_float2(cos_turns((float)t1), sin_turns((float)t1));
-    const int lim = AB == 0 ? a.ny - i0 : a.nx - j0;   // rows / columns inside the image
-    T* p0 = AB == 0 ? &L.A[0][gph][k] : &L.B[0][gph][k];
-    float m1 = -1.f;
-    if constexpr (SPLIT) asm volatile("v_mov_b32 %0, -1.0" : "=v"(m1));
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        const bool ok = FULL || gph + GPH * m < lim;
-        const float re = ok ? v.x : 0.f, im = ok ? v.y : 0.f;
-        T rh, ih;
-        Mfma<T>::cvt2(re, im, rh, ih);
-        T* q = p0 + m * GPH * FrLds<T, SPLIT>::KS;
-        q[0] = rh;
-        q[PS] = ih;
-        if constexpr (SPLIT) {
-            T rl, il;
-            Mfma<T>::lo2(re, im, rh, ih, m1, rl, il);
-            q[2 * PS] = rl;
-            q[3 * PS] = il;
-        }
-        v = make_float2(fmaf(v.x, R.x, -v.y * R.y), fmaf(v.x, R.y, v.y * R.x));
-    }
-}
-
-// Two neighbouring points per thread: a thread runs points k and k + 1 over
-// the same rows, so each operand plane takes one 32-bit LDS store per row
-// (both points' 16-bit parts) instead of two 16-bit stores.  The generation
-// is bound by the LDS store instructions, not by its arithmetic; the extra
-// FP64 set-up of the second run is the cheaper side.
-constexpr int GPH2 = 32;   // run stride with point pairs
-
-template <typename T, bool SPLIT, bool FULL, int AB>
-__device__ __forceinline__ void fr_run_plane2(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y,
-                                              const double* p_c, const float* p_wr, const float* p_wi, int k,
-                                              int gph, int i0, int j0, const PlaneMfmaArgs& a) {
-    constexpr int M = (AB == 0 ? MR : MC) / GPH2;
-    constexpr int KS = FrLds<T, SPLIT>::KS;
-    constexpr int PS = (AB == 0 ? MR : MC) * KS / 2;   // plane stride in 32-bit words
-    const double P0 = AB == 0 ? (i0 + gph - a.ny / 2) * a.p : (j0 + gph - a.nx / 2) * a.p;
-    float2 v[2], R[2];
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const double c = p_c[k + h];
-        const double u = AB == 0 ? p_y[k + h] : p_x[k + h];
-        double t0 = c * u * (u - 2.0 * P0);
-        t0 -= rint(t0);
-        double t1 = -2.0 * c * u * ((double)GPH2 * a.p);
-        t1 -= rint(t1);
-        v[h] = make_float2(cos_turns((float)t0), sin_turns((float)t0));
-        if constexpr (AB == 0) {
-            const float wr_ = p_wr[k + h], wi_ = p_wi[k + h];
-            v[h] = make_float2(wr_ * v[h].x - wi_ * v[h].y, wr_ * v[h].y + wi_ * v[h].x);
-        }
-        R[h] = make_float2(cos_turns((float)t1), sin_turns((float)t1));
-    }
-    const int lim = AB == 0 ? a.ny - i0 : a.nx - j0;   // rows / columns inside the image
-    unsigned* p0 = reinterpret_cast<unsigned*>(AB == 0 ? &L.A[0][gph][k] : &L.B[0][gph][k]);
-    float m1 = -1.f;
-    if constexpr (SPLIT) asm volatile("v_mov_b32 %0, -1.0" : "=v"(m1));
-#pragma unroll
-    for (int m = 0; m < M; ++m) {
-        const bool ok = FULL || gph + GPH2 * m < lim;
-        float re[2], im[2];
-        T rh[2], ih[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            re[h] = ok ? v[h].x : 0.f;
-            im[h] = ok ? v[h].y : 0.f;
-        }
-        Mfma<T>::cvt2(re[0], re[1], rh[0], rh[1]);
-        Mfma<T>::cvt2(im[0], im[1], ih[0], ih[1]);
-        unsigned* q = p0 + m * GPH2 * KS / 2;
-        q[0] = pack2(rh[0], rh[1]);
-        q[PS] = pack2(ih[0], ih[1]);
-        if constexpr (SPLIT) {
-            T rl[2], il[2];
-            Mfma<T>::lo2(re[0], re[1], rh[0], rh[1], m1, rl[0], rl[1]);
-            Mfma<T>::lo2(im[0], im[1], ih[0], ih[1], m1, il[0], il[1]);
-            q[2 * PS] = pack2(rl[0], rl[1]);
-            q[3 * PS] = pack2(il[0], il[1]);
-        }
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-            v[h] = make_float2(fmaf(v[h].x, R[h].x, -v[h].y * R[h].y), fmaf(v[h].x, R[h].y, v[h].y * R[h].x));
-    }
-}
-
-template <typename T, bool SPLIT, bool FULL>
-__device__ __forceinline__ void fr_generate_plane(FrLds<T, SPLIT>& L, const double* p_x, const double* p_y,
-                                                  const double* p_c, const float* p_wr, const float* p_wi, int tid,
-                                                  int i0, int j0, const PlaneMfmaArgs& a) {
-    constexpr int NPAIR = KCH<SPLIT> / 2;
-    if constexpr (!SPLIT) {
-        static_assert(NPAIR * GPH2 == MFMA_THREADS, "one thread per (point pair, phase)");
-        const int k = 2 * (tid % NPAIR), gph = tid / NPAIR;
-        fr_run_plane2<T, SPLIT, FULL, 0>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
-        fr_run_plane2<T, SPLIT, FULL, 1>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
-    } else {
-        static_assert(2 * NPAIR * GPH2 == MFMA_THREADS, "one thread per (point pair, phase, factor)");
-        const int h = tid % (MFMA_THREADS / 2);
-        const int k = 2 * (h % NPAIR), gph = h / NPAIR;
-        if (tid < MFMA_THREADS / 2)
-            fr_run_plane2<T, SPLIT, FULL, 0>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
-        else
-            fr_run_plane2<T, SPLIT, FULL, 1>(L, p_x, p_y, p_c, p_wr, p_wi, k, gph, i0, j0, a);
-    }
-}
-
 // The end of a plane: out (+)= D(Y) D(X) acc over the wave's 64 x 64 sub-tile
 // (stored on the tile's first plane, added on the others), then acc = 0.
 __device__ __forceinline__ void fr_flush(f32x16 (&cr)[2][2], f32x16 (&ci)[2][2], int pl, int wr, int wc, int lane,
@@ -879,10 +785,10 @@ __global__ __launch_bounds__(MFMA_THREADS) void fresnel_mfma_planes(PlaneMfmaArg
     auto generate = [&](int64_t c) {
         const int slot = (int)(c & 1);
         if (full)
-            fr_generate_plane<T, SPLIT, true>(L[slot], p_x[slot], p_y[slot], p_c[slot], p_wr[slot], p_wi[slot],
+            fr_generate_pairs<T, SPLIT, true, true>(L[slot], p_x[slot], p_y[slot], p_c[slot], p_wr[slot], p_wi[slot],
                                               tid, i0, j0, a);
         else
-            fr_generate_plane<T, SPLIT, false>(L[slot], p_x[slot], p_y[slot], p_c[slot], p_wr[slot], p_wi[slot],
+            fr_generate_pairs<T, SPLIT, false, true>(L[slot], p_x[slot], p_y[slot], p_c[slot], p_wr[slot], p_wi[slot],
                                                tid, i0, j0, a);
     };
     FrPoint nq{};

@@ -88,9 +88,14 @@ def test_fresnel_edge_cases(ctx):
 
 
 # config-4 bars: the plain path at the north-star 1e-5; the MFMA forms at the
-# 1e-3 contraction target (FP16) or their factor-rounding level (the split
-# forms measure ~6e-5..8e-5 over 1e6 points; BF16 unsplit ~1.9e-3)
-CONFIG4_BOUND = {"valu": 1e-5, "bf16": 6e-3, "bf16_split": 2e-4, "fp16": 1e-3, "fp16_split": 2e-4}
+# 1e-3 contraction target (FP16) or their factor-rounding level.  The points
+# lie on 64 planes, so the plane-factored form runs: its per-plane partial
+# sums keep the FP32 accumulation error of the 1e6-term sum small (the split
+# forms measure ~2e-6..4e-6; BF16 unsplit ~2e-3).  The general form on the
+# same points accumulates all 1e6 terms in one FP32 sum (split forms
+# ~6e-5..8e-5): CONFIG4_GENERAL_BOUND.
+CONFIG4_BOUND = {"valu": 1e-5, "bf16": 6e-3, "bf16_split": 2e-5, "fp16": 1e-3, "fp16_split": 1e-5}
+CONFIG4_GENERAL_BOUND = {"fp16": 1e-3, "fp16_split": 2e-4}
 
 
 def test_fresnel_config4_sampled(ctx):
@@ -118,6 +123,6 @@ def test_fresnel_config4_sampled(ctx):
             ctx.fresnel(4096, 4096, 8e-6, 532e-9, *d, method=m, out=out, device=True)
             err = rel_l2(out.numpy()[rows, cols], ref)
             print(m, "general form config4 sampled rel-L2", err, flush=True)
-            assert err < CONFIG4_BOUND[m]
+            assert err < CONFIG4_GENERAL_BOUND[m]
     finally:
         del os.environ["OPH_FRESNEL_PLANES"]
