@@ -238,6 +238,10 @@ int oph_pc_huygens(oph_ctx* ctx, int nx, int ny, double pitch, double lambda, do
  * accumulation); the MFMA methods evaluate the same field as the separable
  * complex contraction (pixels x points)(points x pixels) with chirp factors
  * rounded to BF16 or FP16 (optionally split hi + lo) and FP32 accumulation.
+ * When the points lie on few depth planes (at most 4096 distinct z, >= 256
+ * points per plane on average) the contraction runs plane by plane with the
+ * plane's chirp exp(i k (X^2 + Y^2)/(2 z_p)) applied once per plane (same
+ * field; the environment variable OPH_FRESNEL_PLANES=0 disables this form).
  * out: ny x nx complex64.  z_n > 0 (distance of the point's plane). */
 #define OPH_FRESNEL_VALU             0
 #define OPH_FRESNEL_MFMA_BF16        1
