@@ -80,14 +80,15 @@ struct AsmConst {
 // y entry is read from the full table).  Branch-free: the phase is evaluated
 // everywhere and the zero selected afterwards, so a caller's unrolled loop
 // is one basic block whose table loads can all be in flight together.
+// bz: the row's band-limit term (taby[i].z), loaded by the caller with the
+// other row-table entries
 template <bool EXACT, bool BAND>
-__device__ __forceinline__ float2 transfer(const float4 ax, const float2 ay, const float4* __restrict__ tyfull, int c,
-                                           int i, const AsmConst& k) {
+__device__ __forceinline__ float2 transfer(const float4 ax, const float2 ay, float bz, int c, int i,
+                                           const AsmConst& k) {
     const float u = ax.y + ay.y;
     bool zero = u >= 1.f;
     if constexpr (BAND) {
         const float s = 1.f - u;
-        const float bz = tyfull[i].z;  // (loaded unconditionally: no branch around the load)
         zero = zero | (ax.z > s) | (bz > s);
     }
     float ph;
@@ -114,11 +115,15 @@ __device__ __forceinline__ void apply_transfer(float2 (&v)[N], const float4 ax, 
                                                const float4* __restrict__ taby, int c, int t, int m0,
                                                const AsmConst& k) {
     float2 ay[N];
+    float bz[N];
 #pragma unroll
-    for (int j = 0; j < N; ++j) ay[j] = taby2[t + (m0 + j) * (L / 16)];
+    for (int j = 0; j < N; ++j) {
+        ay[j] = taby2[t + (m0 + j) * (L / 16)];
+        bz[j] = BAND ? taby[t + (m0 + j) * (L / 16)].z : 0.f;
+    }
 #pragma unroll
     for (int j = 0; j < N; ++j)
-        v[j] = fft::cmul(v[j], transfer<EXACT, BAND>(ax, ay[j], taby, c, t + (m0 + j) * (L / 16), k));
+        v[j] = fft::cmul(v[j], transfer<EXACT, BAND>(ax, ay[j], bz[j], c, t + (m0 + j) * (L / 16), k));
 }
 
 template <int L>
@@ -301,7 +306,7 @@ __global__ __launch_bounds__(1024) void asm_cols(float2* work, int npx, int ny, 
 constexpr int MID_L = 4096, MID_B = 4;
 constexpr int MID_TW = 768;   // LDS entries ahead of the lines: middle-stage table, W^t, W^(4t)
 
-template <int B, bool EXACT>
+template <int B, bool EXACT, bool BAND>
 __global__ __launch_bounds__(B * (MID_L / 16), 4) void asm_cols_mid(
     float2* work, int npx, const float4* __restrict__ tabx, const float4* __restrict__ taby,
     const float2* __restrict__ taby2, AsmConst k, const float2* __restrict__ tw) {
@@ -332,7 +337,8 @@ __global__ __launch_bounds__(B * (MID_L / 16), 4) void asm_cols_mid(
     if (PREFETCH && g < gend) fetch(g);
     constexpr int TWO = fft::last_stage_tw_offset<L>();
     for (int e = threadIdx.x; e < TWO; e += NTHR) tw1[e] = tw[e];
-    for (int e = threadIdx.x; e < 2 * TPL; e += NTHR) wp[e] = tw[TWO + (e < TPL ? e : 2 * TPL + e)];
+    if constexpr (PREFETCH)
+        for (int e = threadIdx.x; e < 2 * TPL; e += NTHR) wp[e] = tw[TWO + (e < TPL ? e : 2 * TPL + e)];
     const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
     float2* line = lines + b * PL;
     const auto twl = oph_buf_rsrc(tw + TWO, 15 * TPL * sizeof(float2));
@@ -383,8 +389,7 @@ __global__ __launch_bounds__(B * (MID_L / 16), 4) void asm_cols_mid(
         // registers, the inverse's first butterfly's inputs: the transfer is
         // applied in between, without a pass through LDS
         fft::last_stage_to_regs_w<L, -1>(line, w[0], t, v);
-        if (k.band) apply_transfer<L, EXACT, true>(v, ax, taby2, taby, c, tz, 0, k);
-        else apply_transfer<L, EXACT, false>(v, ax, taby2, taby, c, tz, 0, k);
+        apply_transfer<L, EXACT, BAND>(v, ax, taby2, taby, c, tz, 0, k);
         __syncthreads();  // every lane has read the line
         fft::stage0_from_regs<L, +1>(v, line, t);
         // the next group's rows: in flight through the inverse transform
@@ -708,18 +713,21 @@ int launch_rows_inv(oph_ctx* ctx, const float2* work, int nx, int ny, int ox, in
     return OPH_OK;
 }
 
-template <int B, bool EXACT>
-int launch_cols_mid(oph_ctx* ctx, float2* work, int npx, const float4* tx, const float4* ty, const AsmConst& k,
-                    const float2* tw) {
+// (the band limit is a separate instantiation: its row-table terms would
+// otherwise take registers in the unlimited kernel too)
+template <int B, bool EXACT, bool BAND>
+int launch_cols_mid_t(oph_ctx* ctx, float2* work, int npx, const float4* tx, const float4* ty, const AsmConst& k,
+                      const float2* tw) {
     constexpr int NTHR = B * (MID_L / 16);
     const size_t shm = ((size_t)B * fft::padded_len(MID_L) + MID_TW) * sizeof(float2);
     static const int occ = [&] {
-        hipFuncSetAttribute(reinterpret_cast<const void*>(&asm_cols_mid<B, EXACT>),
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&asm_cols_mid<B, EXACT, BAND>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm);
         hipGetLastError();
         int n = 0;
-        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, reinterpret_cast<const void*>(&asm_cols_mid<B, EXACT>),
-                                                         NTHR, shm) != hipSuccess || n < 1) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &n, reinterpret_cast<const void*>(&asm_cols_mid<B, EXACT, BAND>), NTHR, shm) != hipSuccess ||
+            n < 1) {
             hipGetLastError();
             n = 1;
         }
@@ -728,9 +736,16 @@ int launch_cols_mid(oph_ctx* ctx, float2* work, int npx, const float4* tx, const
     // one resident wave of workgroups, a multiple of 8, at most one per group
     const int grid = std::max(8, std::min(npx / B, occ * std::max(ctx->cu_count, 1)) / 8 * 8);
     const float2* ty2 = reinterpret_cast<const float2*>(ty + MID_L);
-    OPH_LAUNCH(ctx, "asm_cols", (asm_cols_mid<B, EXACT>), dim3(grid), dim3(NTHR), shm, work, npx, tx, ty, ty2, k,
-               tw);
+    OPH_LAUNCH(ctx, "asm_cols", (asm_cols_mid<B, EXACT, BAND>), dim3(grid), dim3(NTHR), shm, work, npx, tx, ty, ty2,
+               k, tw);
     return OPH_OK;
+}
+
+template <int B, bool EXACT>
+int launch_cols_mid(oph_ctx* ctx, float2* work, int npx, const float4* tx, const float4* ty, const AsmConst& k,
+                    const float2* tw) {
+    return k.band ? launch_cols_mid_t<B, EXACT, true>(ctx, work, npx, tx, ty, k, tw)
+                  : launch_cols_mid_t<B, EXACT, false>(ctx, work, npx, tx, ty, k, tw);
 }
 
 template <int LY, int B, bool EXACT>
