@@ -769,6 +769,9 @@ __global__ __launch_bounds__(fftm::threads(LX)) void dm_rows_m(DmArgs a, const f
 #pragma unroll
     for (int m = 0; m < EPT; ++m) acca[m] = accb[m] = make_float2(0.f, 0.f);
     const size_t lstride = gstride * a.ngroups;
+    // a layer's slice of I addressed by 32-bit buffer offsets when it is
+    // below 2 GiB (else plain loads with a branch per element)
+    const bool bufs = lstride * sizeof(float2) < ((size_t)1 << 31);
     for (int l = a.lo; l <= a.hi; ++l) {
         if (!a.any[l - 1]) continue;
         // the thread index through a zero the compiler cannot see through:
@@ -777,20 +780,63 @@ __global__ __launch_bounds__(fftm::threads(LX)) void dm_rows_m(DmArgs a, const f
         int zero;
         asm volatile("v_mov_b32 %0, 0" : "=v"(zero));
         const int tl = t + zero;
-        const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * GROUP;
-        const unsigned char* fl = a.flags + (size_t)(l - 1) * a.ngroups;
+        if (bufs) {
+            // range-checked buffer loads, eight elements at a time: the
+            // presence flags of their column groups, then the values, whose
+            // offset is out of range (a zero, no memory access) for absent
+            // groups and columns outside the image -- each batch is
+            // requested before any of it is used, with no branch per element
+            const auto rf = oph_buf_rsrc(a.flags + (size_t)(l - 1) * a.ngroups, (unsigned)a.ngroups);
+            const auto ri = oph_buf_rsrc(a.I + (size_t)(l - 1) * lstride, (unsigned)(lstride * sizeof(float2)));
+            constexpr int CHB = 8;
 #pragma unroll
-        for (int m = 0; m < EPT; ++m) {
-            const int kx = tl + m * NT;
-            if (EPT * NT == LX || kx < LX) {
-                const int x = kx - a.ox;
-                float2 v = make_float2(0.f, 0.f);
-                if (x >= 0 && x < a.w && fl[x / GROUP]) {
-                    const size_t o = (size_t)(x / GROUP) * gstride + x % GROUP;
-                    v = src[o];
-                    if (pair0) v.y = src[nyq + o].x;
+            for (int m0 = 0; m0 < EPT; m0 += CHB) {
+                unsigned char f[CHB];
+                unsigned o[CHB];
+#pragma unroll
+                for (int j = 0; j < CHB; ++j) {
+                    const int kx = tl + (m0 + j) * NT;
+                    const int x = kx - a.ox;
+                    const bool in = m0 + j < EPT && (EPT * NT == LX || kx < LX) && x >= 0 && x < a.w;
+                    f[j] = __builtin_amdgcn_raw_buffer_load_b8(rf, in ? (unsigned)(x / GROUP) : OPH_BUF_OOB, 0, 0);
+                    o[j] = in ? (unsigned)(((size_t)(x / GROUP) * gstride + x % GROUP + (size_t)kya * GROUP) *
+                                           sizeof(float2))
+                              : OPH_BUF_OOB;
                 }
-                line[fft::pad16(kx)] = v;
+                float2 v[CHB];
+                float vy[CHB];
+#pragma unroll
+                for (int j = 0; j < CHB; ++j) {
+                    const unsigned oj = f[j] ? o[j] : OPH_BUF_OOB;
+                    v[j] = oph_buf_load_f2(ri, oj);
+                    // the packed Nyquist row (workgroup 0 only; elsewhere an
+                    // out-of-range offset: no memory access)
+                    const unsigned oy = pair0 && oj != OPH_BUF_OOB ? oj + (unsigned)(nyq * sizeof(float2)) : OPH_BUF_OOB;
+                    vy[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(ri, oy, 0, 0));
+                }
+#pragma unroll
+                for (int j = 0; j < CHB; ++j) {
+                    const int kx = tl + (m0 + j) * NT;
+                    if (m0 + j < EPT && (EPT * NT == LX || kx < LX))
+                        line[fft::pad16(kx)] = pair0 ? make_float2(v[j].x, vy[j]) : v[j];
+                }
+            }
+        } else {
+            const float2* src = a.I + (size_t)(l - 1) * lstride + (size_t)kya * GROUP;
+            const unsigned char* fl = a.flags + (size_t)(l - 1) * a.ngroups;
+#pragma unroll
+            for (int m = 0; m < EPT; ++m) {
+                const int kx = tl + m * NT;
+                if (EPT * NT == LX || kx < LX) {
+                    const int x = kx - a.ox;
+                    float2 v = make_float2(0.f, 0.f);
+                    if (x >= 0 && x < a.w && fl[x / GROUP]) {
+                        const size_t o = (size_t)(x / GROUP) * gstride + x % GROUP;
+                        v = src[o];
+                        if (pair0) v.y = src[nyq + o].x;
+                    }
+                    line[fft::pad16(kx)] = v;
+                }
             }
         }
         __syncthreads();

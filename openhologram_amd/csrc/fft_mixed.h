@@ -90,6 +90,15 @@ __host__ __device__ constexpr int threads(int N) {
 // Elements per thread when a thread owns t + m NT, m < ept(N).
 __host__ __device__ constexpr int ept(int N) { return (N + threads(N) - 1) / threads(N); }
 
+// Marks every element of w (but w[0]) as needed at this point, after all
+// of them were loaded: the loads are issued as one batch and waited for
+// once (left to itself the compiler orders load, wait, use per element).
+template <int R>
+__device__ __forceinline__ void batch_ready(float2 (&w)[R]) {
+#pragma unroll
+    for (int r = 1; r < R; ++r) asm volatile("" : "+v"(w[r].x), "+v"(w[r].y));
+}
+
 // One stage of radix R (Ns = product of the earlier radices) by NT threads;
 // tws is the stage's table (unused for Ns = 1).  Starts and ends with the
 // line complete in LDS (a barrier separates its reads from its writes and
@@ -114,11 +123,14 @@ __device__ __forceinline__ void stage(float2* buf, const float2* __restrict__ tw
         if (PER * NT == NB || j < NB) {
             const int k = j % NS;
             if constexpr (NS > 1) {
+                // the butterfly's R - 1 twiddles are all requested before the
+                // first is used: one wait for the batch, not one per load
+                float2 w[R];
 #pragma unroll
-                for (int r = 1; r < R; ++r) {
-                    const float2 w = tws[(r - 1) * NS + k];
-                    v[p][r] = SIGN < 0 ? fft::cmul(v[p][r], w) : fft::cmulc(v[p][r], w);
-                }
+                for (int r = 1; r < R; ++r) w[r] = tws[(r - 1) * NS + k];
+                batch_ready(w);
+#pragma unroll
+                for (int r = 1; r < R; ++r) v[p][r] = SIGN < 0 ? fft::cmul(v[p][r], w[r]) : fft::cmulc(v[p][r], w[r]);
             }
             fft::Dft<R, SIGN>::run(v[p]);
             const int base = (j - k) * R + k;
