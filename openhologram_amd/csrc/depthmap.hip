@@ -371,6 +371,26 @@ __device__ __forceinline__ void finish_row8(const DmArgs& a, const float2 (&acc)
 
 // PAIR0: workgroup 0, the real rows 0 and Ny/2 packed as re/im.  FBITS:
 // the presence bits of every (layer, group) fit in LDS (a.fbits_lds).
+// Copies n elements of a table to LDS with NT threads, eight loads per
+// thread requested before any is stored (a plain loop waits for each load
+// before the next is issued).
+template <int NT, typename E>
+__device__ __forceinline__ void lds_copy(E* dst, const E* __restrict__ src, int n, int t) {
+    for (int i0 = 0; i0 < n; i0 += 8 * NT) {
+        E v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int i = i0 + t + j * NT;
+            if (i < n) v[j] = src[i];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const int i = i0 + t + j * NT;
+            if (i < n) dst[i] = v[j];
+        }
+    }
+}
+
 template <int LX, bool PAIR0, bool FBITS>
 __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __restrict__ tw8, float2* sm, int kya) {
     constexpr int TPL = LX / 8;
@@ -386,10 +406,9 @@ __device__ __forceinline__ void rows_h_body(const DmArgs& a, const float2* __res
     // then only present groups' rows of I are loaded
     unsigned* fb = amask + MAX_LAYERS / 32;
     const int t = threadIdx.x;
-    for (int i = t; i < NTW; i += TPL) tws[i] = tw8[i];
+    lds_copy<TPL>(tws, tw8, NTW, t);
     for (int i = t; i < MAX_LAYERS / 32; i += TPL) amask[i] = 0u;
-    if constexpr (FBITS)
-        for (int i = t; i < a.L * a.fwpl; i += TPL) fb[i] = a.fbits[i];
+    if constexpr (FBITS) lds_copy<TPL>(fb, a.fbits, a.L * a.fwpl, t);
     __syncthreads();
     for (int i = t; i < a.L; i += TPL)
         if (a.any[i]) atomicOr(&amask[i >> 5], 1u << (i & 31));
