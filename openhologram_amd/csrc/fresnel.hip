@@ -593,11 +593,10 @@ __global__ void fr_starts(const unsigned long long* key, const unsigned* flag, c
     if (i == n - 1) info[1] = (int)min(pid[i], (unsigned)PLANE_MAX + 1);
 }
 
-// One workgroup: decides the form (info[0]), pads each plane to whole
-// chunks (poff: exclusive prefix of the padded counts), and writes the chunk
-// table: the plane of each 32-point chunk, bit 31 on the plane's last one.
+// One workgroup: decides the form (info[0]) and pads each plane to whole
+// chunks (poff: exclusive prefix of the padded counts).
 __global__ __launch_bounds__(1024) void fr_plan(int64_t n, const unsigned* start, unsigned* poff, int* info,
-                                                int* chunk, int64_t cap_chunks, int64_t dlen) {
+                                                int64_t cap_chunks, int64_t dlen) {
     __shared__ unsigned part[1024];
     const int np = info[1];
     const int t = threadIdx.x;
@@ -630,11 +629,7 @@ __global__ __launch_bounds__(1024) void fr_plan(int64_t n, const unsigned* start
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
         const int q = 4 * t + j;
-        if (q < np) {
-            poff[q] = run;
-            const unsigned c0 = run / KC, c1 = (run + pad[j]) / KC;
-            for (unsigned c = c0; c < c1; ++c) chunk[c] = q | (c + 1 == c1 ? (int)0x80000000u : 0);
-        }
+        if (q < np) poff[q] = run;
         run += pad[j];
     }
     if (t == 0) {
@@ -687,9 +682,9 @@ __global__ void fr_chirps(const double* pc, const int* info, int nx, int ny, dou
 
 struct PlaneMfmaArgs {
     const FrPoint* pts;   // grouped by plane, padded
-    const int* chunk;     // plane of each 32-point chunk, bit 31: the plane's last
+    const unsigned* poff; // padded offset of each plane's first point (np + 1)
     const double* pc;     // c of each plane
-    const int* info;      // [0] this form on, [2] 32-point chunks
+    const int* info;      // [0] this form on, [1] planes, [2] 32-point chunks
     const float2* dr;     // plane chirps of the rows, [p ny + i]
     const float2* dc;     // and of the columns, [p nx + j]
     int nx, ny;
@@ -715,29 +710,44 @@ __device__ __forceinline__ void fr_flush(f32x16 (&cr)[2][2], f32x16 (&ci)[2][2],
         col[y] = j0 + wc * 64 + 32 * y + (lane & 31);
         dc[y] = col[y] < a.nx ? a.dc[(size_t)pl * a.nx + col[y]] : make_float2(0.f, 0.f);
     }
+    // four rows at a time: their chirps and previous outputs are all
+    // requested before any is used (addresses clamped into the image, so no
+    // branch guards a load; stores outside the image are skipped)
+    const int cc0 = min(col[0], a.nx - 1), cc1 = min(col[1], a.nx - 1);
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-        for (int v = 0; v < 16; ++v) {
-            const int row = i0 + wr * 64 + 32 * x + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3);
-            if (row >= a.ny) continue;
-            const float2 dr = a.dr[(size_t)pl * a.ny + row];
+        for (int vg = 0; vg < 4; ++vg) {
+            float2 dr[4], pv[4][2];
+            int row[4];
 #pragma unroll
-            for (int y = 0; y < 2; ++y) {
-                if (col[y] >= a.nx) continue;
-                const float2 d = make_float2(dr.x * dc[y].x - dr.y * dc[y].y, dr.x * dc[y].y + dr.y * dc[y].x);
-                const float re = cr[x][y][v], im = ci[x][y][v];
-                float2 val = make_float2(re * d.x - im * d.y, re * d.y + im * d.x);
-                float2* o = a.out + (size_t)row * a.nx + col[y];
-                if (!first) {
-                    const float2 prev = *o;
-                    val = make_float2(prev.x + val.x, prev.y + val.y);
-                }
-                *o = val;
+            for (int q = 0; q < 4; ++q) {
+                row[q] = i0 + wr * 64 + 32 * x + 8 * vg + 4 * (lane >> 5) + q;
+                const int rc = min(row[q], a.ny - 1);
+                dr[q] = a.dr[(size_t)pl * a.ny + rc];
+                // (loaded on the first plane too, then not used: no branch)
+                pv[q][0] = a.out[(size_t)rc * a.nx + cc0];
+                pv[q][1] = a.out[(size_t)rc * a.nx + cc1];
             }
-            // four rows at a time: the loads of later rows are not hoisted
-            // above these stores (their registers would be live at once)
-            if ((v & 3) == 3) asm volatile("" ::: "memory");
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                asm volatile("" : "+v"(dr[q].x), "+v"(dr[q].y), "+v"(pv[q][0].x), "+v"(pv[q][0].y), "+v"(pv[q][1].x),
+                             "+v"(pv[q][1].y));
+                if (first) pv[q][0] = pv[q][1] = make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int v = 4 * vg + q;
+#pragma unroll
+                for (int y = 0; y < 2; ++y) {
+                    const float2 d = make_float2(dr[q].x * dc[y].x - dr[q].y * dc[y].y,
+                                                 dr[q].x * dc[y].y + dr[q].y * dc[y].x);
+                    const float re = cr[x][y][v], im = ci[x][y][v];
+                    const float2 val =
+                        make_float2(pv[q][y].x + (re * d.x - im * d.y), pv[q][y].y + (re * d.y + im * d.x));
+                    if (row[q] < a.ny && col[y] < a.nx) a.out[(size_t)row[q] * a.nx + col[y]] = val;
+                }
+            }
         }
 #pragma unroll
     for (int x = 0; x < 2; ++x)
@@ -766,6 +776,9 @@ __global__ __launch_bounds__(MFMA_THREADS) void fresnel_mfma_planes(PlaneMfmaArg
     const int i0 = (g / ntx) * MR, j0 = (g % ntx) * MC;
     const bool full = i0 + MR <= a.ny && j0 + MC <= a.nx;
     const int64_t nch = (int64_t)a.info[2] * (KC / K);
+    // the current plane and its last K-point chunk
+    int pl = 0;
+    int64_t pend = (int64_t)(a.poff[1] / K) - 1;
 
     f32x16 cr[2][2], ci[2][2];
 #pragma unroll
@@ -808,9 +821,14 @@ __global__ __launch_bounds__(MFMA_THREADS) void fresnel_mfma_planes(PlaneMfmaArg
             stage_points((int)(c & 1), nq);
             if (c + 3 < nch) nq = a.pts[(c + 3) * K + tid];
         }
-        // a plane ends with the last K-point part of a chunk marked last
-        const int ce = a.chunk[(c * K) / KC];
-        if (ce < 0 && ((c + 1) * K) % KC == 0) fr_flush(cr, ci, ce & 0x7fffffff, wr, wc, lane, i0, j0, a);
+        // the plane ends with its last chunk: flush, then the next plane's end
+        // (a load once per plane, not per chunk: in the vector memory count it
+        // would wait for the point prefetch)
+        if (c == pend) {
+            fr_flush(cr, ci, pl, wr, wc, lane, i0, j0, a);
+            ++pl;
+            if (pl < a.info[1]) pend = (int64_t)(a.poff[pl + 1] / K) - 1;
+        }
         __syncthreads();
     }
 }
@@ -837,7 +855,7 @@ int fresnel_planes(oph_ctx* ctx, int method, int nx, int ny, double pitch, doubl
                    unsigned tiles, const int** info_out) {
     const int64_t cap_chunks = (n + (int64_t)PLANE_MAX * (KC - 1)) / KC + 1;
     char* w0 = static_cast<char*>(oph_scratch(ctx, SLOT_WORK0, 32 * (size_t)n + 256));
-    char* tb = static_cast<char*>(oph_scratch(ctx, SLOT_TABLE, 16 * (PLANE_MAX + 1) + 4 * cap_chunks + 256));
+    char* tb = static_cast<char*>(oph_scratch(ctx, SLOT_TABLE, 16 * (PLANE_MAX + 1) + 256));
     // plane chirp tables: at most PLANE_D_MAX entries (the form is off beyond)
     const int64_t dlen = (int64_t)nx + ny;
     const int64_t dent = std::min<int64_t>((n / PLANE_MIN_AVG) * dlen, PLANE_D_MAX);
@@ -855,7 +873,6 @@ int fresnel_planes(oph_ctx* ctx, int method, int nx, int ny, double pitch, doubl
     auto* start = reinterpret_cast<unsigned*>(pc + PLANE_MAX);
     unsigned* poff = start + PLANE_MAX + 1;
     int* info = reinterpret_cast<int*>(poff + PLANE_MAX + 1);
-    int* chunk = info + 4;
     size_t sort_bytes = 0, scan_bytes = 0;
     if (rocprim::radix_sort_pairs(nullptr, sort_bytes, kin, kout, iin, iout, (size_t)n, 0, 64, ctx->stream) !=
             hipSuccess ||
@@ -876,8 +893,7 @@ int fresnel_planes(oph_ctx* ctx, int method, int nx, int ny, double pitch, doubl
         return oph_fail(ctx, OPH_EDEVICE, "plane grouping: scan failed");
     OPH_LAUNCH(ctx, "fresnel_group", fr_starts, dim3(gb), dim3(256), 0, kout, flag, pid, n, lambda, start, pc,
                info);
-    OPH_LAUNCH(ctx, "fresnel_group", fr_plan, dim3(1), dim3(1024), 0, n, start, poff, info, chunk, cap_chunks,
-               dlen);
+    OPH_LAUNCH(ctx, "fresnel_group", fr_plan, dim3(1), dim3(1024), 0, n, start, poff, info, cap_chunks, dlen);
     const int64_t fill_n = std::max<int64_t>(n, (int64_t)PLANE_MAX * KC);
     OPH_LAUNCH(ctx, "fresnel_group", fr_fill, dim3((unsigned)((fill_n + 255) / 256)), dim3(256), 0, dx, dy, dz, da,
                dp, n, lambda, iout, pid, start, poff, pc, info, gp);
@@ -890,7 +906,7 @@ int fresnel_planes(oph_ctx* ctx, int method, int nx, int ny, double pitch, doubl
     m.pts = gp;
     m.dr = dr;
     m.dc = dcl;
-    m.chunk = chunk;
+    m.poff = poff;
     m.pc = pc;
     m.info = info;
     m.nx = nx;
