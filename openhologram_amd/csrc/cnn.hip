@@ -30,6 +30,8 @@ typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
 
 
 struct oph_cnn {
@@ -518,14 +520,28 @@ constexpr int RB_PS = 32 + 8;                         // LDS pixel stride (bf16)
 constexpr int RB_NPRE = (RB_IH * RB_IW * 4 + 255) / 256;  // 16-byte pieces per thread (7)
 
 struct ResArgs {
-    const __bf16* h;      // [B][H][W][32]
+    const __bf16* h;      // [B][H][W][32] (unused by the head form)
     __bf16* out;          // [B][H][W][32], distinct from h
     const bf16x8* w1;     // B fragments of conv1 / conv2 (pack_weights)
     const bf16x8* w2;
     const float* b1;
     const float* b2;
     int B, H, W;
+    // head form: h = conv_head(x) computed on the staged tile
+    const float* x;       // [B][H][W][2]
+    const bf16x8* w0;     // head fragments (2 k steps)
+    const float* b0;
 };
+
+// Head form of the first block: instead of h, the 2-channel network input x
+// is staged (14 x 38 pixels, a three-pixel halo, as BF16 pairs) and the head
+// convolution computes h on the 12 x 36 staged region, in 14 groups of 32
+// pixels (K = 18 in two k steps of 16: k = 2 tap + channel, as conv3x3_fwd
+// orders it), BF16-rounded exactly as the separate head layer stores it and
+// zero outside the image.  h never goes to HBM.
+constexpr int RB_XH = RB_IH + 2, RB_XW = RB_IW + 2;    // staged input of the head form
+constexpr int RB_NPX = (RB_XH * RB_XW + 255) / 256;    // its 8-byte pieces per thread (3)
+constexpr int RB_HG = (RB_IH * RB_IW + 31) / 32;       // head groups of 32 pixels (14)
 
 // Software pipelines of the two convolutions, unrolled at compile time: the
 // LDS operands of step S are read at step S and consumed by the MFMAs at
@@ -600,6 +616,23 @@ __device__ __forceinline__ void rb_c2_pipe(const __bf16* src, const bf16x8* w2l,
     }
 }
 
+// relu(acc + bias) of one lane's 16 channels (4 runs of 4) as BF16, all
+// zero when keep is 0 (pixels outside the image: conv2's zero padding).
+// Two values per conversion, and a mask instead of a branch: the bias is
+// already in registers, so nothing here waits on LDS.
+__device__ __forceinline__ void rb_relu_store(const f32x16& acc, const float4 (&bq)[4], unsigned keep,
+                                              __bf16* dst) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const f32x2 lo = {fmaxf(acc[4 * q + 0] + bq[q].x, 0.f), fmaxf(acc[4 * q + 1] + bq[q].y, 0.f)};
+        const f32x2 hi = {fmaxf(acc[4 * q + 2] + bq[q].z, 0.f), fmaxf(acc[4 * q + 3] + bq[q].w, 0.f)};
+        u32x2 o;
+        o[0] = __builtin_bit_cast(unsigned, __builtin_convertvector(lo, bf16x2)) & keep;
+        o[1] = __builtin_bit_cast(unsigned, __builtin_convertvector(hi, bf16x2)) & keep;
+        *reinterpret_cast<u32x2*>(dst + 8 * q) = o;
+    }
+}
+
 // conv1 (weights in registers) + bias + relu for NR consecutive rows of 32
 // pixels of the halo tile, mid rows r0 .. r0 + NR - 1, mid columns px.
 // Input row r0 + ri feeds output row r0 + j at tap row dy = ri - j, so
@@ -609,6 +642,10 @@ template <int NR>
 __device__ __forceinline__ void rb_conv1_rows(const __bf16* in_t, __bf16* mid_t, const bf16x8 (&Bf)[18],
                                               const float4* bias_s, int r0, int px, int kh, int x0, int y0, int H,
                                               int W) {
+    float4 bq[4];  // bias of this lane's channels 8 q + 4 kh + (0..3)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bq[q] = bias_s[2 * q + kh];
+    __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);  // their own group, ahead of the pipeline's
     f32x16 acc[NR];
 #pragma unroll
     for (int j = 0; j < NR; ++j) acc[j] = (f32x16)0.f;
@@ -616,22 +653,34 @@ __device__ __forceinline__ void rb_conv1_rows(const __bf16* in_t, __bf16* mid_t,
     bf16x8 X[RB_C1_AHEAD + 1];
     rb_c1_pipe<NR, 0>(src, X, acc, Bf);
     const int xx = x0 - 1 + px;
+    const bool xin = xx >= 0 && xx < W;
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
         const int mr = r0 + j;
         const int yy = y0 - 1 + mr;
-        const bool inside = yy >= 0 && yy < H && xx >= 0 && xx < W;  // else conv2's zero padding
-        __bf16* dst = mid_t + (mr * RB_MW + px) * RB_PS + 4 * kh;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float4 bq = bias_s[2 * q + kh];
-            bf16x4 o;
-            o[0] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 0] + bq.x, 0.f) : 0.f);
-            o[1] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 1] + bq.y, 0.f) : 0.f);
-            o[2] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 2] + bq.z, 0.f) : 0.f);
-            o[3] = (__bf16)(inside ? fmaxf(acc[j][4 * q + 3] + bq.w, 0.f) : 0.f);
-            *reinterpret_cast<bf16x4*>(dst + 8 * q) = o;
+        const unsigned keep = xin && yy >= 0 && yy < H ? ~0u : 0u;  // else conv2's zero padding
+        rb_relu_store(acc[j], bq, keep, mid_t + (mr * RB_MW + px) * RB_PS + 4 * kh);
+    }
+}
+
+// The strip's 18 k steps as a software pipeline like rb_c1_pipe: the operand
+// of step S is read at step S and used by the MFMA at step S + AHEAD.
+template <int S>
+__device__ __forceinline__ void rb_strip_pipe(const __bf16* src, bf16x8 (&X)[RB_C1_AHEAD + 1], f32x16 (&ac2)[2],
+                                              const bf16x8 (&Bf)[18]) {
+    constexpr int NS = 18, AH = RB_C1_AHEAD;
+    if constexpr (S < NS + AH) {
+        if constexpr (S < NS) {
+            constexpr int tap = S >> 1;
+            X[S % (AH + 1)] =
+                *reinterpret_cast<const bf16x8*>(src + ((tap / 3) * RB_IW + tap % 3) * RB_PS + 16 * (S & 1));
         }
+        constexpr int U = S - AH;
+        if constexpr (U >= 0)
+            ac2[U & 1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bf[U], X[U % (AH + 1)], ac2[U & 1], 0, 0, 0);
+        if constexpr (S < NS) __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+        if constexpr (U >= 0) __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+        rb_strip_pipe<S + 1>(src, X, ac2, Bf);
     }
 }
 
@@ -641,32 +690,21 @@ __device__ __forceinline__ void rb_conv1_strip(const __bf16* in_t, __bf16* mid_t
                                                const float4* bias_s, int px, int kh, int x0, int y0, int H, int W) {
     const int f = px < 2 * RB_MH ? px : 2 * RB_MH - 1;
     const int mr = f >> 1, mc = RB_TW + (f & 1);
+    float4 bq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bq[q] = bias_s[2 * q + kh];
+    __builtin_amdgcn_sched_group_barrier(0x100, 4, 0);  // their own group, ahead of the pipeline's
     const __bf16* src = in_t + (mr * RB_IW + mc) * RB_PS + 8 * kh;
     // two accumulation chains (even and odd k steps) so that consecutive
     // MFMAs do not depend on each other
     f32x16 ac2[2] = {(f32x16)0.f, (f32x16)0.f};
-#pragma unroll
-    for (int s = 0; s < 18; ++s) {
-        const int tap = s >> 1;
-        const bf16x8 X =
-            *reinterpret_cast<const bf16x8*>(src + ((tap / 3) * RB_IW + tap % 3) * RB_PS + 16 * (s & 1));
-        ac2[s & 1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(Bf[s], X, ac2[s & 1], 0, 0, 0);
-    }
+    bf16x8 X[RB_C1_AHEAD + 1];
+    rb_strip_pipe<0>(src, X, ac2, Bf);
     const f32x16 acc = ac2[0] + ac2[1];
     if (px >= 2 * RB_MH) return;
     const int yy = y0 - 1 + mr, xx = x0 - 1 + mc;
-    const bool inside = yy >= 0 && yy < H && xx >= 0 && xx < W;
-    __bf16* dst = mid_t + (mr * RB_MW + mc) * RB_PS + 4 * kh;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const float4 bq = bias_s[2 * q + kh];
-        bf16x4 o;
-        o[0] = (__bf16)(inside ? fmaxf(acc[4 * q + 0] + bq.x, 0.f) : 0.f);
-        o[1] = (__bf16)(inside ? fmaxf(acc[4 * q + 1] + bq.y, 0.f) : 0.f);
-        o[2] = (__bf16)(inside ? fmaxf(acc[4 * q + 2] + bq.z, 0.f) : 0.f);
-        o[3] = (__bf16)(inside ? fmaxf(acc[4 * q + 3] + bq.w, 0.f) : 0.f);
-        *reinterpret_cast<bf16x4*>(dst + 8 * q) = o;
-    }
+    const unsigned keep = yy >= 0 && yy < H && xx >= 0 && xx < W ? ~0u : 0u;
+    rb_relu_store(acc, bq, keep, mid_t + (mr * RB_MW + mc) * RB_PS + 4 * kh);
 }
 
 // Outputs of one wave for one tile (2 rows x 4 runs of 4 channels, BF16)
@@ -717,6 +755,51 @@ __device__ __forceinline__ void rb_conv2_rows(const __bf16* in_t, const __bf16* 
     }
 }
 
+// Head convolution of the head form: groups g = wave, wave + 4, ... of 32
+// staged pixels p = 32 g + px (row p / 36, column p % 36 of the 12 x 36
+// region); lane half kh holds k = 16 s + 8 kh + e, i.e. taps 8 s + 4 kh ..
+// + 3 (two channels each, one 4-byte read per tap; taps beyond 8 are zero).
+__device__ __forceinline__ void rb_head(const unsigned* x_t, __bf16* in_t, const bf16x8 (&B0)[2],
+                                        const float4* bias0, int wave, int px, int kh, int x0, int y0, int H,
+                                        int W) {
+    float4 bq[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) bq[q] = bias0[2 * q + kh];
+#pragma unroll
+    for (int gi = 0; gi < (RB_HG + 3) / 4; ++gi) {
+        const int g = wave + 4 * gi;
+        if (g >= RB_HG) break;  // uniform per wave
+        const int p = 32 * g + px;
+        const int pc = p < RB_IH * RB_IW ? p : RB_IH * RB_IW - 1;
+        const int r = pc / RB_IW, c = pc % RB_IW;
+        const unsigned* src = x_t + r * RB_XW + c;
+        u32x4 v0, v1 = {0u, 0u, 0u, 0u};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int tap = 4 * kh + e;  // k step 0: taps 0..7
+            v0[e] = src[(tap / 3) * RB_XW + tap % 3];
+        }
+        if (kh == 0) v1[0] = src[2 * RB_XW + 2];  // k step 1: tap 8 (lane half 0 only)
+        f32x16 acc = (f32x16)0.f;
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(B0[0], __builtin_bit_cast(bf16x8, v0), acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(B0[1], __builtin_bit_cast(bf16x8, v1), acc, 0, 0, 0);
+        const int yy = y0 - 2 + r, xx = x0 - 2 + c;
+        const unsigned keep = p < RB_IH * RB_IW && yy >= 0 && yy < H && xx >= 0 && xx < W ? ~0u : 0u;
+        __bf16* dst = in_t + p * RB_PS + 4 * kh;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            // bias, then + 0 as the separate layer adds its (absent) residual
+            const f32x2 lo = {(acc[4 * q + 0] + bq[q].x) + 0.f, (acc[4 * q + 1] + bq[q].y) + 0.f};
+            const f32x2 hi = {(acc[4 * q + 2] + bq[q].z) + 0.f, (acc[4 * q + 3] + bq[q].w) + 0.f};
+            u32x2 o;
+            o[0] = __builtin_bit_cast(unsigned, __builtin_convertvector(lo, bf16x2)) & keep;
+            o[1] = __builtin_bit_cast(unsigned, __builtin_convertvector(hi, bf16x2)) & keep;
+            if (p < RB_IH * RB_IW) *reinterpret_cast<u32x2*>(dst + 8 * q) = o;
+        }
+    }
+}
+
+template <bool HEAD>
 __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
     constexpr int KSTEPS = ksteps_c(32);
     // staged input plus room for the unused tail of the last 16-byte pieces
@@ -724,7 +807,7 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
     __shared__ __attribute__((aligned(16))) __bf16 in_t[(RB_NPRE * 256 / 4) * RB_PS];
     __shared__ __attribute__((aligned(16))) __bf16 mid_t[RB_MID * RB_PS];
     __shared__ bf16x8 w2_s[KSTEPS * 64];
-    __shared__ float4 bias_s[16];  // b1, b2
+    __shared__ float4 bias_s[24];  // b1, b2, b0 (head form)
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar branches
     // conv1's fragments in registers, conv2's in LDS, loaded once for every
@@ -733,20 +816,40 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
 #pragma unroll
     for (int s = 0; s < KSTEPS; ++s) Bf[s] = a.w1[s * 64 + lane];
     for (int i = threadIdx.x; i < KSTEPS * 64; i += 256) w2_s[i] = a.w2[i];
-    if (threadIdx.x < 16)
-        bias_s[threadIdx.x] = reinterpret_cast<const float4*>(threadIdx.x < 8 ? a.b1 : a.b2)[threadIdx.x & 7];
+    if (threadIdx.x < (HEAD ? 24 : 16))
+        bias_s[threadIdx.x] =
+            reinterpret_cast<const float4*>(threadIdx.x < 8 ? a.b1 : threadIdx.x < 16 ? a.b2 : a.b0)[threadIdx.x & 7];
+    bf16x8 B0[2];  // head fragments (head form)
+    if constexpr (HEAD) {
+        B0[0] = a.w0[lane];
+        B0[1] = a.w0[64 + lane];
+    }
     const int tiles_x = (a.W + RB_TW - 1) / RB_TW, tiles_y = (a.H + RB_TH - 1) / RB_TH;
     const int ntiles = tiles_x * tiles_y * a.B;
     // The staged input of the next tile (12 x 36 pixels x 64 bytes, 16-byte
     // pieces i = threadIdx.x + 256 n) is loaded into registers while the
     // current tile computes; the epilogues read only LDS, so no wait on
     // global loads falls inside the convolutions.
-    constexpr int NPRE = RB_NPRE;
-    bf16x8 pre[NPRE];
+    constexpr int NPRE = HEAD ? 0 : RB_NPRE;
+    constexpr int NPX = HEAD ? RB_NPX : 0;
+    bf16x8 pre[NPRE > 0 ? NPRE : 1];
+    u32x2 prex[NPX > 0 ? NPX : 1];
     const unsigned img_bytes = (unsigned)a.H * (unsigned)a.W * 64u;  // < 2^31 (checked by the launcher)
     auto fetch = [&](int tile) {
         const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
         const int x0 = tx * RB_TW, y0 = ty * RB_TH;
+        if constexpr (HEAD) {
+            const __amdgpu_buffer_rsrc_t rx = oph_buf_rsrc(a.x + (size_t)b * a.H * a.W * 2, img_bytes / 8);
+#pragma unroll
+            for (int n = 0; n < NPX; ++n) {
+                const int i = threadIdx.x + 256 * n;
+                const int yy = y0 + i / RB_XW - 3, xx = x0 + i % RB_XW - 3;
+                const bool ok = i < RB_XH * RB_XW && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+                const unsigned off = ok ? (unsigned)((yy * a.W + xx) * 8) : OPH_BUF_OOB;
+                prex[n] = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rx, off, 0, 0));
+            }
+            return;
+        }
         const __amdgpu_buffer_rsrc_t rs = oph_buf_rsrc(a.h + (size_t)b * a.H * a.W * 32, img_bytes);
 #pragma unroll
         for (int n = 0; n < NPRE; ++n) {
@@ -765,7 +868,11 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
     RbOut pend;
     __amdgpu_buffer_rsrc_t pend_img = oph_buf_rsrc(a.out, img_bytes);
     if ((int)blockIdx.x < ntiles) fetch(blockIdx.x);
-    for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    // the grid stride kept in a register (otherwise it is reloaded from the
+    // dispatch packet every tile, with a wait that also drains LDS)
+    int gstep = (int)gridDim.x;
+    asm volatile("" : "+s"(gstep));
+    for (int tile = blockIdx.x; tile < ntiles; tile += gstep) {
         const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
         const int x0 = tx * RB_TW, y0 = ty * RB_TH;
         const size_t img = (size_t)b * a.H * a.W;
@@ -776,14 +883,30 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
         asm volatile("" : "+v"(lane_o), "+s"(wave_o));
         const int px = lane_o & 31, kh = lane_o >> 5;
         __syncthreads();  // the previous tile's LDS reads are done
+        // head form: the staged input (BF16 pairs) in the mid tile's space,
+        // which is free until conv1
+        unsigned* x_t = reinterpret_cast<unsigned*>(mid_t);
+        if constexpr (HEAD) {
 #pragma unroll
-        for (int n = 0; n < NPRE; ++n) {
-            const int i = threadIdx.x + 256 * n;
-            *reinterpret_cast<bf16x8*>(in_t + (i >> 2) * RB_PS + 8 * (i & 3)) = pre[n];
+            for (int n = 0; n < NPX; ++n) {
+                const int i = threadIdx.x + 256 * n;
+                const f32x2 v = __builtin_bit_cast(f32x2, prex[n]);
+                if (i < RB_XH * RB_XW) x_t[i] = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < NPRE; ++n) {
+                const int i = threadIdx.x + 256 * n;
+                *reinterpret_cast<bf16x8*>(in_t + (i >> 2) * RB_PS + 8 * (i & 3)) = pre[n];
+            }
         }
         __syncthreads();
         if (tile != (int)blockIdx.x) rb_store(pend, pend_img);
-        if (tile + (int)gridDim.x < ntiles) fetch(tile + gridDim.x);
+        if (tile + gstep < ntiles) fetch(tile + gstep);
+        if constexpr (HEAD) {
+            rb_head(x_t, in_t, B0, bias_s + 16, wave_o, px, kh, x0, y0, a.H, a.W);
+            __syncthreads();
+        }
 
         // conv1 + bias + relu on the 10 x 34 halo tile: rows of 32 (mid
         // columns 0..31) as 3, 3, 2, 2 consecutive rows per wave, and the
@@ -958,7 +1081,9 @@ int forward_bf16(oph_ctx* ctx, oph_cnn* net, const float* x, float* y, int B, in
     int rc;
     if (net->wf_rev != net->params_rev && (rc = pack(ctx, net, false))) return rc;
     auto h = [&](int b) { return reinterpret_cast<__bf16*>(A.h[b]); };
-    if ((rc = conv_fwd<2, 32, float, __bf16>(ctx, x, WF + F.head(), P + L.head_b(), nullptr, h(0), B, H, W, 0)))
+    // with blocks, the first one computes the head on its staged tiles
+    if (net->blocks == 0 &&
+        (rc = conv_fwd<2, 32, float, __bf16>(ctx, x, WF + F.head(), P + L.head_b(), nullptr, h(0), B, H, W, 0)))
         return rc;
     // persistent: two workgroups per CU (LDS-bound), each walking tiles
     const int64_t ntiles = (int64_t)((W + RB_TW - 1) / RB_TW) * ((H + RB_TH - 1) / RB_TH) * B;
@@ -969,8 +1094,12 @@ int forward_bf16(oph_ctx* ctx, oph_cnn* net, const float* x, float* y, int B, in
         return oph_fail(ctx, OPH_ENOTSUP, "image %dx%d too large for the BF16 inference path (< 2^25 pixels)", H, W);
     const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 2LL * std::max(ctx->cu_count, 1));
     for (int b = 0; b < net->blocks; ++b) {
-        const ResArgs r{h(b), h(b + 1), WF + F.conv1(b), WF + F.conv2(b), P + L.b1(b), P + L.b2(b), B, H, W};
-        OPH_LAUNCH(ctx, "resblock_fwd", resblock_fwd, dim3(grid), dim3(256), 0, r);
+        const ResArgs r{h(b), h(b + 1), WF + F.conv1(b), WF + F.conv2(b), P + L.b1(b), P + L.b2(b), B, H, W,
+                        x, WF + F.head(), P + L.head_b()};
+        if (b == 0)
+            OPH_LAUNCH(ctx, "resblock_fwd", resblock_fwd<true>, dim3(grid), dim3(256), 0, r);
+        else
+            OPH_LAUNCH(ctx, "resblock_fwd", resblock_fwd<false>, dim3(grid), dim3(256), 0, r);
     }
     return conv_fwd<32, 2, __bf16, float>(ctx, h(net->blocks), WF + F.tail(), P + L.tail_b(), x, y, B, H, W, 0);
 }
