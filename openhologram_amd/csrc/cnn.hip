@@ -925,6 +925,105 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
     if ((int)blockIdx.x < ntiles) rb_store(pend, pend_img);
 }
 
+// ---- tail convolution (inference, BF16 input) ----------------------------------
+// y = x + conv_tail(h), 32 -> 2 channels.  A persistent workgroup stages h
+// on 8 x 32 tiles with a one-pixel halo (10 x 34 pixels: the geometry of the
+// residual block's mid tile) through prefetch registers and runs the block's
+// conv2 pipeline on it with the tail's fragments in LDS; lanes of the first
+// half hold output channels 0 and 1.  Same fragments, MFMA order and
+// rounding as conv3x3_fwd<32, 2> (whose results it reproduces bit for bit),
+// but the staging of the next tile overlaps the current one's MFMAs.
+constexpr int TL_NPRE = (RB_MH * RB_MW * 4 + 255) / 256;  // 16-byte pieces per thread (6)
+
+struct TailArgs {
+    const __bf16* h;   // [B][H][W][32]
+    const float* x;    // [B][H][W][2], the global skip
+    float* y;          // [B][H][W][2]
+    const bf16x8* w;   // tail fragments (pack_weights)
+    const float* b;    // [2]
+    int B, H, W;
+};
+
+__global__ __launch_bounds__(256, 2) void tail_fwd(TailArgs a) {
+    constexpr int KSTEPS = ksteps_c(32);
+    __shared__ __attribute__((aligned(16))) __bf16 in_t[(TL_NPRE * 256 / 4) * RB_PS];
+    __shared__ bf16x8 w_s[KSTEPS * 64];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int i = threadIdx.x; i < KSTEPS * 64; i += 256) w_s[i] = a.w[i];
+    const float bias0 = a.b[0], bias1 = a.b[1];
+    const int tiles_x = (a.W + RB_TW - 1) / RB_TW, tiles_y = (a.H + RB_TH - 1) / RB_TH;
+    const int ntiles = tiles_x * tiles_y * a.B;
+    const unsigned img_bytes = (unsigned)a.H * (unsigned)a.W * 64u;  // < 2^31 (checked by the launcher)
+    const unsigned xy_bytes = img_bytes / 8;
+    bf16x8 pre[TL_NPRE];
+    auto fetch = [&](int tile) {
+        const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
+        const int x0 = tx * RB_TW, y0 = ty * RB_TH;
+        const __amdgpu_buffer_rsrc_t rs = oph_buf_rsrc(a.h + (size_t)b * a.H * a.W * 32, img_bytes);
+#pragma unroll
+        for (int n = 0; n < TL_NPRE; ++n) {
+            const int i = threadIdx.x + 256 * n;
+            const int pix = i >> 2, q = i & 3;
+            const int yy = y0 + pix / RB_MW - 1, xx = x0 + pix % RB_MW - 1;
+            const bool ok = i < RB_MH * RB_MW * 4 && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+            const unsigned off = ok ? (unsigned)((yy * a.W + xx) * 64 + 16 * q) : OPH_BUF_OOB;
+            pre[n] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+        }
+    };
+    // outputs of the previous tile, stored after the next staging (as in
+    // resblock_fwd: the prefetch stays the newest memory operation)
+    u32x2 pv[2] = {{0u, 0u}, {0u, 0u}};
+    unsigned pdst[2] = {OPH_BUF_OOB, OPH_BUF_OOB};
+    __amdgpu_buffer_rsrc_t py = oph_buf_rsrc(a.y, xy_bytes);
+    if ((int)blockIdx.x < ntiles) fetch(blockIdx.x);
+    int gstep = (int)gridDim.x;
+    asm volatile("" : "+s"(gstep));
+    for (int tile = blockIdx.x; tile < ntiles; tile += gstep) {
+        const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
+        const int x0 = tx * RB_TW, y0 = ty * RB_TH;
+        const size_t img = (size_t)b * a.H * a.W;
+        int lane_o = lane, wave_o = wave;
+        asm volatile("" : "+v"(lane_o), "+s"(wave_o));
+        const int px = lane_o & 31, kh = lane_o >> 5;
+        __syncthreads();  // the previous tile's LDS reads are done
+#pragma unroll
+        for (int n = 0; n < TL_NPRE; ++n) {
+            const int i = threadIdx.x + 256 * n;
+            *reinterpret_cast<bf16x8*>(in_t + (i >> 2) * RB_PS + 8 * (i & 3)) = pre[n];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < 2; ++j) __builtin_amdgcn_raw_buffer_store_b64(pv[j], py, pdst[j], 0, 0);
+        // the skip input of this tile's two rows (lane half 0: channels 0, 1)
+        const int r0 = 2 * wave_o, x = x0 + px;
+        const __amdgpu_buffer_rsrc_t rx = oph_buf_rsrc(a.x + img * 2, xy_bytes);
+        unsigned dst[2];
+        u32x2 xv[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int y = y0 + r0 + j;
+            dst[j] = kh == 0 && y < a.H && x < a.W ? (unsigned)((y * a.W + x) * 8) : OPH_BUF_OOB;
+            xv[j] = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rx, dst[j], 0, 0));
+        }
+        if (tile + gstep < ntiles) fetch(tile + gstep);
+
+        f32x16 acc[2] = {(f32x16)0.f, (f32x16)0.f};
+        bf16x8 Xr[4][6], Wr[RB_C2_AHEAD + 1];
+        rb_c2_pipe<0>(in_t + (r0 * RB_MW + px) * RB_PS + 8 * kh, w_s + lane_o, Xr, Wr, acc);
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const f32x2 xs = __builtin_bit_cast(f32x2, xv[j]);
+            const f32x2 o = {(acc[j][0] + bias0) + xs[0], (acc[j][1] + bias1) + xs[1]};
+            pv[j] = __builtin_bit_cast(u32x2, o);
+            pdst[j] = dst[j];
+        }
+        py = oph_buf_rsrc(a.y + img * 2, xy_bytes);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) __builtin_amdgcn_raw_buffer_store_b64(pv[j], py, pdst[j], 0, 0);
+}
+
 // ---- host helpers ------------------------------------------------------------------
 
 template <int CIN, int COUT, typename TI = float, typename TO = float>
@@ -1101,7 +1200,11 @@ int forward_bf16(oph_ctx* ctx, oph_cnn* net, const float* x, float* y, int B, in
         else
             OPH_LAUNCH(ctx, "resblock_fwd", resblock_fwd<false>, dim3(grid), dim3(256), 0, r);
     }
-    return conv_fwd<32, 2, __bf16, float>(ctx, h(net->blocks), WF + F.tail(), P + L.tail_b(), x, y, B, H, W, 0);
+    // three workgroups per CU fit (49 KB of LDS, 142 registers)
+    const TailArgs t{h(net->blocks), x, y, WF + F.tail(), P + L.tail_b(), B, H, W};
+    const unsigned tgrid = (unsigned)std::min<int64_t>(ntiles, 3LL * std::max(ctx->cu_count, 1));
+    OPH_LAUNCH(ctx, "tail_fwd", tail_fwd, dim3(tgrid), dim3(256), 0, t);
+    return OPH_OK;
 }
 
 // gradients of loss = mean (y - t)^2 into net->grads; loss into *loss
