@@ -593,11 +593,17 @@ __device__ __forceinline__ void rb_c1_pipe(const __bf16* src, bf16x8 (&X)[RB_C1_
 
 // conv2 step S = 6 dy + k: weight fragment S, operands of mid rows r0 + dy
 // (read at step S - 6, or here for dy = 0) and r0 + dy + 1 (read here)
-template <int S>
+// `late` issues the epilogue's NL LDS reads once the operand reads are done,
+// so that they complete under the last steps' MFMAs.
+template <int S, int NL, typename F>
 __device__ __forceinline__ void rb_c2_pipe(const __bf16* src, const bf16x8* w2l, bf16x8 (&Xr)[4][6],
-                                           bf16x8 (&Wr)[RB_C2_AHEAD + 1], f32x16 (&acc)[2]) {
+                                           bf16x8 (&Wr)[RB_C2_AHEAD + 1], f32x16 (&acc)[2], F& late) {
     constexpr int NS = 18, AH = RB_C2_AHEAD;
     if constexpr (S < NS + AH) {
+        if constexpr (S == NS && NL > 0) {
+            late();
+            __builtin_amdgcn_sched_group_barrier(0x100, NL, 0);
+        }
         if constexpr (S < NS) {
             constexpr int dy = S / 6, k = S % 6, dx = k >> 1, half = k & 1;
             Wr[S % (AH + 1)] = w2l[S * 64];
@@ -612,7 +618,7 @@ __device__ __forceinline__ void rb_c2_pipe(const __bf16* src, const bf16x8* w2l,
         }
         if constexpr (S < NS) __builtin_amdgcn_sched_group_barrier(0x100, S < 6 ? 3 : 2, 0);
         if constexpr (U >= 0) __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
-        rb_c2_pipe<S + 1>(src, w2l, Xr, Wr, acc);
+        rb_c2_pipe<S + 1, NL>(src, w2l, Xr, Wr, acc, late);
     }
 }
 
@@ -732,7 +738,19 @@ __device__ __forceinline__ void rb_conv2_rows(const __bf16* in_t, const __bf16* 
     f32x16 acc[2] = {(f32x16)0.f, (f32x16)0.f};
     const __bf16* src = mid_t + (r0 * RB_MW + px) * RB_PS + 8 * kh;
     bf16x8 Xr[4][6], Wr[RB_C2_AHEAD + 1];
-    rb_c2_pipe<0>(src, w2_s + lane, Xr, Wr, acc);
+    // bias and residual, read under the last MFMAs
+    float4 bq[4];
+    bf16x4 rv[2][4];
+    const __bf16* res = in_t + ((r0 + 2) * RB_IW + px + 2) * RB_PS + 4 * kh;
+    auto late = [&]() {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bq[q] = bias_s[8 + 2 * q + kh];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) rv[j][q] = *reinterpret_cast<const bf16x4*>(res + j * RB_IW * RB_PS + 8 * q);
+    };
+    rb_c2_pipe<0, 12>(src, w2_s + lane, Xr, Wr, acc, late);
     const int x = x0 + px;
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -740,16 +758,13 @@ __device__ __forceinline__ void rb_conv2_rows(const __bf16* in_t, const __bf16* 
         const int y = y0 + r;
         // pixels outside the image: offset out of range, store dropped
         out.dst[j] = y < H && x < W ? (unsigned)((y * W + x) * 64 + 8 * kh) : OPH_BUF_OOB;
-        const __bf16* res = in_t + ((r + 2) * RB_IW + px + 2) * RB_PS + 4 * kh;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const float4 bq = bias_s[8 + 2 * q + kh];
-            const bf16x4 rv = *reinterpret_cast<const bf16x4*>(res + 8 * q);
             bf16x4 o;
-            o[0] = (__bf16)(acc[j][4 * q + 0] + bq.x + (float)rv[0]);
-            o[1] = (__bf16)(acc[j][4 * q + 1] + bq.y + (float)rv[1]);
-            o[2] = (__bf16)(acc[j][4 * q + 2] + bq.z + (float)rv[2]);
-            o[3] = (__bf16)(acc[j][4 * q + 3] + bq.w + (float)rv[3]);
+            o[0] = (__bf16)(acc[j][4 * q + 0] + bq[q].x + (float)rv[j][q][0]);
+            o[1] = (__bf16)(acc[j][4 * q + 1] + bq[q].y + (float)rv[j][q][1]);
+            o[2] = (__bf16)(acc[j][4 * q + 2] + bq[q].z + (float)rv[j][q][2]);
+            o[3] = (__bf16)(acc[j][4 * q + 3] + bq[q].w + (float)rv[j][q][3]);
             out.v[j][q] = __builtin_bit_cast(u32x2, o);
         }
     }
@@ -1010,7 +1025,8 @@ __global__ __launch_bounds__(256, 2) void tail_fwd(TailArgs a) {
 
         f32x16 acc[2] = {(f32x16)0.f, (f32x16)0.f};
         bf16x8 Xr[4][6], Wr[RB_C2_AHEAD + 1];
-        rb_c2_pipe<0>(in_t + (r0 * RB_MW + px) * RB_PS + 8 * kh, w_s + lane_o, Xr, Wr, acc);
+        auto none = []() {};
+        rb_c2_pipe<0, 0>(in_t + (r0 * RB_MW + px) * RB_PS + 8 * kh, w_s + lane_o, Xr, Wr, acc, none);
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
             const f32x2 xs = __builtin_bit_cast(f32x2, xv[j]);
