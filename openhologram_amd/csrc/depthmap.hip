@@ -608,12 +608,25 @@ __global__ __launch_bounds__(B * (LY / 16)) void dm_final(DmArgs a, const float2
     const int b = threadIdx.x / TPL, t = threadIdx.x % TPL;
     fft::Twiddles<LY> Tw;
     Tw.load(twy, t);
-    for (int e = threadIdx.x; e < LY * B; e += nthr) {
-        const int bb = e % B, ky = e / B;
-        const int x = c0 + bb;
-        const int yc = (ky + a.Ny / 2) & (a.Ny - 1);
-        const bool kept = !a.encode || (yc >= a.cy1 && yc <= a.cy2);
-        lds[bb * PL + fft::pad16(ky)] = (x < a.w && kept) ? a.T[(size_t)ky * a.w + x] : make_float2(0.f, 0.f);
+    // 16 elements per thread (LY B / nthr), all requested before any is
+    // stored (range-checked buffer loads: dropped rows and columns beyond
+    // the scene read zeros; T is at most 8192 x 8192 x 8 bytes)
+    {
+        const auto rt = oph_buf_rsrc(a.T, (unsigned)((size_t)a.Ny * a.w * sizeof(float2)));
+        float2 v[16];
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int e = threadIdx.x + m * nthr;
+            const int ky = e / B, x = c0 + e % B;
+            const int yc = (ky + a.Ny / 2) & (a.Ny - 1);
+            const bool kept = !a.encode || (yc >= a.cy1 && yc <= a.cy2);
+            v[m] = oph_buf_load_f2(rt, x < a.w && kept ? ((unsigned)ky * (unsigned)a.w + (unsigned)x) * 8u : OPH_BUF_OOB);
+        }
+#pragma unroll
+        for (int m = 0; m < 16; ++m) {
+            const int e = threadIdx.x + m * nthr;
+            lds[(e % B) * PL + fft::pad16(e / B)] = v[m];
+        }
     }
     __syncthreads();
     fft::line<LY, +1>(lds + b * PL, Tw, t);
