@@ -32,6 +32,7 @@ typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
 
 
 struct oph_cnn {
@@ -624,17 +625,24 @@ __device__ __forceinline__ void rb_c2_pipe(const __bf16* src, const bf16x8* w2l,
 
 // relu(acc + bias) of one lane's 16 channels (4 runs of 4) as BF16, all
 // zero when keep is 0 (pixels outside the image: conv2's zero padding).
-// Two values per conversion, and a mask instead of a branch: the bias is
-// already in registers, so nothing here waits on LDS.
+// Two values per conversion; the relu is taken on the BF16 pair as a signed
+// 16-bit maximum with 0 (a negative BF16, -0 included, has the sign bit
+// set), which equals rounding relu(x) and is one instruction per pair; a
+// mask instead of a branch; the bias is already in registers, so nothing
+// here waits on LDS.
+__device__ __forceinline__ unsigned rb_relu_bf16x2(f32x2 v) {
+    const i16x2 h = __builtin_bit_cast(i16x2, __builtin_convertvector(v, bf16x2));
+    return __builtin_bit_cast(unsigned, __builtin_elementwise_max(h, (i16x2)0));
+}
 __device__ __forceinline__ void rb_relu_store(const f32x16& acc, const float4 (&bq)[4], unsigned keep,
                                               __bf16* dst) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const f32x2 lo = {fmaxf(acc[4 * q + 0] + bq[q].x, 0.f), fmaxf(acc[4 * q + 1] + bq[q].y, 0.f)};
-        const f32x2 hi = {fmaxf(acc[4 * q + 2] + bq[q].z, 0.f), fmaxf(acc[4 * q + 3] + bq[q].w, 0.f)};
+        const f32x2 lo = {acc[4 * q + 0] + bq[q].x, acc[4 * q + 1] + bq[q].y};
+        const f32x2 hi = {acc[4 * q + 2] + bq[q].z, acc[4 * q + 3] + bq[q].w};
         u32x2 o;
-        o[0] = __builtin_bit_cast(unsigned, __builtin_convertvector(lo, bf16x2)) & keep;
-        o[1] = __builtin_bit_cast(unsigned, __builtin_convertvector(hi, bf16x2)) & keep;
+        o[0] = rb_relu_bf16x2(lo) & keep;
+        o[1] = rb_relu_bf16x2(hi) & keep;
         *reinterpret_cast<u32x2*>(dst + 8 * q) = o;
     }
 }
