@@ -548,7 +548,7 @@ constexpr int RB_HG = (RB_IH * RB_IW + 31) / 32;       // head groups of 32 pixe
 // LDS operands of step S are read at step S and consumed by the MFMAs at
 // step S + AHEAD, and sched_group_barrier keeps that order (masks: 0x100 LDS
 // read, 0x008 MFMA).
-constexpr int RB_C1_AHEAD = 4, RB_C2_AHEAD = 3;
+constexpr int RB_C1_AHEAD = 6, RB_C2_AHEAD = 3;
 
 // conv1 steps: S = (NR + 2) k + i takes operand k = 2 dx + half of input
 // row rb_c1_row<NR>(i); the row order makes consecutive MFMAs alternate
@@ -948,6 +948,187 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
     if ((int)blockIdx.x < ntiles) rb_store(pend, pend_img);
 }
 
+// ---- pipelined residual block: two wave groups ----------------------------------
+// The same block as resblock_fwd, one workgroup of 8 waves per CU with the
+// work split by phase instead of by tile: waves 0-3 (group A) compute
+// relu(conv1) of tile i + 1 into one mid buffer and stage tile i + 2's input
+// (prefetched into registers one tile earlier) into one input buffer, while
+// waves 4-7 (group B) compute conv2 of tile i from the other mid buffer and
+// store it, its residual read into registers one tile earlier; one barrier
+// per tile.  Each group keeps only its own convolution's weight fragments,
+// so both sets live in registers and no weight passes through LDS (in
+// resblock_fwd conv2's fragments are read from LDS for every tile, a fifth
+// of the block's LDS reads).  LDS: two staged inputs and two mid tiles
+// (123 KB).  The arithmetic (fragments, k order, rounding) is
+// resblock_fwd's.
+constexpr int RP_IN = (RB_NPRE * 256 / 4) * RB_PS;  // bf16 per staged-input buffer
+constexpr int RP_MID = RB_MID * RB_PS;              // bf16 per mid buffer
+
+__device__ __forceinline__ void rb_read_res(const __bf16* in_t, int r0, int px, int kh, bf16x4 (&rv)[2][4]) {
+    const __bf16* res = in_t + ((r0 + 2) * RB_IW + px + 2) * RB_PS + 4 * kh;
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) rv[j][q] = *reinterpret_cast<const bf16x4*>(res + j * RB_IW * RB_PS + 8 * q);
+}
+
+// conv2 pipeline with the weight fragments in registers (rb_c2_pipe's
+// operand order and MFMA order)
+template <int S>
+__device__ __forceinline__ void rb_c2r_pipe(const __bf16* src, const bf16x8 (&W)[18], bf16x8 (&Xr)[4][6],
+                                            f32x16 (&acc)[2]) {
+    constexpr int NS = 18, AH = 6;
+    if constexpr (S < NS + AH) {
+        if constexpr (S < NS) {
+            constexpr int dy = S / 6, k = S % 6, dx = k >> 1, half = k & 1;
+            if constexpr (dy == 0) Xr[0][k] = *reinterpret_cast<const bf16x8*>(src + dx * RB_PS + 16 * half);
+            Xr[dy + 1][k] = *reinterpret_cast<const bf16x8*>(src + ((dy + 1) * RB_MW + dx) * RB_PS + 16 * half);
+        }
+        constexpr int U = S - AH;
+        if constexpr (U >= 0) {
+            constexpr int dy = U / 6, k = U % 6;
+            acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[U], Xr[dy][k], acc[0], 0, 0, 0);
+            acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(W[U], Xr[dy + 1][k], acc[1], 0, 0, 0);
+        }
+        if constexpr (S < NS) __builtin_amdgcn_sched_group_barrier(0x100, S < 6 ? 2 : 1, 0);
+        if constexpr (U >= 0) __builtin_amdgcn_sched_group_barrier(0x008, 2, 0);
+        rb_c2r_pipe<S + 1>(src, W, Xr, acc);
+    }
+}
+
+// conv2 + bias + residual (from registers) for output rows r0, r0 + 1
+__device__ __forceinline__ void rb_conv2_regs(const __bf16* mid_t, const bf16x8 (&W)[18], const float4 (&bq)[4],
+                                              const bf16x4 (&rv)[2][4], RbOut& out, int r0, int lane, int x0,
+                                              int y0, int H, int W_) {
+    const int px = lane & 31, kh = lane >> 5;
+    f32x16 acc[2] = {(f32x16)0.f, (f32x16)0.f};
+    const __bf16* src = mid_t + (r0 * RB_MW + px) * RB_PS + 8 * kh;
+    bf16x8 Xr[4][6];
+    rb_c2r_pipe<0>(src, W, Xr, acc);
+    const int x = x0 + px;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int y = y0 + r0 + j;
+        out.dst[j] = y < H && x < W_ ? (unsigned)((y * W_ + x) * 64 + 8 * kh) : OPH_BUF_OOB;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            bf16x4 o;
+            o[0] = (__bf16)(acc[j][4 * q + 0] + bq[q].x + (float)rv[j][q][0]);
+            o[1] = (__bf16)(acc[j][4 * q + 1] + bq[q].y + (float)rv[j][q][1]);
+            o[2] = (__bf16)(acc[j][4 * q + 2] + bq[q].z + (float)rv[j][q][2]);
+            o[3] = (__bf16)(acc[j][4 * q + 3] + bq[q].w + (float)rv[j][q][3]);
+            out.v[j][q] = __builtin_bit_cast(u32x2, o);
+        }
+    }
+}
+
+__global__ __launch_bounds__(512, 1) void resblock_pipe(ResArgs a) {
+    constexpr int KSTEPS = ksteps_c(32);
+    __shared__ __attribute__((aligned(16))) __bf16 in_t[2 * RP_IN];
+    __shared__ __attribute__((aligned(16))) __bf16 mid_t[2 * RP_MID];
+    __shared__ float4 bias_s[8];  // b1 (conv1's epilogue reads it from LDS)
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (threadIdx.x < 8) bias_s[threadIdx.x] = reinterpret_cast<const float4*>(a.b1)[threadIdx.x];
+    const int tiles_x = (a.W + RB_TW - 1) / RB_TW, tiles_y = (a.H + RB_TH - 1) / RB_TH;
+    const int ntiles = tiles_x * tiles_y * a.B;
+    int gstep = (int)gridDim.x;
+    asm volatile("" : "+s"(gstep));
+    const int g0 = blockIdx.x;
+    // this workgroup's tiles: g0 + k gstep, k < nmy
+    const int nmy = g0 < ntiles ? (ntiles - 1 - g0) / gstep + 1 : 0;
+    const unsigned img_bytes = (unsigned)a.H * (unsigned)a.W * 64u;  // < 2^31 (checked by the launcher)
+    auto tile_of = [&](int k, int& x0, int& y0, int& b) {
+        const int tile = g0 + k * gstep;
+        x0 = (tile % tiles_x) * RB_TW;
+        y0 = ((tile / tiles_x) % tiles_y) * RB_TH;
+        b = tile / (tiles_x * tiles_y);
+    };
+    if (wave < 4) {
+        // group A: conv1 and the staging (threads 0..255, as in resblock_fwd)
+        bf16x8 Bf[KSTEPS];
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) Bf[s] = a.w1[s * 64 + lane];
+        bf16x8 pre[RB_NPRE];
+        auto fetch = [&](int k) {
+            int x0, y0, b;
+            tile_of(k, x0, y0, b);
+            const __amdgpu_buffer_rsrc_t rs = oph_buf_rsrc(a.h + (size_t)b * a.H * a.W * 32, img_bytes);
+#pragma unroll
+            for (int n = 0; n < RB_NPRE; ++n) {
+                const int i = threadIdx.x + 256 * n;
+                const int pix = i >> 2, q = i & 3;
+                const int yy = y0 + pix / RB_IW - 2, xx = x0 + pix % RB_IW - 2;
+                const bool ok = i < RB_IH * RB_IW * 4 && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+                const unsigned off = ok ? (unsigned)((yy * a.W + xx) * 64 + 16 * q) : OPH_BUF_OOB;
+                pre[n] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+            }
+        };
+        auto stage = [&](int buf) {
+#pragma unroll
+            for (int n = 0; n < RB_NPRE; ++n) {
+                const int i = threadIdx.x + 256 * n;
+                *reinterpret_cast<bf16x8*>(in_t + buf * RP_IN + (i >> 2) * RB_PS + 8 * (i & 3)) = pre[n];
+            }
+        };
+        auto conv1 = [&](int k) {
+            int x0, y0, b;
+            tile_of(k, x0, y0, b);
+            int lane_o = lane, wave_o = wave;
+            asm volatile("" : "+v"(lane_o), "+s"(wave_o));
+            const int px = lane_o & 31, kh = lane_o >> 5;
+            const __bf16* it = in_t + (k & 1) * RP_IN;
+            __bf16* mt = mid_t + (k & 1) * RP_MID;
+            if (wave_o < 2)
+                rb_conv1_rows<3>(it, mt, Bf, bias_s, 3 * wave_o, px, kh, x0, y0, a.H, a.W);
+            else
+                rb_conv1_rows<2>(it, mt, Bf, bias_s, 6 + 2 * (wave_o - 2), px, kh, x0, y0, a.H, a.W);
+            if (wave_o == 3) rb_conv1_strip(it, mt, Bf, bias_s, px, kh, x0, y0, a.H, a.W);
+        };
+        if (nmy > 0) {
+            fetch(0);
+            stage(0);
+        }
+        if (nmy > 1) fetch(1);
+        __syncthreads();
+        if (nmy > 0) conv1(0);
+        if (nmy > 1) stage(1);
+        if (nmy > 2) fetch(2);
+        __syncthreads();
+        for (int k = 0; k < nmy; ++k) {
+            if (k + 1 < nmy) conv1(k + 1);
+            if (k + 2 < nmy) stage(k & 1);  // tile k + 2 into the buffer tile k used
+            if (k + 3 < nmy) fetch(k + 3);
+            __syncthreads();
+        }
+    } else {
+        // group B: conv2 (fragments and bias in registers), the residual
+        // one tile ahead, the stores
+        bf16x8 W2[KSTEPS];
+#pragma unroll
+        for (int s = 0; s < KSTEPS; ++s) W2[s] = a.w2[s * 64 + lane];
+        float4 bq[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bq[q] = reinterpret_cast<const float4*>(a.b2)[2 * q + (lane >> 5)];
+        bf16x4 rv[2][4];
+        const int r0 = 2 * (wave - 4);
+        __syncthreads();
+        if (nmy > 0) rb_read_res(in_t, r0, lane & 31, lane >> 5, rv);
+        __syncthreads();
+        for (int k = 0; k < nmy; ++k) {
+            int x0, y0, b;
+            tile_of(k, x0, y0, b);
+            int lane_o = lane;
+            asm volatile("" : "+v"(lane_o));
+            RbOut out;
+            rb_conv2_regs(mid_t + (k & 1) * RP_MID, W2, bq, rv, out, r0, lane_o, x0, y0, a.H, a.W);
+            rb_store(out, oph_buf_rsrc(a.out + (size_t)b * a.H * a.W * 32, img_bytes));
+            if (k + 1 < nmy) rb_read_res(in_t + ((k + 1) & 1) * RP_IN, r0, lane_o & 31, lane_o >> 5, rv);
+            __syncthreads();
+        }
+    }
+}
+
 // ---- tail convolution (inference, BF16 input) ----------------------------------
 // y = x + conv_tail(h), 32 -> 2 channels.  A persistent workgroup stages h
 // on 8 x 32 tiles with a one-pixel halo (10 x 34 pixels: the geometry of the
@@ -1216,13 +1397,14 @@ int forward_bf16(oph_ctx* ctx, oph_cnn* net, const float* x, float* y, int B, in
     if ((int64_t)H * W * 64 >= (1LL << 31))
         return oph_fail(ctx, OPH_ENOTSUP, "image %dx%d too large for the BF16 inference path (< 2^25 pixels)", H, W);
     const unsigned grid = (unsigned)std::min<int64_t>(ntiles, 2LL * std::max(ctx->cu_count, 1));
+    const unsigned pgrid = (unsigned)std::min<int64_t>(ntiles, std::max(ctx->cu_count, 1));  // one per CU
     for (int b = 0; b < net->blocks; ++b) {
         const ResArgs r{h(b), h(b + 1), WF + F.conv1(b), WF + F.conv2(b), P + L.b1(b), P + L.b2(b), B, H, W,
                         x, WF + F.head(), P + L.head_b()};
         if (b == 0)
             OPH_LAUNCH(ctx, "resblock_fwd", resblock_fwd<true>, dim3(grid), dim3(256), 0, r);
         else
-            OPH_LAUNCH(ctx, "resblock_fwd", resblock_fwd<false>, dim3(grid), dim3(256), 0, r);
+            OPH_LAUNCH(ctx, "resblock_fwd", resblock_pipe, dim3(pgrid), dim3(512), 0, r);
     }
     // three workgroups per CU fit (49 KB of LDS, 142 registers)
     const TailArgs t{h(net->blocks), x, y, WF + F.tail(), P + L.tail_b(), B, H, W};
