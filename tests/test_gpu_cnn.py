@@ -85,6 +85,15 @@ def test_cnn_bf16_activations_many_tiles(ctx):
     for b in range(2):
         for sl in (np.s_[:, :8, :], np.s_[:, -8:, :], np.s_[:, :, -8:]):
             assert _rel((y16 - x)[b][sl[1:]], (y32 - x)[b][sl[1:]]) < 2e-2
+    # 289 tiles: with one workgroup per CU most workgroups take one tile and
+    # some two (the pipelined blocks' shortest schedules), ragged both ways
+    x1 = rng.normal(0, 1, (1, 135, 513, 2)).astype(np.float32)
+    y32, y16 = net.forward(x1), net.forward(x1, bf16_acts=True)
+    e1 = _rel(y16 - x1, y32 - x1)
+    print("135 x 513: bf16-act forward rel vs float32 form", e1)
+    assert e1 < 1e-2
+    for sl in (np.s_[:8, :], np.s_[-8:, :], np.s_[:, -8:], np.s_[:, :8]):
+        assert _rel((y16 - x1)[0][sl], (y32 - x1)[0][sl]) < 2e-2
 
 
 def test_cnn_training_reduces_loss_and_persists(ctx, tmp_path):
