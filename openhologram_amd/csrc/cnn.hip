@@ -964,6 +964,28 @@ __global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
 constexpr int RP_IN = (RB_NPRE * 256 / 4) * RB_PS;  // bf16 per staged-input buffer
 constexpr int RP_MID = RB_MID * RB_PS;              // bf16 per mid buffer
 
+// Tiles of a persistent workgroup g of gstep: with gstep a multiple of 8,
+// each XCD (workgroups are placed on the 8 XCDs round-robin by index) walks
+// its own contiguous share of the tiles in row-major order with its
+// workgroups side by side, so that the halo columns neighbouring tiles share
+// and the halo rows shared with the previous tile row come from its own L2;
+// otherwise tiles g, g + gstep, ...  rb_count: how many; rb_tile: the k-th.
+struct RbSched {
+    int g, gstep, per, x, s, share, lim;
+    __device__ RbSched(int g_, int gstep_, int ntiles) : g(g_), gstep(gstep_) {
+        per = gstep % 8 == 0 ? gstep / 8 : 0;
+        x = g % 8;
+        s = g / 8;
+        share = (ntiles + 7) / 8;
+        lim = per ? min(share, ntiles - x * share) : ntiles;
+    }
+    __device__ int count() const {
+        if (!per) return g < lim ? (lim - 1 - g) / gstep + 1 : 0;
+        return lim > s ? (lim - 1 - s) / per + 1 : 0;
+    }
+    __device__ int tile(int k) const { return per ? x * share + s + k * per : g + k * gstep; }
+};
+
 __device__ __forceinline__ void rb_read_res(const __bf16* in_t, int r0, int px, int kh, bf16x4 (&rv)[2][4]) {
     const __bf16* res = in_t + ((r0 + 2) * RB_IW + px + 2) * RB_PS + 4 * kh;
 #pragma unroll
@@ -1034,12 +1056,11 @@ __global__ __launch_bounds__(512, 1) void resblock_pipe(ResArgs a) {
     const int ntiles = tiles_x * tiles_y * a.B;
     int gstep = (int)gridDim.x;
     asm volatile("" : "+s"(gstep));
-    const int g0 = blockIdx.x;
-    // this workgroup's tiles: g0 + k gstep, k < nmy
-    const int nmy = g0 < ntiles ? (ntiles - 1 - g0) / gstep + 1 : 0;
+    const RbSched sched(blockIdx.x, gstep, ntiles);
+    const int nmy = sched.count();
     const unsigned img_bytes = (unsigned)a.H * (unsigned)a.W * 64u;  // < 2^31 (checked by the launcher)
     auto tile_of = [&](int k, int& x0, int& y0, int& b) {
-        const int tile = g0 + k * gstep;
+        const int tile = sched.tile(k);
         x0 = (tile % tiles_x) * RB_TW;
         y0 = ((tile / tiles_x) % tiles_y) * RB_TH;
         b = tile / (tiles_x * tiles_y);
@@ -1049,8 +1070,12 @@ __global__ __launch_bounds__(512, 1) void resblock_pipe(ResArgs a) {
         bf16x8 Bf[KSTEPS];
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) Bf[s] = a.w1[s * 64 + lane];
-        bf16x8 pre[RB_NPRE];
-        auto fetch = [&](int k) {
+        // two tiles' input in flight: tile t is prefetched into pa (t even)
+        // or pb (t odd) two tiles before it is staged (one tile ahead, the
+        // staging waited on memory latency: the block without any MFMA
+        // took 37 of its ~60 us)
+        bf16x8 pa[RB_NPRE], pb[RB_NPRE];
+        auto fetch = [&](int k, bf16x8 (&pre)[RB_NPRE]) {
             int x0, y0, b;
             tile_of(k, x0, y0, b);
             const __amdgpu_buffer_rsrc_t rs = oph_buf_rsrc(a.h + (size_t)b * a.H * a.W * 32, img_bytes);
@@ -1064,7 +1089,7 @@ __global__ __launch_bounds__(512, 1) void resblock_pipe(ResArgs a) {
                 pre[n] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
             }
         };
-        auto stage = [&](int buf) {
+        auto stage = [&](int buf, const bf16x8 (&pre)[RB_NPRE]) {
 #pragma unroll
             for (int n = 0; n < RB_NPRE; ++n) {
                 const int i = threadIdx.x + 256 * n;
@@ -1086,19 +1111,27 @@ __global__ __launch_bounds__(512, 1) void resblock_pipe(ResArgs a) {
             if (wave_o == 3) rb_conv1_strip(it, mt, Bf, bias_s, px, kh, x0, y0, a.H, a.W);
         };
         if (nmy > 0) {
-            fetch(0);
-            stage(0);
+            fetch(0, pa);
+            stage(0, pa);
         }
-        if (nmy > 1) fetch(1);
+        if (nmy > 1) fetch(1, pb);
+        if (nmy > 2) fetch(2, pa);
         __syncthreads();
         if (nmy > 0) conv1(0);
-        if (nmy > 1) stage(1);
-        if (nmy > 2) fetch(2);
+        if (nmy > 1) stage(1, pb);
+        if (nmy > 3) fetch(3, pb);
         __syncthreads();
-        for (int k = 0; k < nmy; ++k) {
+        // two tiles per trip (the prefetch arrays alternate); one barrier
+        // per tile, nmy in all, as group B
+        for (int k = 0; k < nmy; k += 2) {
             if (k + 1 < nmy) conv1(k + 1);
-            if (k + 2 < nmy) stage(k & 1);  // tile k + 2 into the buffer tile k used
-            if (k + 3 < nmy) fetch(k + 3);
+            if (k + 2 < nmy) stage(0, pa);  // tile k + 2 into the buffer tile k used
+            if (k + 4 < nmy) fetch(k + 4, pa);
+            __syncthreads();
+            if (k + 1 >= nmy) break;
+            if (k + 2 < nmy) conv1(k + 2);
+            if (k + 3 < nmy) stage(1, pb);
+            if (k + 5 < nmy) fetch(k + 5, pb);
             __syncthreads();
         }
     } else {
