@@ -62,12 +62,12 @@ def test_depthmap_layer_range_and_empty(ctx):
     assert np.all(z == 0)
 
 
-def test_depthmap_config3_full_size(ctx):
+def test_depthmap_rgb_1920x1080_full_size(ctx):
     """The RGB workload at full size: 1920x1080 scene, 4096x2048 grid, 256
     layers x 3 colours.  The oracle cannot run all 768 layers in seconds, so
     (a) four populated layers are compared with the oracle at full size and
     (b) linearity over a split of the layer range holds at full size."""
-    print("case: depthmap config3 full", flush=True)
+    print("case: depthmap RGB 1920x1080 on 4096x2048 full", flush=True)
     img, dep = scenes.depth_scene(1920, 1080, seed=3)
     lams = (638e-9, 532e-9, 473e-9)
     args = (1920, 1080, 4096, 2048, 8e-6, 0.09, 0.11, 256, lams, img, dep)
@@ -128,7 +128,7 @@ def test_depthmap_shipped_config_11520x2160(ctx):
     w, h = c.pn[0], c.pn[1]
     assert (w, h) == (11520, 2160)
     lam = c.lambda_
-    print("case: depthmap shipped config 3-level", flush=True)
+    print("case: depthmap shipped configuration, 3 levels", flush=True)
     img, _ = scenes.depth_scene(w, h, seed=21, colors=1, gaussians=2)
     dep = np.full((h, w), 40, np.uint8)
     dep[:, w // 3:2 * w // 3] = 128

@@ -202,17 +202,17 @@ def test_wrp_hologram_small(ctx):
     assert err < 1e-5
 
 
-def test_wrp_hologram_config2_full(ctx):
+def test_wrp_hologram_100k_2048sq_full(ctx):
     """The full WRP workload: 100,000 points, 2048^2 WRP, 100 mm ASM on a
     4096^2 padded grid, complex64 out; rel-L2 vs the FP64 oracle <= 1e-5
     (the plain path's bar)."""
-    print("case: wrp_hologram config2 full", flush=True)
+    print("case: wrp_hologram 100k points 2048^2 full", flush=True)
     x, y, z, a, zw = scenes.wrp_points(100_000, seed=2)
     out = ctx.wrp_hologram(2048, 8e-6, 532e-9, zw, -zw, x, y, z, a, pad=2)
     w = O.wrp(2048, 2048, 8e-6, 532e-9, zw, x, y, z, a)
     ref = O.asm(w, 8e-6, 8e-6, 532e-9, -zw, pad=2)
     err = rel_l2(out, ref)
-    print("rel-L2 config2", err)
+    print("rel-L2 wrp_hologram 100k", err)
     assert err < 1e-5
 
 

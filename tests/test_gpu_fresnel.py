@@ -87,21 +87,21 @@ def test_fresnel_edge_cases(ctx):
     assert np.allclose(np.abs(U), 0.7, atol=1e-4)
 
 
-# config-4 bars: the plain path at the north-star 1e-5; the MFMA forms at the
-# 1e-3 contraction target (FP16) or their factor-rounding level.  The points
+# Bars for 1e6 points on a 4096^2 plane: the plain path at 1e-5; the MFMA
+# forms at the 1e-3 contraction target (FP16) or their factor-rounding level.  The points
 # lie on 64 planes, so the plane-factored form runs: its per-plane partial
 # sums keep the FP32 accumulation error of the 1e6-term sum small (the split
 # forms measure ~2e-6..4e-6; BF16 unsplit ~2e-3).  The general form on the
 # same points accumulates all 1e6 terms in one FP32 sum (split forms
-# ~6e-5..8e-5): CONFIG4_GENERAL_BOUND.
-CONFIG4_BOUND = {"valu": 1e-5, "bf16": 6e-3, "bf16_split": 2e-5, "fp16": 1e-3, "fp16_split": 1e-5}
-CONFIG4_GENERAL_BOUND = {"fp16": 1e-3, "fp16_split": 2e-4}
+# ~6e-5..8e-5): FR1E6_GENERAL_BOUND.
+FR1E6_BOUND = {"valu": 1e-5, "bf16": 6e-3, "bf16_split": 2e-5, "fp16": 1e-3, "fp16_split": 1e-5}
+FR1E6_GENERAL_BOUND = {"fp16": 1e-3, "fp16_split": 2e-4}
 
 
-def test_fresnel_config4_sampled(ctx):
+def test_fresnel_1e6pts_4096sq_sampled(ctx):
     """1,000,000 points in 64 planes, 4096^2 at 8 um: the VALU kernel and
     every MFMA precision against the FP64 oracle on 4096 sampled pixels."""
-    print("case: fresnel config4 sampled", flush=True)
+    print("case: fresnel 1e6 points 4096^2 sampled", flush=True)
     x, y, p, z, a, ph = scenes.fresnel_points(1_000_000, planes=64, seed=4)
     rng = np.random.default_rng(9)
     rows, cols = rng.integers(0, 4096, (2, 4096))
@@ -112,8 +112,8 @@ def test_fresnel_config4_sampled(ctx):
         ctx.fresnel(4096, 4096, 8e-6, 532e-9, *d, method=m, out=out, device=True)
         U = out.numpy()[rows, cols]
         err = rel_l2(U, ref)
-        print(m, "config4 sampled rel-L2", err, flush=True)
-        assert err < CONFIG4_BOUND[m]
+        print(m, "1e6 points sampled rel-L2", err, flush=True)
+        assert err < FR1E6_BOUND[m]
     # the general contraction on the same points (the plane-factored form is
     # the default here: 64 planes of ~15,600 points)
     import os
@@ -122,7 +122,7 @@ def test_fresnel_config4_sampled(ctx):
         for m in ("fp16", "fp16_split"):
             ctx.fresnel(4096, 4096, 8e-6, 532e-9, *d, method=m, out=out, device=True)
             err = rel_l2(out.numpy()[rows, cols], ref)
-            print(m, "general form config4 sampled rel-L2", err, flush=True)
-            assert err < CONFIG4_GENERAL_BOUND[m]
+            print(m, "general form 1e6 points sampled rel-L2", err, flush=True)
+            assert err < FR1E6_GENERAL_BOUND[m]
     finally:
         del os.environ["OPH_FRESNEL_PLANES"]

@@ -19,9 +19,9 @@ def _s64(spec):
                 tilt_deg=spec.get("tilt_deg", (0, 0)))
 
 
-def test_rs_config1_full_size(ctx):
+def test_rs_1000pts_1024sq_full_size(ctx):
     """1000 seeded points, 1024^2 at 8 um, 532 nm, complex128 out."""
-    print("case: rs config1 1024x1024 P=1000", flush=True)
+    print("case: rs 1024x1024 P=1000", flush=True)
     x, y, z, a, ph = scenes.rs_points(1000, seed=1)
     spec = dict(scenes.RS_SPEC)
     U = ctx.pc_rs(spec, x, y, z, a, ph)
