@@ -495,17 +495,18 @@ def bench_learned(ctx, args):
     a2 = argparse.Namespace(**vars(args))
     a2.steps = max(5, args.steps // 2)
     w_full, per_full = _time_steps(ctx, full_step, a2, DM_KERNELS)
-    w_l, per_l = _time_steps(ctx, learned_step, a2, DM_KERNELS + ["resblock_pipe", "tail_fwd"])
+    w_l, per_l = _time_steps(ctx, learned_step, a2, DM_KERNELS + ["resblock_fwd", "resblock_pipe", "tail_fwd"])
     hz_full, hz_l = a2.steps / w_full, a2.steps / w_l
     passed = ev[:, 3] >= 35.0
     # with every frame audited, each frame pays the learned path plus the
     # 256-layer check (and its 6 reconstructions); a failing frame returns
     # the check's own 256-layer field
     conv_flops = 2.0 * n * n * 9 * (2 * 32 + 16 * 32 * 32 + 32 * 2)
-    # convolution time per frame: the fused residual blocks (resblock_pipe,
-    # BF16 activations; the first computes the head) and the tail (tail_fwd)
+    # convolution time per frame: the fused residual blocks (resblock_fwd for
+    # the first, which also computes the head; resblock_pipe for the rest;
+    # BF16 activations) and the tail (tail_fwd)
     conv_us = sum(per_l[k]["launches_per_step"] * per_l[k]["avg_us"]
-                  for k in ("resblock_pipe", "tail_fwd") if k in per_l)
+                  for k in ("resblock_fwd", "resblock_pipe", "tail_fwd") if k in per_l)
     line = {"value": hz_l, "unit": "holograms/s (16-layer + CNN, no per-frame check)",
             "ms_per_step": 1e3 / hz_l,
             "config": {"workload": "learned_1024sq_grid2048_16_to_256_layers_8x32ch_532nm",

@@ -499,18 +499,20 @@ __global__ void add_inplace(float* a, const float* b, int64_t n) {
 }
 
 // ---- fused residual block (inference, BF16 activations) ----------------------
-// h' = h + conv2(relu(conv1(h))) for C = 32 channels in one kernel, on 8 x 32
-// output tiles: h is staged on the tile plus a two-pixel halo (12 x 36
-// pixels) in LDS, relu(conv1) is computed on the tile plus a one-pixel halo
-// (10 x 34 pixels: ten rows of 32 and a 10 x 2 strip; pixels outside the
-// image are the zero padding of conv2) into a second LDS tile, then conv2 on
-// the 8 rows of 32 with the residual taken from the staged h.  Each wave
-// takes consecutive rows so that one LDS operand serves the taps of up to
-// three output rows.  HBM traffic per block: h read once (plus halo), h'
-// written once.  The MFMAs take the weights as the A operand, so each lane
-// ends with 4 runs of 4 consecutive channels of one pixel (8-byte stores,
-// one bounds check per pixel).  Same fragments, BF16 rounding and FP32
-// accumulation as conv3x3_fwd; the kernel is resblock_pipe below.
+// h' = h + conv2(relu(conv1(h))) for C = 32 channels in one kernel.  A
+// persistent workgroup of 4 waves (conv1's weight fragments in registers,
+// conv2's in LDS, loaded once) walks 8 x 32 output tiles; for each it
+// stages h on the tile plus a two-pixel halo (12 x 36 pixels) in LDS,
+// computes relu(conv1) on the tile plus a one-pixel halo (10 x 34 pixels:
+// ten rows of 32 and a 10 x 2 strip; pixels outside the image are the zero
+// padding of conv2) into a second LDS tile, then conv2 on the 8 rows of 32
+// with the residual read from the staged h.  Each wave takes consecutive
+// rows so that one LDS operand serves the taps of up to three output rows.
+// HBM traffic per block: h read once (plus halo), h' written once.  The
+// MFMAs take the weights as the A
+// operand, so each lane ends with 4 runs of 4 consecutive channels of one
+// pixel (8-byte stores, one bounds check per pixel).  Same fragments,
+// BF16 rounding and FP32 accumulation as conv3x3_fwd.
 constexpr int RB_TH = 8, RB_TW = 32;
 constexpr int RB_IH = RB_TH + 4, RB_IW = RB_TW + 4;   // staged input
 constexpr int RB_MH = RB_TH + 2, RB_MW = RB_TW + 2;   // conv1 output
@@ -539,6 +541,7 @@ struct ResArgs {
 // orders it), BF16-rounded exactly as the separate head layer stores it and
 // zero outside the image.  h never goes to HBM.
 constexpr int RB_XH = RB_IH + 2, RB_XW = RB_IW + 2;    // staged input of the head form
+constexpr int RB_NPX = (RB_XH * RB_XW + 255) / 256;    // its 8-byte pieces per thread (3)
 constexpr int RB_HG = (RB_IH * RB_IW + 31) / 32;       // head groups of 32 pixels (14)
 
 // Software pipelines of the two convolutions, unrolled at compile time: the
@@ -732,6 +735,49 @@ __device__ __forceinline__ void rb_store(const RbOut& o, __amdgpu_buffer_rsrc_t 
         for (int q = 0; q < 4; ++q) __builtin_amdgcn_raw_buffer_store_b64(o.v[j][q], img, o.dst[j] + 16 * q, 0, 0);
 }
 
+// conv2 (weights from LDS) + bias + residual for output rows r0, r0 + 1 of
+// the tile.  Taps are taken row by row (dy outer): the operands of mid row
+// r0 + dy + 1 serve output row r0 + 1 at dy and output row r0 at dy + 1, so
+// each is read once; each weight fragment is read once for both rows.
+__device__ __forceinline__ void rb_conv2_rows(const __bf16* in_t, const __bf16* mid_t, const bf16x8* w2_s,
+                                              const float4* bias_s, RbOut& out, int r0, int lane, int x0, int y0,
+                                              int H, int W) {
+    const int px = lane & 31, kh = lane >> 5;
+    f32x16 acc[2] = {(f32x16)0.f, (f32x16)0.f};
+    const __bf16* src = mid_t + (r0 * RB_MW + px) * RB_PS + 8 * kh;
+    bf16x8 Xr[4][6], Wr[RB_C2_AHEAD + 1];
+    // bias and residual, read under the last MFMAs
+    float4 bq[4];
+    bf16x4 rv[2][4];
+    const __bf16* res = in_t + ((r0 + 2) * RB_IW + px + 2) * RB_PS + 4 * kh;
+    auto late = [&]() {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) bq[q] = bias_s[8 + 2 * q + kh];
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) rv[j][q] = *reinterpret_cast<const bf16x4*>(res + j * RB_IW * RB_PS + 8 * q);
+    };
+    rb_c2_pipe<0, 12>(src, w2_s + lane, Xr, Wr, acc, late);
+    const int x = x0 + px;
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int r = r0 + j;
+        const int y = y0 + r;
+        // pixels outside the image: offset out of range, store dropped
+        out.dst[j] = y < H && x < W ? (unsigned)((y * W + x) * 64 + 8 * kh) : OPH_BUF_OOB;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            bf16x4 o;
+            o[0] = (__bf16)(acc[j][4 * q + 0] + bq[q].x + (float)rv[j][q][0]);
+            o[1] = (__bf16)(acc[j][4 * q + 1] + bq[q].y + (float)rv[j][q][1]);
+            o[2] = (__bf16)(acc[j][4 * q + 2] + bq[q].z + (float)rv[j][q][2]);
+            o[3] = (__bf16)(acc[j][4 * q + 3] + bq[q].w + (float)rv[j][q][3]);
+            out.v[j][q] = __builtin_bit_cast(u32x2, o);
+        }
+    }
+}
+
 // Head convolution of the head form: groups g = wave, wave + 4, ... of 32
 // staged pixels p = 32 g + px (row p / 36, column p % 36 of the 12 x 36
 // region); lane half kh holds k = 16 s + 8 kh + e, i.e. taps 8 s + 4 kh ..
@@ -776,16 +822,145 @@ __device__ __forceinline__ void rb_head(const unsigned* x_t, __bf16* in_t, const
     }
 }
 
+template <bool HEAD>
+__global__ __launch_bounds__(256, 2) void resblock_fwd(ResArgs a) {
+    constexpr int KSTEPS = ksteps_c(32);
+    // staged input plus room for the unused tail of the last 16-byte pieces
+    // (the staging writes are unconditional)
+    __shared__ __attribute__((aligned(16))) __bf16 in_t[(RB_NPRE * 256 / 4) * RB_PS];
+    __shared__ __attribute__((aligned(16))) __bf16 mid_t[RB_MID * RB_PS];
+    __shared__ bf16x8 w2_s[KSTEPS * 64];
+    __shared__ float4 bias_s[24];  // b1, b2, b0 (head form)
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar branches
+    // conv1's fragments in registers, conv2's in LDS, loaded once for every
+    // tile this (persistent) workgroup takes
+    bf16x8 Bf[KSTEPS];
+#pragma unroll
+    for (int s = 0; s < KSTEPS; ++s) Bf[s] = a.w1[s * 64 + lane];
+    for (int i = threadIdx.x; i < KSTEPS * 64; i += 256) w2_s[i] = a.w2[i];
+    if (threadIdx.x < (HEAD ? 24 : 16))
+        bias_s[threadIdx.x] =
+            reinterpret_cast<const float4*>(threadIdx.x < 8 ? a.b1 : threadIdx.x < 16 ? a.b2 : a.b0)[threadIdx.x & 7];
+    bf16x8 B0[2];  // head fragments (head form)
+    if constexpr (HEAD) {
+        B0[0] = a.w0[lane];
+        B0[1] = a.w0[64 + lane];
+    }
+    const int tiles_x = (a.W + RB_TW - 1) / RB_TW, tiles_y = (a.H + RB_TH - 1) / RB_TH;
+    const int ntiles = tiles_x * tiles_y * a.B;
+    // The staged input of the next tile (12 x 36 pixels x 64 bytes, 16-byte
+    // pieces i = threadIdx.x + 256 n) is loaded into registers while the
+    // current tile computes; the epilogues read only LDS, so no wait on
+    // global loads falls inside the convolutions.
+    constexpr int NPRE = HEAD ? 0 : RB_NPRE;
+    constexpr int NPX = HEAD ? RB_NPX : 0;
+    bf16x8 pre[NPRE > 0 ? NPRE : 1];
+    u32x2 prex[NPX > 0 ? NPX : 1];
+    const unsigned img_bytes = (unsigned)a.H * (unsigned)a.W * 64u;  // < 2^31 (checked by the launcher)
+    auto fetch = [&](int tile) {
+        const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
+        const int x0 = tx * RB_TW, y0 = ty * RB_TH;
+        if constexpr (HEAD) {
+            const __amdgpu_buffer_rsrc_t rx = oph_buf_rsrc(a.x + (size_t)b * a.H * a.W * 2, img_bytes / 8);
+#pragma unroll
+            for (int n = 0; n < NPX; ++n) {
+                const int i = threadIdx.x + 256 * n;
+                const int yy = y0 + i / RB_XW - 3, xx = x0 + i % RB_XW - 3;
+                const bool ok = i < RB_XH * RB_XW && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+                const unsigned off = ok ? (unsigned)((yy * a.W + xx) * 8) : OPH_BUF_OOB;
+                prex[n] = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rx, off, 0, 0));
+            }
+            return;
+        }
+        const __amdgpu_buffer_rsrc_t rs = oph_buf_rsrc(a.h + (size_t)b * a.H * a.W * 32, img_bytes);
+#pragma unroll
+        for (int n = 0; n < NPRE; ++n) {
+            const int i = threadIdx.x + 256 * n;
+            const int pix = i >> 2, q = i & 3;
+            const int yy = y0 + pix / RB_IW - 2, xx = x0 + pix % RB_IW - 2;
+            const bool ok = i < RB_IH * RB_IW * 4 && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
+            const unsigned off = ok ? (unsigned)((yy * a.W + xx) * 64 + 16 * q) : OPH_BUF_OOB;
+            pre[n] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
+        }
+    };
+    // A tile's outputs are stored after the next tile's staging and before
+    // its prefetch, so that the prefetch is always the newest memory
+    // operation when the staging waits for it (the wait counter is in order)
+    // and the stores have a whole tile to complete.
+    RbOut pend;
+    __amdgpu_buffer_rsrc_t pend_img = oph_buf_rsrc(a.out, img_bytes);
+    if ((int)blockIdx.x < ntiles) fetch(blockIdx.x);
+    // the grid stride kept in a register (otherwise it is reloaded from the
+    // dispatch packet every tile, with a wait that also drains LDS)
+    int gstep = (int)gridDim.x;
+    asm volatile("" : "+s"(gstep));
+    for (int tile = blockIdx.x; tile < ntiles; tile += gstep) {
+        const int tx = tile % tiles_x, ty = (tile / tiles_x) % tiles_y, b = tile / (tiles_x * tiles_y);
+        const int x0 = tx * RB_TW, y0 = ty * RB_TH;
+        const size_t img = (size_t)b * a.H * a.W;
+        // lane and wave indices through opaque copies: everything derived
+        // from them is recomputed per tile instead of being hoisted out of
+        // the loop into registers held for the kernel's lifetime
+        int lane_o = lane, wave_o = wave;
+        asm volatile("" : "+v"(lane_o), "+s"(wave_o));
+        const int px = lane_o & 31, kh = lane_o >> 5;
+        __syncthreads();  // the previous tile's LDS reads are done
+        // head form: the staged input (BF16 pairs) in the mid tile's space,
+        // which is free until conv1
+        unsigned* x_t = reinterpret_cast<unsigned*>(mid_t);
+        if constexpr (HEAD) {
+#pragma unroll
+            for (int n = 0; n < NPX; ++n) {
+                const int i = threadIdx.x + 256 * n;
+                const f32x2 v = __builtin_bit_cast(f32x2, prex[n]);
+                if (i < RB_XH * RB_XW) x_t[i] = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
+            }
+        } else {
+#pragma unroll
+            for (int n = 0; n < NPRE; ++n) {
+                const int i = threadIdx.x + 256 * n;
+                *reinterpret_cast<bf16x8*>(in_t + (i >> 2) * RB_PS + 8 * (i & 3)) = pre[n];
+            }
+        }
+        __syncthreads();
+        if (tile != (int)blockIdx.x) rb_store(pend, pend_img);
+        if (tile + gstep < ntiles) fetch(tile + gstep);
+        if constexpr (HEAD) {
+            rb_head(x_t, in_t, B0, bias_s + 16, wave_o, px, kh, x0, y0, a.H, a.W);
+            __syncthreads();
+        }
+
+        // conv1 + bias + relu on the 10 x 34 halo tile: rows of 32 (mid
+        // columns 0..31) as 3, 3, 2, 2 consecutive rows per wave, and the
+        // 10 x 2 strip of columns 32, 33 as one flat group on wave 3
+        if (wave_o < 2)
+            rb_conv1_rows<3>(in_t, mid_t, Bf, bias_s, 3 * wave_o, px, kh, x0, y0, a.H, a.W);
+        else
+            rb_conv1_rows<2>(in_t, mid_t, Bf, bias_s, 6 + 2 * (wave_o - 2), px, kh, x0, y0, a.H, a.W);
+        if (wave_o == 3) rb_conv1_strip(in_t, mid_t, Bf, bias_s, px, kh, x0, y0, a.H, a.W);
+        __syncthreads();
+
+        // conv2 + bias + residual on the 8 x 32 tile: rows 2 wave, 2 wave + 1
+        rb_conv2_rows(in_t, mid_t, w2_s, bias_s, pend, 2 * wave_o, lane_o, x0, y0, a.H, a.W);
+        pend_img = oph_buf_rsrc(a.out + img * 32, img_bytes);
+    }
+    if ((int)blockIdx.x < ntiles) rb_store(pend, pend_img);
+}
+
 // ---- pipelined residual block: two wave groups ----------------------------------
-// One persistent workgroup of 8 waves per CU, the work split by phase: waves
-// 0-3 (group A) compute
+// The same block as resblock_fwd, one workgroup of 8 waves per CU with the
+// work split by phase instead of by tile: waves 0-3 (group A) compute
 // relu(conv1) of tile i + 1 into one mid buffer and stage tile i + 2's input
 // (prefetched into registers one tile earlier) into one input buffer, while
 // waves 4-7 (group B) compute conv2 of tile i from the other mid buffer and
 // store it, its residual read into registers one tile earlier; one barrier
 // per tile.  Each group keeps only its own convolution's weight fragments,
-// so both sets live in registers and no weight passes through LDS.  LDS: two
-// staged inputs and two mid tiles (123 KB; +8.5 KB in the head form).
+// so both sets live in registers and no weight passes through LDS (in
+// resblock_fwd conv2's fragments are read from LDS for every tile, a fifth
+// of the block's LDS reads).  LDS: two staged inputs and two mid tiles
+// (123 KB).  The arithmetic (fragments, k order, rounding) is
+// resblock_fwd's.
 constexpr int RP_IN = (RB_NPRE * 256 / 4) * RB_PS;  // bf16 per staged-input buffer
 constexpr int RP_MID = RB_MID * RB_PS;              // bf16 per mid buffer
 
@@ -869,23 +1044,14 @@ __device__ __forceinline__ void rb_conv2_regs(const __bf16* mid_t, const bf16x8 
     }
 }
 
-// Head form (the first block): group A stages the 2-channel input instead of
-// h, each of its waves the whole 14 x 38 tile into its own LDS copy (so no
-// barrier separates the staging from the head convolution, rb_head, which
-// then writes h of the tile into the input buffer).
-constexpr int RP_NXL = (RB_XH * RB_XW + 63) / 64;  // input pixels per lane, head form (9)
-
-template <bool HEAD>
 __global__ __launch_bounds__(512, 1) void resblock_pipe(ResArgs a) {
     constexpr int KSTEPS = ksteps_c(32);
     __shared__ __attribute__((aligned(16))) __bf16 in_t[2 * RP_IN];
     __shared__ __attribute__((aligned(16))) __bf16 mid_t[2 * RP_MID];
-    __shared__ unsigned x_s[HEAD ? 4 * RB_XH * RB_XW : 1];  // head form: each A wave's staged input
-    __shared__ float4 bias_s[16];  // b1 (conv1's epilogue reads it from LDS), b0 (head form)
+    __shared__ float4 bias_s[8];  // b1 (conv1's epilogue reads it from LDS)
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (threadIdx.x < (HEAD ? 16 : 8))
-        bias_s[threadIdx.x] = reinterpret_cast<const float4*>(threadIdx.x < 8 ? a.b1 : a.b0)[threadIdx.x & 7];
+    if (threadIdx.x < 8) bias_s[threadIdx.x] = reinterpret_cast<const float4*>(a.b1)[threadIdx.x];
     const int tiles_x = (a.W + RB_TW - 1) / RB_TW, tiles_y = (a.H + RB_TH - 1) / RB_TH;
     const int ntiles = tiles_x * tiles_y * a.B;
     int gstep = (int)gridDim.x;
@@ -900,7 +1066,7 @@ __global__ __launch_bounds__(512, 1) void resblock_pipe(ResArgs a) {
         b = tile / (tiles_x * tiles_y);
     };
     if (wave < 4) {
-        // group A: conv1 and the staging (threads 0..255)
+        // group A: conv1 and the staging (threads 0..255, as in resblock_fwd)
         bf16x8 Bf[KSTEPS];
 #pragma unroll
         for (int s = 0; s < KSTEPS; ++s) Bf[s] = a.w1[s * 64 + lane];
@@ -908,48 +1074,13 @@ __global__ __launch_bounds__(512, 1) void resblock_pipe(ResArgs a) {
         // or pb (t odd) two tiles before it is staged (one tile ahead, the
         // staging waited on memory latency: the block without any MFMA
         // took 37 of its ~60 us)
-        constexpr int NP = HEAD ? 1 : RB_NPRE;
-        constexpr int NX = HEAD ? RP_NXL : 1;
-        bf16x8 pa[NP], pb[NP];
-        u32x2 xa[NX], xb[NX];
-        bf16x8 B0[2];
-        if constexpr (HEAD) {
-            B0[0] = a.w0[lane];
-            B0[1] = a.w0[64 + lane];
-        }
-        auto fetch_x = [&](int k, u32x2 (&px)[NX]) {
-            int x0, y0, b;
-            tile_of(k, x0, y0, b);
-            const __amdgpu_buffer_rsrc_t rx = oph_buf_rsrc(a.x + (size_t)b * a.H * a.W * 2, img_bytes / 8);
-#pragma unroll
-            for (int m = 0; m < NX; ++m) {
-                const int i = lane + 64 * m;
-                const int yy = y0 + i / RB_XW - 3, xx = x0 + i % RB_XW - 3;
-                const bool ok = i < RB_XH * RB_XW && yy >= 0 && yy < a.H && xx >= 0 && xx < a.W;
-                const unsigned off = ok ? (unsigned)((yy * a.W + xx) * 8) : OPH_BUF_OOB;
-                px[m] = __builtin_bit_cast(u32x2, __builtin_amdgcn_raw_buffer_load_b64(rx, off, 0, 0));
-            }
-        };
-        auto stage_x = [&](int k, int buf, const u32x2 (&px)[NX]) {
-            int x0, y0, b;
-            tile_of(k, x0, y0, b);
-            unsigned* xt = x_s + wave * (RB_XH * RB_XW);
-#pragma unroll
-            for (int m = 0; m < NX; ++m) {
-                const int i = lane + 64 * m;
-                const f32x2 v = __builtin_bit_cast(f32x2, px[m]);
-                if (i < RB_XH * RB_XW) xt[i] = __builtin_bit_cast(unsigned, __builtin_convertvector(v, bf16x2));
-            }
-            int lane_o = lane;
-            asm volatile("" : "+v"(lane_o));
-            rb_head(xt, in_t + buf * RP_IN, B0, bias_s + 8, wave, lane_o & 31, lane_o >> 5, x0, y0, a.H, a.W);
-        };
-        auto fetch = [&](int k, bf16x8 (&pre)[NP]) {
+        bf16x8 pa[RB_NPRE], pb[RB_NPRE];
+        auto fetch = [&](int k, bf16x8 (&pre)[RB_NPRE]) {
             int x0, y0, b;
             tile_of(k, x0, y0, b);
             const __amdgpu_buffer_rsrc_t rs = oph_buf_rsrc(a.h + (size_t)b * a.H * a.W * 32, img_bytes);
 #pragma unroll
-            for (int n = 0; n < NP; ++n) {
+            for (int n = 0; n < RB_NPRE; ++n) {
                 const int i = threadIdx.x + 256 * n;
                 const int pix = i >> 2, q = i & 3;
                 const int yy = y0 + pix / RB_IW - 2, xx = x0 + pix % RB_IW - 2;
@@ -958,9 +1089,9 @@ __global__ __launch_bounds__(512, 1) void resblock_pipe(ResArgs a) {
                 pre[n] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
             }
         };
-        auto stage = [&](int buf, const bf16x8 (&pre)[NP]) {
+        auto stage = [&](int buf, const bf16x8 (&pre)[RB_NPRE]) {
 #pragma unroll
-            for (int n = 0; n < NP; ++n) {
+            for (int n = 0; n < RB_NPRE; ++n) {
                 const int i = threadIdx.x + 256 * n;
                 *reinterpret_cast<bf16x8*>(in_t + buf * RP_IN + (i >> 2) * RB_PS + 8 * (i & 3)) = pre[n];
             }
@@ -979,44 +1110,28 @@ __global__ __launch_bounds__(512, 1) void resblock_pipe(ResArgs a) {
                 rb_conv1_rows<2>(it, mt, Bf, bias_s, 6 + 2 * (wave_o - 2), px, kh, x0, y0, a.H, a.W);
             if (wave_o == 3) rb_conv1_strip(it, mt, Bf, bias_s, px, kh, x0, y0, a.H, a.W);
         };
-        // tile t's input: fetched into the t-parity arrays, staged into
-        // input buffer t & 1 (through the head convolution in the head form)
-        auto fetch_t = [&](int t) {
-            if constexpr (HEAD) {
-                if (t & 1) fetch_x(t, xb); else fetch_x(t, xa);
-            } else {
-                if (t & 1) fetch(t, pb); else fetch(t, pa);
-            }
-        };
-        auto stage_t = [&](int t) {
-            if constexpr (HEAD) {
-                if (t & 1) stage_x(t, 1, xb); else stage_x(t, 0, xa);
-            } else {
-                if (t & 1) stage(1, pb); else stage(0, pa);
-            }
-        };
         if (nmy > 0) {
-            fetch_t(0);
-            stage_t(0);
+            fetch(0, pa);
+            stage(0, pa);
         }
-        if (nmy > 1) fetch_t(1);
-        if (nmy > 2) fetch_t(2);
+        if (nmy > 1) fetch(1, pb);
+        if (nmy > 2) fetch(2, pa);
         __syncthreads();
         if (nmy > 0) conv1(0);
-        if (nmy > 1) stage_t(1);
-        if (nmy > 3) fetch_t(3);
+        if (nmy > 1) stage(1, pb);
+        if (nmy > 3) fetch(3, pb);
         __syncthreads();
-        // two tiles per trip (the prefetch arrays alternate with the tile's
-        // parity); one barrier per tile, nmy in all, as group B
+        // two tiles per trip (the prefetch arrays alternate); one barrier
+        // per tile, nmy in all, as group B
         for (int k = 0; k < nmy; k += 2) {
             if (k + 1 < nmy) conv1(k + 1);
-            if (k + 2 < nmy) stage_t(k + 2);  // into the buffer tile k used
-            if (k + 4 < nmy) fetch_t(k + 4);
+            if (k + 2 < nmy) stage(0, pa);  // tile k + 2 into the buffer tile k used
+            if (k + 4 < nmy) fetch(k + 4, pa);
             __syncthreads();
             if (k + 1 >= nmy) break;
             if (k + 2 < nmy) conv1(k + 2);
-            if (k + 3 < nmy) stage_t(k + 3);
-            if (k + 5 < nmy) fetch_t(k + 5);
+            if (k + 3 < nmy) stage(1, pb);
+            if (k + 5 < nmy) fetch(k + 5, pb);
             __syncthreads();
         }
     } else {
@@ -1093,9 +1208,8 @@ __global__ __launch_bounds__(256, 2) void tail_fwd(TailArgs a) {
             pre[n] = __builtin_bit_cast(bf16x8, __builtin_amdgcn_raw_buffer_load_b128(rs, off, 0, 0));
         }
     };
-    // outputs of the previous tile, stored after the next staging, so that
-    // the prefetch stays the newest memory operation when the staging waits
-    // for it (the wait counter is in order)
+    // outputs of the previous tile, stored after the next staging (as in
+    // resblock_fwd: the prefetch stays the newest memory operation)
     u32x2 pv[2] = {{0u, 0u}, {0u, 0u}};
     unsigned pdst[2] = {OPH_BUF_OOB, OPH_BUF_OOB};
     __amdgpu_buffer_rsrc_t py = oph_buf_rsrc(a.y, xy_bytes);
@@ -1321,9 +1435,9 @@ int forward_bf16(oph_ctx* ctx, oph_cnn* net, const float* x, float* y, int B, in
         const ResArgs r{h(b), h(b + 1), WF + F.conv1(b), WF + F.conv2(b), P + L.b1(b), P + L.b2(b), B, H, W,
                         x, WF + F.head(), P + L.head_b()};
         if (b == 0)
-            OPH_LAUNCH(ctx, "resblock_pipe", resblock_pipe<true>, dim3(pgrid), dim3(512), 0, r);
+            OPH_LAUNCH(ctx, "resblock_fwd", resblock_fwd<true>, dim3(grid), dim3(256), 0, r);
         else
-            OPH_LAUNCH(ctx, "resblock_pipe", resblock_pipe<false>, dim3(pgrid), dim3(512), 0, r);
+            OPH_LAUNCH(ctx, "resblock_pipe", resblock_pipe, dim3(pgrid), dim3(512), 0, r);
     }
     // three workgroups per CU fit (49 KB of LDS, 142 registers)
     const TailArgs t{h(net->blocks), x, y, WF + F.tail(), P + L.tail_b(), B, H, W};
